@@ -1,0 +1,128 @@
+/*
+ * vkdt_hip.h -- the C ABI between vkdt's host-side graph runtime and the
+ * hand-written HIP kernels of the raw-development path on MI355X (gfx950).
+ *
+ * Plain C, plain pointers and sizes.  The host side (libvkdt_pipe.so and the
+ * module DSOs) never sees a HIP or torch type: streams travel as void*.
+ *
+ * In the reference this boundary is the Vulkan back end of the graph
+ * runtime; each entry point below names the reference interface it takes
+ * over.  Errors map onto the three outcomes the reference distinguishes
+ * (modules/api.h VkResult use, graph.c:776-793):
+ *   VKDT_HIP_OK          success
+ *   VKDT_HIP_INCOMPLETE  bad node/graph configuration (unknown kernel,
+ *                        wrong connector count, format or extent)
+ *   VKDT_HIP_OOM         device allocation failed
+ *   VKDT_HIP_DEVICE      any other HIP runtime error
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VKDT_HIP_ABI_VERSION 1
+
+enum
+{
+  VKDT_HIP_OK         =  0,
+  VKDT_HIP_INCOMPLETE = -1,
+  VKDT_HIP_OOM        = -2,
+  VKDT_HIP_DEVICE     = -3,
+};
+
+// buffer formats, as the connector "format" tokens of the reference
+// (graph-run-nodes-allocate.h:39-96): ui16 = R16_UNORM, f16 = R16F/RGBA16F,
+// f32 = R32F/RGBA32F.
+enum
+{
+  VKDT_FMT_UI16 = 1,
+  VKDT_FMT_F16  = 2,
+  VKDT_FMT_F32  = 3,
+};
+
+// one connector's device buffer.  Replaces the descriptor-set image
+// binding of a node connector (graph-run-nodes-allocate.h:628-940).
+// Texels are stored row major, channels interleaved, rows `pitch` texels
+// apart, planes of an array connector `plane_stride` bytes apart.
+typedef struct vkdt_buf_t
+{
+  void     *ptr;           // device address (NULL: connector unused/dead store)
+  uint32_t  wd, ht;        // extent in texels (imageSize/textureSize)
+  uint32_t  pitch;         // texels per row (>= wd)
+  uint32_t  chan;          // 1 or 4
+  uint32_t  format;        // VKDT_FMT_*
+  uint32_t  array_length;  // number of planes (1 for plain images)
+  uint64_t  plane_stride;  // bytes between planes
+}
+vkdt_buf_t;
+
+// ABI version of the loaded library (for loaders that dlsym by name).
+int vkdt_hip_abi_version(void);
+
+// Select device `dev` and initialise the kernel registry.  Takes over
+// qvk_init (qvk/qvk.c:138) for the compute path: no instance, queues or
+// samplers, only the device.
+int vkdt_hip_init(int dev);
+
+// Run one node.  Takes over one vkCmdDispatch of
+// graph-run-nodes-record-cmd.h:372-380 (bind pipeline, push constants,
+// dispatch ceil(wd/8) x ceil(ht/8) x dp): the (module, kernel) token pair
+// picks the HIP kernel that replaces src/pipe/modules/<module>/<kernel>.comp,
+// `pc` is the push-constant blob dt_node_add stored (modules/api.h:160-210),
+// `params` the module's uniform block (committed_param if the module has
+// one, else its params blob: graph-run-modules.h:6-31), and `conn` the
+// node's connectors in declaration order.  Asynchronous on `stream`
+// (a hipStream_t, NULL = default stream).
+int vkdt_hip_node_launch(
+    uint64_t          module,
+    uint64_t          kernel,
+    const void       *pc,
+    size_t            pc_size,
+    const void       *params,
+    size_t            params_size,
+    const vkdt_buf_t *conn,
+    int               nconn,
+    uint32_t          wd,
+    uint32_t          ht,
+    uint32_t          dp,
+    void             *stream);
+
+// 1 if (module, kernel) has a HIP implementation, else 0.
+int vkdt_hip_node_supported(uint64_t module, uint64_t kernel);
+
+// Device memory for the graph's arena.  Takes over the vkAllocateMemory
+// of the image pool (graph-run-nodes-allocate.h:17-37, alloc.c).
+int vkdt_hip_alloc(void **ptr, size_t bytes);
+int vkdt_hip_free(void *ptr);
+
+// Pinned host staging.  Takes over the mapped staging buffers that
+// read_source fills and write_sink reads (graph-run-nodes-upload.h:23-80,
+// graph-run-nodes-download.h:3-32).
+int vkdt_hip_host_alloc(void **ptr, size_t bytes);
+int vkdt_hip_host_free(void *ptr);
+
+// Asynchronous copies on `stream`.  Take over the staging<->image copies
+// recorded in graph-run-nodes-record-cmd.h:255-320.
+int vkdt_hip_upload(void *dst_dev, const void *src_host, size_t bytes, void *stream);
+int vkdt_hip_download(void *dst_host, const void *src_dev, size_t bytes, void *stream);
+
+// Streams and timing.  Take over the command buffer submit + timeline
+// semaphore wait (graph.c:842-874) and the timestamp queries of -d perf
+// (graph.c:96-112, 881-932).
+int vkdt_hip_stream_create(void **stream);
+int vkdt_hip_stream_destroy(void *stream);
+int vkdt_hip_stream_sync(void *stream);
+int vkdt_hip_event_create(void **event);
+int vkdt_hip_event_destroy(void *event);
+int vkdt_hip_event_record(void *event, void *stream);
+int vkdt_hip_event_elapsed_ms(float *ms, void *ev_begin, void *ev_end);
+
+// Human readable text of the last error on this thread.
+const char *vkdt_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,1236 @@
+/*
+ * oracle.c -- CPU restatement of vkdt's raw-development node kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Each kernel below restates the
+ * GLSL compute shader named in its header comment; the sampler semantics of
+ * the reference (qvk/qvk.c:596-611: LINEAR filter, MIRRORED_REPEAT address
+ * mode, normalised coordinates; R16_UNORM for ui16 images,
+ * graph-run-nodes-allocate.h:39-96) are emulated explicitly:
+ *   - texture() at texture-space coordinate s: taps floor(s-0.5) and +1,
+ *     weight frac(s-0.5), indices mirrored into [0,n)
+ *   - texelFetch() out of bounds returns 0 (undefined in the reference,
+ *     robustness is off: qvk/qvk.c:61,398-399)
+ *   - single channel images read as (r, 0, 0, 1)
+ *   - ui16 reads as the float32 value v/65535
+ *   - f16 stores round to binary16 (RNE) when storage mode is on
+ *   - clamp(x,a,b) = fmin(fmax(x,a),b): a NaN x yields a (SPIR-V FClamp as
+ *     lowered by LLVM returns the non-NaN operand)
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef ORACLE_F32
+typedef float real;
+#define EXP   expf
+#define EXP2  exp2f
+#define POW   powf
+#define SQRT  sqrtf
+#define FABS  fabsf
+#define FMIN  fminf
+#define FMAX  fmaxf
+#else
+typedef double real;
+#define EXP   exp
+#define EXP2  exp2
+#define POW   pow
+#define SQRT  sqrt
+#define FABS  fabs
+#define FMIN  fmin
+#define FMAX  fmax
+#endif
+#define R(x) ((real)(x))
+
+static int g_storage_f16 = 1;
+void or_set_storage_f16(int on) { g_storage_f16 = on; }
+int or_real_bytes(void) { return (int)sizeof(real) * 2; }
+
+/* ------------------------------------------------------------------------
+ * scalar helpers
+ * --------------------------------------------------------------------- */
+
+double or_f16_round(double x)
+{
+  if(isnan(x)) return x;
+  const double ax = fabs(x);
+  if(ax >= 65520.0) return copysign(INFINITY, x);
+  if(ax == 0.0) return x;
+  int e;
+  frexp(ax, &e);              // ax = m 2^e, m in [0.5,1)
+  int qe = e - 11;            // 11 significant bits in binary16
+  if(qe < -24) qe = -24;      // subnormal spacing
+  const double q = ldexp(1.0, qe);
+  return copysign(nearbyint(ax / q) * q, x);
+}
+
+static inline real clampr(real x, real a, real b) { return FMIN(FMAX(x, a), b); }
+static inline real mixr(real a, real b, real t) { return a * (R(1.0) - t) + b * t; }
+static inline real signr(real x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
+static inline real smoothstepr(real e0, real e1, real x)
+{
+  real t = clampr((x - e0) / (e1 - e0), R(0), R(1));
+  return t * t * (R(3) - R(2) * t);
+}
+
+int or_mirror(int i, int n)
+{ // MIRRORED_REPEAT: (n-1) - mirror((i mod 2n) - n), mirror(a) = a>=0 ? a : -(1+a)
+  int t = i % (2 * n);
+  if(t < 0) t += 2 * n;
+  t -= n;
+  t = t >= 0 ? t : -(1 + t);
+  return n - 1 - t;
+}
+
+typedef struct { real x, y, z, w; } v4;
+
+static inline double load_raw(const or_buf_t *b, int x, int y, int pl, int c)
+{ // one channel of one texel, no bounds handling
+  const size_t idx = ((size_t)pl * b->ht + y) * b->wd + x;
+  if(b->format == OR_UI16)
+  {
+    if(c > 0) return c == 3 ? 1.0 : 0.0;
+    const uint16_t v = ((const uint16_t *)b->data)[idx];
+    return (double)(float)(v / 65535.0);   // UNORM16 -> float32
+  }
+  if(b->chan == 1)
+  {
+    if(c > 0) return c == 3 ? 1.0 : 0.0;
+    return ((const double *)b->data)[idx];
+  }
+  return ((const double *)b->data)[idx * b->chan + c];
+}
+
+static inline v4 texel(const or_buf_t *b, int x, int y, int pl)
+{
+  v4 r = { R(load_raw(b, x, y, pl, 0)), R(load_raw(b, x, y, pl, 1)),
+           R(load_raw(b, x, y, pl, 2)), R(load_raw(b, x, y, pl, 3)) };
+  return r;
+}
+
+// texelFetch: out of bounds reads zero
+static inline v4 fetch(const or_buf_t *b, int x, int y, int pl)
+{
+  if(x < 0 || y < 0 || x >= b->wd || y >= b->ht) { v4 z = {0, 0, 0, 0}; return z; }
+  return texel(b, x, y, pl);
+}
+
+// texture() exactly at the centre of texel (x,y): mirrored address
+static inline v4 centre(const or_buf_t *b, int x, int y, int pl)
+{
+  return texel(b, or_mirror(x, b->wd), or_mirror(y, b->ht), pl);
+}
+
+// texture() at texture-space coordinate (s,t) (texel i spans [i,i+1))
+static inline v4 bilin(const or_buf_t *b, real s, real t, int pl)
+{
+  const real fs = s - R(0.5), ft = t - R(0.5);
+  const real fl_s = floor(fs), fl_t = floor(ft);
+  const int i0 = (int)fl_s, j0 = (int)fl_t;
+  const real a = fs - fl_s, c = ft - fl_t;
+  const int x0 = or_mirror(i0, b->wd), x1 = or_mirror(i0 + 1, b->wd);
+  const int y0 = or_mirror(j0, b->ht), y1 = or_mirror(j0 + 1, b->ht);
+  const v4 p00 = texel(b, x0, y0, pl), p10 = texel(b, x1, y0, pl);
+  const v4 p01 = texel(b, x0, y1, pl), p11 = texel(b, x1, y1, pl);
+  v4 r;
+  r.x = (p00.x * (R(1) - a) + p10.x * a) * (R(1) - c) + (p01.x * (R(1) - a) + p11.x * a) * c;
+  r.y = (p00.y * (R(1) - a) + p10.y * a) * (R(1) - c) + (p01.y * (R(1) - a) + p11.y * a) * c;
+  r.z = (p00.z * (R(1) - a) + p10.z * a) * (R(1) - c) + (p01.z * (R(1) - a) + p11.z * a) * c;
+  r.w = (p00.w * (R(1) - a) + p10.w * a) * (R(1) - c) + (p01.w * (R(1) - a) + p11.w * a) * c;
+  return r;
+}
+
+static inline double st(const or_buf_t *b, double x)
+{
+  return (b->format == OR_F16 && g_storage_f16) ? or_f16_round(x) : x;
+}
+
+static inline void store(const or_buf_t *b, int x, int y, int pl, v4 v)
+{
+  if(x < 0 || y < 0 || x >= b->wd || y >= b->ht) return;
+  const size_t idx = ((size_t)pl * b->ht + y) * b->wd + x;
+  double *d = (double *)b->data;
+  if(b->chan == 1) { d[idx] = st(b, v.x); return; }
+  d[4 * idx + 0] = st(b, v.x);
+  d[4 * idx + 1] = st(b, v.y);
+  d[4 * idx + 2] = st(b, v.z);
+  d[4 * idx + 3] = st(b, v.w);
+}
+
+/* exposed single-channel helpers for the sampler unit tests */
+double or_bilinear(const double *img, int wd, int ht, double u, double v)
+{
+  or_buf_t b = { (void *)img, wd, ht, 1, OR_F32, 1 };
+  return (double)bilin(&b, R(u), R(v), 0).x;
+}
+
+void or_gather(const uint16_t *img, int wd, int ht, double u, double v, double out[4])
+{ // textureGather(.., 0): x=(i0,j1) y=(i1,j1) z=(i1,j0) w=(i0,j0)
+  or_buf_t b = { (void *)img, wd, ht, 1, OR_UI16, 1 };
+  const int i0 = (int)floor(u - 0.5), j0 = (int)floor(v - 0.5);
+  const int x0 = or_mirror(i0, wd), x1 = or_mirror(i0 + 1, wd);
+  const int y0 = or_mirror(j0, ht), y1 = or_mirror(j0 + 1, ht);
+  out[0] = load_raw(&b, x0, y1, 0, 0);
+  out[1] = load_raw(&b, x1, y1, 0, 0);
+  out[2] = load_raw(&b, x1, y0, 0, 0);
+  out[3] = load_raw(&b, x0, y0, 0, 0);
+}
+
+/* ------------------------------------------------------------------------
+ * shared GLSL helpers (src/pipe/modules/shared.glsl)
+ * --------------------------------------------------------------------- */
+
+// luminance_rec2020, shared.glsl:150-154
+static inline real lum2020(real r, real g, real b)
+{
+  return R(2.62700212e-01) * r + R(6.77998072e-01) * g + R(5.93017165e-02) * b;
+}
+
+// evd2x2, shared.glsl:244-293.  M = {M[0][0], M[0][1], M[1][0], M[1][1]} (column major)
+static void evd2x2(real eval[2], real evec0[2], real evec1[2], const real M[4])
+{
+  const real a = M[0];     // M[0][0]
+  const real b = M[2];     // M[1][0]
+  const real c = M[3];     // M[1][1]
+  const real pHalf = R(-0.5) * (a + c);
+  const real q = a * c - b * b;
+  const real droot = SQRT(pHalf * pHalf - q);
+  eval[0] = -pHalf + droot;
+  eval[1] = -pHalf - droot;
+  const real a0 = a - eval[0], b0 = b, c0 = c - eval[0];
+  const real sl0 = a0 * a0 + b0 * b0;
+  const real sl1 = b0 * b0 + c0 * c0;
+  real sl;
+  if(sl0 > sl1) { evec1[0] = a0; evec1[1] = b0; sl = sl0; }
+  else          { evec1[0] = b0; evec1[1] = c0; sl = sl1; }
+  evec1[0] = (sl == R(0)) ? R(1) : evec1[0];
+  sl = (sl == R(0)) ? R(1) : sl;
+  const real il = R(1) / SQRT(sl);
+  evec1[0] *= il;
+  evec1[1] *= il;
+  evec0[0] = evec1[1];
+  evec0[1] = -evec1[0];
+}
+
+void or_evd2x2(const double M[4], double eval[2], double evec0[2], double evec1[2])
+{
+  real m[4] = { R(M[0]), R(M[1]), R(M[2]), R(M[3]) }, e[2], v0[2], v1[2];
+  evd2x2(e, v0, v1, m);
+  for(int k = 0; k < 2; k++) { eval[k] = e[k]; evec0[k] = v0[k]; evec1[k] = v1[k]; }
+}
+
+static inline real det2(const real M[4]) { return M[0] * M[3] - M[2] * M[1]; }
+
+// noise_sigma, denoise/noise.glsl:1-11 (all three components share one sqrt)
+static inline void noise_sigma(real out[3], real na, real nb, real black, real white,
+                               const float edges[4], real val)
+{
+  const real s = SQRT(na + FMAX(R(0), (val - black) / (white - black)) * nb);
+  for(int k = 0; k < 3; k++)
+    out[k] = clampr(EXP2(R(12) * R(edges[k]) + R(edges[3])) * s, R(1e-3), R(1e3));
+}
+
+// x-trans colour at (x,y) >= 0 in the canonical layout (demosaic/splat.comp:52-68):
+// 1 = green, 2 = blue, 0 = red
+static inline int xtrans_colour(int x, int y)
+{
+  const int blue_top = ((x / 3 + y / 3) & 1) > 0;
+  const int px = x - (x / 3) * 3, py = y - (y / 3) * 3;
+  if(((px + py) & 1) == 0) return 1;
+  if(blue_top ^ (py == 1)) return 2;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * denoise (src/pipe/modules/denoise)
+ * --------------------------------------------------------------------- */
+
+// noop.comp:36-57 (gainmap path not restated: the DNGs carry no OpcodeList2)
+static int dn_noop(const int32_t *pci, const float *pcf, const void *params,
+                   const or_buf_t *c, int wd, int ht)
+{
+  (void)params;
+  const or_buf_t *in = c + 0, *out = c + 1;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 col = fetch(in, x + pci[0], y + pci[1], 0);
+    real r = FMAX(R(0), (col.x - R(pcf[4])) / (R(pcf[8]) - R(pcf[4])));
+    real g = FMAX(R(0), (col.y - R(pcf[5])) / (R(pcf[9]) - R(pcf[5])));
+    real b = FMAX(R(0), (col.z - R(pcf[6])) / (R(pcf[10]) - R(pcf[6])));
+    v4 o = { r, g, b, R(1) };
+    store(out, x, y, 0, o);
+  }
+  (void)wd; (void)ht;
+  return 0;
+}
+
+// half.comp:24-74
+static int dn_half(const int32_t *pci, const float *pcf, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const real white = R(pcf[9]);
+  const int cx = pci[12], cy = pci[13];
+  const uint32_t filters = (uint32_t)pci[16];
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 rgba;
+    if(filters == 9)
+    {
+      const int bx = cx + 3 * x, by = cy + 3 * y;
+      const real c0 = fetch(in, bx + 0, by + 0, 0).x;
+      const real c1 = fetch(in, bx + 0, by + 1, 0).x;
+      const real c2 = fetch(in, bx + 0, by + 2, 0).x;
+      const real c3 = fetch(in, bx + 1, by + 0, 0).x;
+      const real c4 = fetch(in, bx + 1, by + 1, 0).x;
+      const real c5 = fetch(in, bx + 1, by + 2, 0).x;
+      const real c6 = fetch(in, bx + 2, by + 0, 0).x;
+      const real c7 = fetch(in, bx + 2, by + 1, 0).x;
+      const real c8 = fetch(in, bx + 2, by + 2, 0).x;
+      const real colx = (c1 + c7) * R(0.5), coly = (c3 + c5) * R(0.5);
+      if(((x + y + cx + cy) & 1) > 0) { rgba.x = colx; rgba.z = coly; }
+      else                            { rgba.z = colx; rgba.x = coly; }
+      rgba.y = (c0 + c2 + c4 + c6 + c8) * R(1.0) / R(5.0);
+      rgba.w = R(1);
+    }
+    else
+    { // textureGather at texture-space (cx + 2x + 1, cy + 2y + 1)
+      const int i0 = cx + 2 * x, j0 = cy + 2 * y;
+      const int x0 = or_mirror(i0, in->wd), x1 = or_mirror(i0 + 1, in->wd);
+      const int y0 = or_mirror(j0, in->ht), y1 = or_mirror(j0 + 1, in->ht);
+      real gx = texel(in, x0, y1, 0).x;  // (i0,j1)
+      real gy = texel(in, x1, y1, 0).x;  // (i1,j1)
+      real gz = texel(in, x1, y0, 0).x;  // (i1,j0)
+      real gw = texel(in, x0, y0, 0).x;  // (i0,j0)
+      if(gx >= white) gx = gz;
+      if(gz >= white) gz = gx;
+      rgba.x = gw; rgba.y = (gx + gz) / R(2.0); rgba.z = gy; rgba.w = R(1);
+    }
+    store(out, x, y, 0, rgba);
+  }
+  return 0;
+}
+
+// cov.glsl:21-138 response(); p is the unsorted texel position
+static void dn_response(const or_buf_t *img, int px_, int py_, real rgb[3], real cov[4])
+{
+  real mw[2] = {0, 0}, mb[2] = {0, 0}, smw = 0, smb = 0, mean_b = 0;
+  real lum[25];
+  for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+  {
+    v4 t = centre(img, px_ + i, py_ + j, 0);
+    const real px = lum2020(t.x, t.y, t.z);
+    lum[(j + 2) * 5 + i + 2] = px;
+    const real w = R(1);
+    mw[0] += R(i) * px * w; mw[1] += R(j) * px * w;
+    smw += px * w;
+    mb[0] += R(i) / px * w; mb[1] += R(j) / px * w;
+    smb += R(1.0) / px * w;
+  }
+  mw[0] /= smw; mw[1] /= smw;
+  mb[0] /= smb; mb[1] /= smb;
+  real Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0;
+  for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+  {
+    const real px = lum[(j + 2) * 5 + i + 2];
+    const real w = R(1);
+    mean_b += px / R(25.0);
+    real p2 = px * px * w;
+    real dx = R(i) - mw[0], dy = R(j) - mw[1];
+    Sw[0] += p2 * dx * dx;   // [0][0]
+    Sw[1] += p2 * dx * dy;   // [0][1]
+    Sw[2] += p2 * dy * dx;   // [1][0]
+    Sw[3] += p2 * dy * dy;   // [1][1]
+    sw += p2;
+    dx = R(i) - mb[0]; dy = R(j) - mb[1];
+    p2 = R(1.0) / (px * px) * w;
+    Sb[0] += p2 * dx * dx;
+    Sb[1] += p2 * dx * dy;
+    Sb[2] += p2 * dy * dx;
+    Sb[3] += p2 * dy * dy;
+    sb += p2;
+  }
+  for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
+  const real *S = det2(Sw) < det2(Sb) ? Sw : Sb;
+  real eval[2], e0[2], e1[2];
+  evd2x2(eval, e0, e1, S);
+  eval[0] *= R(1.0); eval[1] *= R(0.05);
+  eval[0] = clampr(eval[0], R(0.01), R(25.0));
+  eval[1] = clampr(eval[1], R(0.01), R(25.0));
+  cov[0] = eval[0]; cov[1] = eval[1]; cov[2] = e0[0]; cov[3] = e0[1];
+
+  real res[3] = {0, 0, 0}, wt = 0;
+  for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+  {
+    v4 t = centre(img, px_ + i, py_ + j, 0);
+    if(t.x > R(2.0) * mean_b) continue;   // hot pixel
+    const real x0 = R(i) * e0[0] + R(j) * e0[1];
+    const real x1 = R(i) * e1[0] + R(j) * e1[1];
+    const real w = FMAX(R(1e-9), EXP(R(-0.5) * ((x0 / eval[0]) * x0 + (x1 / eval[1]) * x1)));
+    res[0] += w * t.x; res[1] += w * t.y; res[2] += w * t.z;
+    wt += w;
+  }
+  const real d = FMAX(wt, R(1e-8));
+  rgb[0] = res[0] / d; rgb[1] = res[1] / d; rgb[2] = res[2] / d;
+}
+
+static inline void sorted_pos(int x, int y, int wd, int ht, int *sx, int *sy)
+{ // down.comp:102-104: ipos/2 + (ipos&1)*(size+1)/2
+  *sx = x / 2 + (x & 1) * ((wd + 1) / 2);
+  *sy = y / 2 + (y & 1) * ((ht + 1) / 2);
+}
+
+// downcov.comp:41-63
+static int dn_downcov(const int32_t *pci, const float *pcf, const or_buf_t *c, int nconn)
+{
+  (void)pcf;
+  const or_buf_t *in = c + 0, *out = c + 1, *cv = nconn > 2 ? c + 2 : 0;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real rgb[3], cov[4];
+    dn_response(in, x + pci[12], y + pci[13], rgb, cov);
+    if(cv && cv->data) { v4 cc = { cov[0], cov[1], cov[2], cov[3] }; store(cv, x + pci[12], y + pci[13], 0, cc); }
+    int sx, sy;
+    sorted_pos(x, y, out->wd, out->ht, &sx, &sy);
+    real edge = clampr(R(75.0) * FMAX(R(0), cov[1] - R(0.09)), R(0), R(1));
+    edge = smoothstepr(R(0.4), R(0.75), edge);
+    edge = clampr(R(0.02) + edge, R(0), R(1));
+    v4 o = { rgb[0], rgb[1], rgb[2], edge };
+    store(out, sx, sy, 0, o);
+  }
+  return 0;
+}
+
+static inline real gamma08(real f) { return f < R(0) ? f : POW(f, R(0.8)); }
+
+// down.comp:59-108 (level >= 0 branch)
+static int dn_down(const int32_t *pci, const float *pcf, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const real t = R(0.2);
+  const int cx = pci[12], cy = pci[13];
+  const int level = pci[18];
+  const uint32_t blk = (uint32_t)pci[19];
+  const real block = (blk == 3) ? R(2.23607) : (blk == 2 ? R(1.414213) : R(1.0));
+  static const real off[4][2] = { {1.2, 0.4}, {-1.2, -0.4}, {0.4, -1.2}, {-0.4, 1.2} };
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 c0 = centre(in, x + cx, y + cy, 0);
+    real rgb[3] = { c0.x, c0.y, c0.z };
+    real sum[3] = { t * rgb[0], t * rgb[1], t * rgb[2] };
+    real wgt[3] = { t, t, t };
+    real sig[3];
+    noise_sigma(sig, R(pcf[16]), R(pcf[17]), R(pcf[5]), R(pcf[9]), par + 4, rgb[0]);
+    const real lv = POW(R(0.7), R(level));
+    real wc[3];
+    for(int k = 0; k < 3; k++) { sig[k] = lv * sig[k] / block; wc[k] = R(1.0) / sig[k]; }
+    for(int o = 0; o < 4; o++)
+    {
+      v4 cl = bilin(in, R(x + cx) + R(0.5) + off[o][0], R(y + cy) + R(0.5) + off[o][1], 0);
+      const real col[3] = { cl.x, cl.y, cl.z };
+      for(int k = 0; k < 3; k++)
+      {
+        const real e = clampr(R(1.0) - R(0.5) * (wc[k] * FABS(gamma08(col[k]) - gamma08(rgb[k]))), R(0), R(1));
+        const real w = e * (R(1.0) - t) / R(4.0);
+        sum[k] += w * col[k];
+        wgt[k] += w;
+      }
+    }
+    int sx, sy;
+    sorted_pos(x, y, out->wd, out->ht, &sx, &sy);
+    v4 ov = { sum[0] / wgt[0], sum[1] / wgt[1], sum[2] / wgt[2], R(1) };
+    store(out, sx, sy, 0, ov);
+  }
+  return 0;
+}
+
+// matrices.h: rows of rec709_to_yuv * rec2020_to_rec709 and rec709_to_rec2020 * yuv_to_rec709
+static const double M_709_yuv[9]  = { 0.299, 0.587, 0.114, -0.147, -0.289, 0.436, 0.615, -0.515, -0.100 };
+static const double M_2020_709[9] = { 1.66022677, -0.58754761, -0.07283825, -0.12455334, 1.13292605, -0.00834963, -0.01815514, -0.10060303, 1.11899817 };
+static const double M_709_2020[9] = { 0.62750375, 0.32927542, 0.04330266, 0.06910828, 0.91951916, 0.0113596, 0.01639406, 0.08801125, 0.89538035 };
+static const double M_yuv_709[9]  = { 1.0, -3.94570707e-05, 1.13982797, 1.0, -3.94610164e-01, -5.80500316e-01, 1.0, 2.03199968, -4.81376263e-04 };
+
+static void matmul3(double out[9], const double a[9], const double b[9])
+{
+  for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++)
+  {
+    double s = 0;
+    for(int k = 0; k < 3; k++) s += a[3 * j + k] * b[3 * k + i];
+    out[3 * j + i] = s;
+  }
+}
+
+static inline void mv3(real o[3], const real M[9], const real v[3])
+{
+  for(int j = 0; j < 3; j++) o[j] = M[3 * j + 0] * v[0] + M[3 * j + 1] * v[1] + M[3 * j + 2] * v[2];
+}
+
+// assemble.comp:43-165
+static int dn_assemble(const int32_t *pci, const float *pcf, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *s0 = c + 0, *s1 = c + 1, *s2 = c + 2, *s3 = c + 3, *s4 = c + 4, *out = c + 5;
+  double m1[9], m2[9];
+  matmul3(m1, M_709_yuv, M_2020_709);
+  matmul3(m2, M_709_2020, M_yuv_709);
+  real rgb_to_yuv[9], yuv_to_rgb[9];
+  for(int k = 0; k < 9; k++) { rgb_to_yuv[k] = R(m1[k]); yuv_to_rgb[k] = R(m2[k]); }
+  const uint32_t filters = (uint32_t)pci[18];
+  const real block = filters == 0u ? R(1.0) : (filters == 9u ? R(2.23607) : R(1.414213));
+  const real bl[4] = { R(0.7000) / block, R(0.4900) / block, R(0.3430) / block, R(0.2401) / block };
+  const real strength = R(par[0]), luma = R(par[1]);
+  const real thrs0 = POW(strength, R(4.0));
+  const real wb[3] = { R(pcf[0]), R(pcf[1]), R(pcf[2]) };
+  const real black[3] = { R(pcf[4]), R(pcf[5]), R(pcf[6]) };
+  const real white[3] = { R(pcf[8]), R(pcf[9]), R(pcf[10]) };
+  const int szx = s1->wd + 1, szy = s1->ht + 1;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 o0 = fetch(s0, x + pci[12], y + pci[13], 0);
+    const real orig[3] = { o0.x, o0.y, o0.z };
+    int ex = x, ey = y;
+    ex = ex / 2 + (ex & 1) * (szx / 2); ey = ey / 2 + (ey & 1) * (szy / 2);
+    v4 d1 = fetch(s1, ex, ey, 0);
+    ex = ex / 2 + (ex & 1) * (szx / 2); ey = ey / 2 + (ey & 1) * (szy / 2);
+    v4 d2 = fetch(s2, ex, ey, 0);
+    ex = ex / 2 + (ex & 1) * (szx / 2); ey = ey / 2 + (ey & 1) * (szy / 2);
+    v4 d3 = fetch(s3, ex, ey, 0);
+    ex = ex / 2 + (ex & 1) * (szx / 2); ey = ey / 2 + (ey & 1) * (szy / 2);
+    v4 d4 = fetch(s4, ex, ey, 0);
+    real down[5][3] = {
+      { orig[0], orig[1], orig[2] }, { d1.x, d1.y, d1.z }, { d2.x, d2.y, d2.z },
+      { d3.x, d3.y, d3.z }, { d4.x, d4.y, d4.z } };
+    real sig[3];
+    noise_sigma(sig, R(pcf[16]), R(pcf[17]), R(pcf[5]), R(pcf[9]), par + 4, FMAX(down[2][0], R(0)));
+    real d[4][3];
+    for(int l = 0; l < 4; l++) for(int k = 0; k < 3; k++)
+      d[l][k] = (down[l][k] - down[l + 1][k]) / (sig[k] * bl[l]);
+    real len[4];
+    for(int l = 0; l < 4; l++) len[l] = SQRT(d[l][0] * d[l][0] + d[l][1] * d[l][1] + d[l][2] * d[l][2]);
+    const real slope = ((len[3] - len[0]) / R(3.0) + (len[2] - len[1]) / R(1.0) + (len[1] - len[0]) / R(1.0)
+                      + (len[3] - len[2]) / R(1.0) + (len[2] - len[0]) / R(2.0) + (len[3] - len[1]) / R(2.0)) / R(6.0);
+    real test = FMAX(R(0), -slope);
+    test = FMAX(R(0.0), R(1.0) - test);
+    test = POW(test, R(16.0));
+    test = clampr(R(1.5) * test, R(0), R(1));
+    real d4v[3] = { down[4][0], down[4][1], down[4][2] };
+    for(int l = 3; l >= 0; l--)
+    {
+      const real thrs = FABS(d[l][0]) > R(10.0) ? R(10000.0) : thrs0;
+      for(int k = 0; k < 3; k++)
+      {
+        const real ad = FABS(d[l][k]);
+        const real tt = FMIN(R(1.0), ad / (R(2) * thrs));
+        d4v[k] += sig[k] * bl[l] * signr(d[l][k]) * mixr(FMAX(ad - thrs, R(0.0)), ad, tt);
+      }
+    }
+    real t0[3], t1[3], yuv[3], yuvo[3];
+    for(int k = 0; k < 3; k++)
+    {
+      t0[k] = (d4v[k] - black[k]) / (white[k] - black[k]) * wb[k];
+      t1[k] = (orig[k] - black[k]) / (white[k] - black[k]) * wb[k];
+    }
+    mv3(yuv, rgb_to_yuv, t0);
+    mv3(yuvo, rgb_to_yuv, t1);
+    yuv[0] = mixr(yuvo[0], yuv[0], luma);
+    real rgb[3];
+    mv3(rgb, yuv_to_rgb, yuv);
+    for(int k = 0; k < 3; k++) rgb[k] = rgb[k] / wb[k] * (white[k] - black[k]) + black[k];
+    v4 ov = { rgb[0], rgb[1], rgb[2], test };
+    store(out, x, y, 0, ov);
+  }
+  return 0;
+}
+
+// doub.comp:35-116 (gainmap path not restated)
+static int dn_doub(const int32_t *pci, const float *pcf, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *crs0 = c + 1, *crs1b = c + 2, *out = c + 3;
+  const uint32_t filters = (uint32_t)pci[16];
+  const real strength = R(par[0]), luma = R(par[1]), detail = R(par[2]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const real sx0 = (R(x) + R(0.5)) / R(out->wd), sy0 = (R(y) + R(0.5)) / R(out->ht);
+    v4 upsm = bilin(crs0, sx0 * R(crs0->wd), sy0 * R(crs0->ht), 0);
+    v4 down = bilin(crs1b, sx0 * R(crs1b->wd), sy0 * R(crs1b->ht), 0);
+    real black = R(pcf[5]), white = R(pcf[9]), crs = upsm.y, crs1 = down.y;
+    real T = R(0.5) * strength * upsm.w, blendw = luma;
+    if(filters == 9)
+    {
+      const int col = xtrans_colour(x, y);
+      if(col == 2)
+      {
+        black = R(pcf[6]); white = R(pcf[10]); crs = upsm.z; crs1 = down.z; blendw = R(1.0);
+        T /= FMAX(R(1e-4), upsm.w);
+      }
+      else if(col == 0)
+      {
+        black = R(pcf[4]); white = R(pcf[8]); crs = upsm.x; crs1 = down.x; blendw = R(1.0);
+        T /= FMAX(R(1e-4), upsm.w);
+      }
+    }
+    else
+    {
+      const int ox = x & 1, oy = y & 1;
+      if((ox == 0 && oy == 1) || (ox == 1 && oy == 0)) { }
+      else if(ox == 0 && oy == 0)
+      { black = R(pcf[4]); white = R(pcf[8]); crs = upsm.x; crs1 = down.x; blendw = R(1.0); }
+      else
+      { black = R(pcf[6]); white = R(pcf[10]); crs = upsm.z; crs1 = down.z; blendw = R(1.0); }
+    }
+    real sig[3];
+    noise_sigma(sig, R(pcf[17]), R(pcf[18]), black, white, par + 4, crs);
+    real val = centre(in, x + pci[12], y + pci[13], 0).x;
+    blendw = R(0.5) * (blendw + R(1.0));
+    if(val < white)
+    {
+      const real wav = (val - crs1) / FMAX(sig[0] + sig[2], R(1e-8));
+      const real tt = FMIN(R(1.0), wav / FMAX(R(2.0) * T, R(1e-8)));
+      real uw = POW(FMIN(R(1.0), R(1.0) * upsm.w), R(4.0));
+      uw = R(1.0) - (R(1.0) - uw) * detail;
+      const real aw = FABS(wav);
+      val = mixr(val, FMAX(R(0), crs + sig[1] * signr(wav) * mixr(FMAX(R(0.0), aw - T), aw, tt)), uw * blendw);
+    }
+    val = FMAX(R(0), (val - black) / (white - black));
+    v4 ov = { val, val, val, R(1) };
+    store(out, x, y, 0, ov);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * demosaic (src/pipe/modules/demosaic)
+ * --------------------------------------------------------------------- */
+
+// down.comp:18-47 (its output is never sampled by gauss.comp)
+static int dm_down(const int32_t *pci, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const uint32_t filters = (uint32_t)pci[4];
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real lum = 0;
+    if(filters == 9)
+    {
+      for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) lum += fetch(in, 3 * x + i, 3 * y + j, 0).x;
+      lum = lum / R(9.0);
+    }
+    else
+    {
+      const real c0 = fetch(in, 2 * x, 2 * y, 0).x, c1 = fetch(in, 2 * x, 2 * y + 1, 0).x;
+      const real c2 = fetch(in, 2 * x + 1, 2 * y, 0).x, c3 = fetch(in, 2 * x + 1, 2 * y + 1, 0).x;
+      lum = (c0 + c1 + c2 + c3) / R(4.0);
+    }
+    v4 o = { lum, lum, lum, lum };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// gauss.comp:19-126
+static int dm_gauss(const int32_t *pci, const or_buf_t *c)
+{
+  const or_buf_t *orig = c + 1, *out = c + 2;
+  const uint32_t filters = (uint32_t)pci[4];
+  const int xt = filters == 9;
+  const int n0 = xt ? 0 : -1, n1 = xt ? 3 : 3, bs = xt ? 3 : 2;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real mw[2] = {0, 0}, mb[2] = {0, 0}, smw = 0, smb = 0;
+    for(int j = n0; j < n1; j++) for(int i = n0; i < n1; i++)
+    {
+      if(xt ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
+      const real px = centre(orig, bs * x + i, bs * y + j, 0).x;
+      mw[0] += R(i) * px; mw[1] += R(j) * px;
+      smw += px;
+      mb[0] += R(i) / px; mb[1] += R(j) / px;
+      smb += R(1.0) / px;
+    }
+    mw[0] /= smw; mw[1] /= smw;
+    mb[0] /= smb; mb[1] /= smb;
+    real Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0;
+    for(int j = n0; j < n1; j++) for(int i = n0; i < n1; i++)
+    {
+      if(xt ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
+      const real px = centre(orig, bs * x + i, bs * y + j, 0).x;
+      real p2 = px * px;
+      real dx = R(i) - mw[0], dy = R(j) - mw[1];
+      Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
+      sw += p2;
+      dx = R(i) - mb[0]; dy = R(j) - mb[1];
+      p2 = R(1.0) / p2;
+      Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
+      sb += p2;
+    }
+    for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
+    const real *S = det2(Sw) < det2(Sb) ? Sw : Sb;
+    real eval[2], e0[2], e1[2];
+    evd2x2(eval, e0, e1, S);
+    if(!xt)
+    {
+      eval[0] *= R(0.2); eval[1] *= R(0.2);
+      if(FABS(e0[0]) > FABS(e0[1])) { e0[0] = 1; e0[1] = 0; }
+      else                          { e0[0] = 0; e0[1] = 1; }
+    }
+    else
+    {
+      eval[0] *= R(0.4); eval[1] *= R(0.4);
+      if     (FABS(e0[0]) > R(2.) * FABS(e0[1])) { e0[0] = 1; e0[1] = 0; }
+      else if(FABS(e0[1]) > R(2.) * FABS(e0[0])) { e0[0] = 0; e0[1] = 1; }
+      else { eval[0] = R(.1); eval[1] = R(.1); }
+    }
+    v4 o = { eval[0], eval[1], e0[0], e0[1] };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// splat.comp:16-39 eval_gauss
+static inline void splat_eval(const or_buf_t *in, const v4 cov, int ox, int oy, int i, int j,
+                              real *col, real *weight)
+{
+  int px = ox + i, py = oy + j;
+  if(px < 0) px += 6;
+  if(py < 0) py += 6;
+  if(px >= in->wd) px -= 6;
+  if(py >= in->ht) py -= 6;
+  *col = fetch(in, px, py, 0).x;
+  const real ex = clampr(cov.x, R(0.01), R(25)), ey = clampr(cov.y, R(0.01), R(25));
+  // E = mat2(cov.z, -cov.w, cov.w, cov.z): of = E*o
+  const real ofx = cov.z * R(i) + cov.w * R(j);
+  const real ofy = -cov.w * R(i) + cov.z * R(j);
+  real w = clampr(EXP(R(-0.5) * ((ofx / ex) * ofx + (ofy / ey) * ofy)), R(1e-4), R(1.0));
+  if(i == 0 && j == 0) w = R(666.0);
+  *weight = w;
+}
+
+// splat.comp:98-140
+static int dm_splat(const int32_t *pci, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *ga = c + 1, *out = c + 2;
+  const uint32_t filters = (uint32_t)pci[4];
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    if(filters == 9)
+    {
+      const v4 cov = fetch(ga, x / 3, y / 3, 0);
+      for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+      {
+        real col, wt;
+        splat_eval(in, cov, x, y, i, j, &col, &wt);
+        const int cl = xtrans_colour(x + i + 6, y + j + 6);
+        const int k = cl == 1 ? 1 : (cl == 2 ? 2 : 0);
+        rgb[k] += col * wt;
+        w[k] += wt;
+      }
+    }
+    else
+    {
+      const v4 cov = fetch(ga, (x + 1) / 2, (y + 1) / 2, 0);
+      for(int j = -1; j <= 1; j++) for(int i = -1; i <= 1; i++)
+      {
+        real col, wt;
+        splat_eval(in, cov, x, y, i, j, &col, &wt);
+        const int px = x + i, py = y + j;
+        col *= wt;
+        if(((px & 1) == 0 && (py & 1) == 1) || ((px & 1) == 1 && (py & 1) == 0)) { rgb[1] += col; w[1] += wt; }
+        else if((px & 1) == 0 && (py & 1) == 0) { rgb[0] += col; w[0] += wt; }
+        else { rgb[2] += col; w[2] += wt; }
+      }
+    }
+    const real g = rgb[1] / FMAX(R(1e-8), w[1]);
+    v4 o = { g, g, g, R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// fix.comp:21-29 eval_gauss with the eigenvalues pre-scaled by the caller
+static inline real fix_eval(real ex, real ey, real cz, real cw, int i, int j)
+{
+  const real ofx = cz * R(i) + cw * R(j);
+  const real ofy = -cw * R(i) + cz * R(j);
+  return clampr(EXP(R(-0.5) * ((ofx / ex) * ofx + (ofy / ey) * ofy)), R(1e-3), R(1.0));
+}
+
+// fix.comp:99-135
+static int dm_fix(const int32_t *pci, const int32_t *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *gr = c + 1, *cv = c + 2, *out = c + 3;
+  const uint32_t filters = (uint32_t)pci[4];
+  const int fixup = par ? par[0] : 0;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    const real gc = centre(gr, x, y, 0).x;
+    if(filters == 9)
+    {
+      v4 cov = fetch(cv, (x + 1) / 3, (y + 1) / 3, 0);
+      cov.x = clampr(cov.x, R(1.), R(10.)); cov.y = clampr(cov.y, R(1.), R(10.));
+      int r = fixup + 2; r = r < 2 ? 2 : (r > 3 ? 3 : r);
+      for(int j = -r; j <= r; j++) for(int i = -r; i <= r; i++)
+      {
+        const real gh = centre(gr, x + i, y + j, 0).x;
+        const real col = centre(in, x + i, y + j, 0).x;
+        const real corr = (R(1e-4) + gc) / (R(1e-4) + gh);
+        const int px = x + i, py = y + j;   // may be negative: truncating division
+        const int blue_top = ((px / 3 + py / 3) & 1) > 0;
+        const int mx = px - (px / 3) * 3, my = py - (py / 3) * 3;
+        if(((mx + my) & 1) == 0) { rgb[1] = gc; w[1] = 1; }
+        else
+        {
+          const real wt = fix_eval(R(3) * cov.x, R(3) * cov.y, cov.z, cov.w, i, j);
+          const int k = (blue_top ^ (my == 1)) ? 2 : 0;
+          rgb[k] += col * corr * wt;
+          w[k] += wt;
+        }
+      }
+    }
+    else
+    {
+      v4 cov = fetch(cv, (x + 1) / 2, (y + 1) / 2, 0);
+      cov.x = clampr(cov.x, R(1.0), R(49)); cov.y = clampr(cov.y, R(1.0), R(49));
+      int r = fixup + 1; r = r < 1 ? 1 : (r > 2 ? 2 : r);
+      for(int j = -r; j <= r; j++) for(int i = -r; i <= r; i++)
+      {
+        const real gh = centre(gr, x + i, y + j, 0).x;
+        const real col = centre(in, x + i, y + j, 0).x;
+        const int ox = x + i, oy = y + j;
+        if((ox & 1) == 0 && (oy & 1) == 0)
+        {
+          const real wt = fix_eval(R(2) * cov.x, R(2) * cov.y, cov.z, cov.w, i, j);
+          rgb[0] += col * (R(1e-4) + gc) / (R(1e-4) + gh) * wt;
+          w[0] += wt;
+        }
+        else if(((ox & 1) == 0 && (oy & 1) == 1) || ((ox & 1) == 1 && (oy & 1) == 0))
+        { rgb[1] = gc; w[1] = 1; }
+        else
+        {
+          const real wt = fix_eval(R(2) * cov.x, R(2) * cov.y, cov.z, cov.w, i, j);
+          rgb[2] += col * (R(1e-4) + gc) / (R(1e-4) + gh) * wt;
+          w[2] += wt;
+        }
+      }
+    }
+    v4 o = { rgb[0] / FMAX(R(1e-8), w[0]), rgb[1] / FMAX(R(1e-8), w[1]), rgb[2] / FMAX(R(1e-8), w[2]), R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * colour (src/pipe/modules/colour/main-impl.glsl:200-341, matrix path)
+ * --------------------------------------------------------------------- */
+
+static const double M_cat16[9]  = { 0.401288, 0.650173, -0.051461, -0.250268, 1.204414, 0.045854, -0.002079, 0.048952, 0.953127 };
+static const double M_cat16i[9] = { 1.86206786, -1.01125463, 0.14918677, 0.38752654, 0.62144744, -0.00897398, -0.01584150, -0.03412294, 1.04996444 };
+static const double M_2020_xyz[9] = { 0.636958048301290991, 0.144616903586208406, 0.168880975164172054, 0.26270021201126692, 0.677998071518871148, 0.0593017164698619384, 4.9999999999999999e-17, 0.0280726930490874452, 1.06098505771079066 };
+static const double M_xyz_2020[9] = { 1.71665119, -0.35567078, -0.25336628, -0.66668435, 1.61648124, 0.01576855, 0.01763986, -0.04277061, 0.94210312 };
+
+static int co_main(const int32_t *pci, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const uint32_t *pu = (const uint32_t *)par;
+  const int have_clut = pci ? pci[0] : 0, have_pick = pci ? pci[1] : 0, have_abney = pci ? pci[2] : 0;
+  const uint32_t N = pu[16], colour_mode = pu[225], primaries = pu[229], trc = pu[230];
+  const float saturation = par[226], clip = par[231];
+  if(have_clut || have_pick || have_abney || N > 0 || saturation != 1.0f || clip > 0.0f) return -2;
+  if(colour_mode != 0 || trc != 0) return -2;
+  if(primaries != 0 && primaries != 2) return -2;
+  real M[9];   // row j col i = f[4+4i+j]
+  for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++) M[3 * j + i] = R(par[4 + 4 * i + j]);
+  // cat16 from (1,1,1) to mul.rgb
+  double A[9];
+  matmul3(A, M_cat16, M_2020_xyz);
+  real toc[9], fromc[9];
+  double B[9];
+  matmul3(B, M_xyz_2020, M_cat16i);
+  for(int k = 0; k < 9; k++) { toc[k] = R(A[k]); fromc[k] = R(B[k]); }
+  const real one[3] = { 1, 1, 1 }, mul[3] = { R(par[0]), R(par[1]), R(par[2]) };
+  real cl_src[3], cl_dst[3];
+  mv3(cl_src, toc, one);
+  mv3(cl_dst, toc, mul);
+  const real expo = R(par[3]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 t = centre(in, x, y, 0);
+    real rgb[3] = { t.x, t.y, t.z }, r2[3], cl[3];
+    if(primaries == 0) { mv3(r2, M, rgb); memcpy(rgb, r2, sizeof(rgb)); }
+    mv3(cl, toc, rgb);
+    for(int k = 0; k < 3; k++) cl[k] *= cl_dst[k] / cl_src[k];
+    mv3(rgb, fromc, cl);
+    for(int k = 0; k < 3; k++) rgb[k] = clampr(rgb[k] * expo, R(-65535.0), R(65535.0));
+    v4 o = { rgb[0], rgb[1], rgb[2], R(1) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * filmcurv (src/pipe/modules/filmcurv/main.comp:153-252, modes 1 and 3)
+ * --------------------------------------------------------------------- */
+
+static inline real weibull(real x, real il, real k)
+{ // params.glsl:16-20
+  return R(1.0) - EXP(-POW(FMAX(x, R(1e-7)) * il, k));
+}
+
+// shared.glsl:371-387
+static void adjust_colour_dng(const real c0in[3], real c1[3])
+{
+  real c0[3] = { c0in[0], c0in[1], c0in[2] }, t;
+  int fx = 0, fy = 0, fz = 0;
+  if(c0[2] > c0[1]) { t = c0[2]; c0[2] = c0[1]; c0[1] = t; t = c1[2]; c1[2] = c1[1]; c1[1] = t; fx = 1; }
+  if(c0[1] > c0[0]) { t = c0[0]; c0[0] = c0[1]; c0[1] = t; t = c1[0]; c1[0] = c1[1]; c1[1] = t; fy = 1; }
+  if(c0[2] > c0[1]) { t = c0[2]; c0[2] = c0[1]; c0[1] = t; t = c1[2]; c1[2] = c1[1]; c1[1] = t; fz = 1; }
+  c1[1] = mixr(c1[2], c1[0], (c0[1] - c0[2] + R(1e-6)) / (c0[0] - c0[2] + R(1e-6)));
+  if(fz) { t = c1[2]; c1[2] = c1[1]; c1[1] = t; }
+  if(fy) { t = c1[0]; c1[0] = c1[1]; c1[1] = t; }
+  if(fx) { t = c1[2]; c1[2] = c1[1]; c1[1] = t; }
+}
+
+static int fc_main(const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const int mode = ((const int32_t *)par)[3];
+  if(mode != 1 && mode != 3) return -2;
+  const real il = FMAX(R(5e-3), R(par[0])), k = FMAX(R(1e-4), R(par[1])), bias = R(par[2]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 t = fetch(in, x, y, 0);
+    real c0[3] = { t.x + bias, t.y + bias, t.z + bias }, c1[3];
+    for(int q = 0; q < 3; q++) c1[q] = weibull(c0[q], il, k);
+    if(mode == 3) adjust_colour_dng(c0, c1);
+    v4 o = { c1[0], c1[1], c1[2], R(1) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * llap (src/pipe/modules/llap)
+ * --------------------------------------------------------------------- */
+
+// curve.comp:39-63
+static inline real ll_curve(real x, real g, real sigma, real shadows, real highlights, real clarity)
+{
+  const real c = x - g;
+  real val;
+  const real ssigma = c > R(0) ? sigma : -sigma;
+  const real shadhi = c > R(0) ? shadows : highlights;
+  if(FABS(c) > R(2) * sigma) val = g + ssigma + shadhi * (c - ssigma);
+  else
+  {
+    const real t = clampr(c / (R(2.0) * ssigma), R(0), R(1));
+    const real t2 = t * t;
+    const real mt = R(1.0) - t;
+    val = g + ssigma * R(2.0) * mt * t + t2 * (ssigma + ssigma * shadhi);
+  }
+  val += clarity * c * EXP(-c * c / (R(2.0) * sigma * sigma / R(3.0)));
+  return val;
+}
+
+// llap.glsl:3-21 (linear spacing; ng = number of curves)
+static inline real gamma_from_i(int i, int ng) { return R(i) / (R(ng) - R(1.0)); }
+static inline int gamma_hi_from_v(real v, int ng)
+{
+  int hi = 1;
+  for(; hi < ng - 1 && gamma_from_i(hi, ng) <= v; hi++);
+  return hi;
+}
+
+static int ll_curve_node(const int32_t *pci, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const int ng = pci[0];
+  if(out->array_length < ng + 1) return -3;
+  for(int y = 0; y < in->ht; y++) for(int x = 0; x < in->wd; x++)
+  {
+    v4 t = fetch(in, x, y, 0);
+    const real yl = lum2020(clampr(t.x, R(-1000.0), R(1000.0)), clampr(t.y, R(-1000.0), R(1000.0)),
+                            clampr(t.z, R(-1000.0), R(1000.0)));
+    for(int i = 0; i < ng; i++)
+    {
+      const real v = ll_curve(yl, gamma_from_i(i, ng), R(par[0]), R(par[1]), R(par[2]), R(par[3]));
+      v4 o = { v, v, v, v };
+      store(out, x, y, i, o);
+    }
+    v4 o = { yl, yl, yl, yl };
+    store(out, x, y, ng, o);
+  }
+  return 0;
+}
+
+// reduce.comp:17-35 with sample_semisoft (shared.glsl:130-148)
+static int ll_reduce(const or_buf_t *c, int dp)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  for(int z = 0; z < dp && z < out->array_length; z++)
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const real sc = R(2 * x) + R(0.5), tc = R(2 * y) + R(0.5);
+    const real s0 = sc - R(.5), s1 = sc + R(.5), t0 = tc - R(.5), t1 = tc + R(.5);
+    const real r = (bilin(in, s0, t0, z).x + bilin(in, s1, t0, z).x + bilin(in, s0, t1, z).x
+                  + bilin(in, s1, t1, z).x) / R(4.0);
+    v4 o = { r, r, r, r };
+    store(out, x, y, z, o);
+  }
+  return 0;
+}
+
+// assemble.comp:134-138 gauss_expand with sample_soft (shared.glsl:99-127)
+static inline real expand(const or_buf_t *img, int pl, int ox, int oy)
+{
+  const real sc = R(ox) * R(0.5) + R(0.5), tc = R(oy) * R(0.5) + R(0.5);
+  const real s[3] = { sc - R(1.5), sc, sc + R(1.5) }, t[3] = { tc - R(1.5), tc, tc + R(1.5) };
+  real r = 0;
+  for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++) r += bilin(img, s[i], t[j], pl).x;
+  return r / R(9.0);
+}
+
+// assemble.comp:169-203
+static int ll_assemble(const int32_t *pci, const or_buf_t *c)
+{
+  const or_buf_t *coarse = c + 0, *l0 = c + 1, *l1 = c + 2, *out = c + 3;
+  const int ng = pci[0], first = pci[1];
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const real res = first > 0 ? expand(l1, ng, x, y) : expand(coarse, 0, x, y);
+    const real v = fetch(l0, x, y, ng).x;
+    const int hi = gamma_hi_from_v(v, ng);
+    const int lo = hi - 1;
+    const real glo = gamma_from_i(lo, ng), ghi = gamma_from_i(hi, ng);
+    const real a = clampr((v - glo) / (ghi - glo), R(0.0), R(1.0));
+    const real lv0 = fetch(l0, x, y, hi - 1).x - expand(l1, hi - 1, x, y);
+    const real lv1 = fetch(l0, x, y, hi).x - expand(l1, hi, x, y);
+    const real r = res + lv0 * (R(1.0) - a) + lv1 * a;
+    v4 o = { r, r, r, r };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// colour.comp:221-240
+static int ll_colour(const or_buf_t *c)
+{
+  const or_buf_t *lum = c + 0, *org = c + 1, *out = c + 2;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real yv = fetch(lum, x, y, 0).x;
+    v4 t = fetch(org, x, y, 0);
+    const real yo = FMAX(lum2020(t.x, t.y, t.z), R(1e-8));
+    const real cc = R(1.0);
+    if(yv < yo) yv = yo * EXP(cc * (yv - yo));
+    v4 o = { FMAX(R(0), t.x * yv / yo), FMAX(R(0), t.y * yv / yo), FMAX(R(0), t.z * yv / yo), R(1) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * node dispatch
+ * --------------------------------------------------------------------- */
+
+int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
+            const void *params, int params_size, const or_buf_t *conn, int nconn,
+            int wd, int ht, int dp)
+{
+  (void)pc_size; (void)params_size;
+  const int32_t *pci = (const int32_t *)pc;
+  const float *pcf = (const float *)pc;
+  const float *par = (const float *)params;
+#define IS(m, k) (!strcmp(module, m) && !strcmp(kernel, k))
+  if(IS("denoise", "noop"))     return nconn >= 2 ? dn_noop(pci, pcf, params, conn, wd, ht) : -1;
+  if(IS("denoise", "half"))     return nconn >= 2 ? dn_half(pci, pcf, conn) : -1;
+  if(IS("denoise", "downcov"))  return nconn >= 2 ? dn_downcov(pci, pcf, conn, nconn) : -1;
+  if(IS("denoise", "down"))     return nconn >= 2 ? dn_down(pci, pcf, par, conn) : -1;
+  if(IS("denoise", "assemble")) return nconn >= 6 ? dn_assemble(pci, pcf, par, conn) : -1;
+  if(IS("denoise", "doub"))     return nconn >= 4 ? dn_doub(pci, pcf, par, conn) : -1;
+  if(IS("demosaic", "down"))    return nconn >= 2 ? dm_down(pci, conn) : -1;
+  if(IS("demosaic", "gauss"))   return nconn >= 3 ? dm_gauss(pci, conn) : -1;
+  if(IS("demosaic", "splat"))   return nconn >= 3 ? dm_splat(pci, conn) : -1;
+  if(IS("demosaic", "fix"))     return nconn >= 4 ? dm_fix(pci, (const int32_t *)params, conn) : -1;
+  if(IS("colour", "main"))      return nconn >= 2 ? co_main(pci, par, conn) : -1;
+  if(IS("filmcurv", "main"))    return nconn >= 2 ? fc_main(par, conn) : -1;
+  if(IS("llap", "curve"))       return nconn >= 2 ? ll_curve_node(pci, par, conn) : -1;
+  if(IS("llap", "reduce"))      return nconn >= 2 ? ll_reduce(conn, dp) : -1;
+  if(IS("llap", "assemble"))    return nconn >= 4 ? ll_assemble(pci, conn) : -1;
+  if(IS("llap", "colour"))      return nconn >= 3 ? ll_colour(conn) : -1;
+#undef IS
+  return -1;
+}
+
+/* ------------------------------------------------------------------------
+ * whole graph: the node DAG the modules' create_nodes build
+ * (denoise/main.c:133-333, demosaic/main.c:59-203, colour/main.c:219-365,
+ * filmcurv/main.c:25-51, llap/main.c:14-114), run in dependency order.
+ * --------------------------------------------------------------------- */
+
+static double *alloc_img(int wd, int ht, int chan, int planes)
+{
+  return (double *)calloc((size_t)wd * ht * chan * planes, sizeof(double));
+}
+
+int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double *out_rgba,
+               double *st_denoise, double *st_demosaic, double *st_colour, double *st_filmcurv)
+{
+  const int xt = p->filters == 9;
+  const int block = xt ? 3 : 2;
+  const int Qw = W / block, Qh = H / block;
+  float black[4], white[4];
+  for(int k = 0; k < 4; k++) { black[k] = p->black[k] / 65535.0f; white[k] = p->white[k] / 65535.0f; }
+  float dpar[9] = { p->dn_strength, p->dn_luma, p->dn_detail, 0.0f,
+                    p->dn_edges[0], p->dn_edges[1], p->dn_edges[2], p->dn_edges[3], 0 };
+  ((int32_t *)dpar)[8] = 1;
+
+  or_buf_t raw = { (void *)cfa, W, H, 1, OR_UI16, 1 };
+  double *half = alloc_img(Qw, Qh, 4, 1), *s[4], *asm_ = alloc_img(Qw, Qh, 4, 1);
+  for(int l = 0; l < 4; l++) s[l] = alloc_img(Qw, Qh, 4, 1);
+  double *dn = alloc_img(W, H, 1, 1);
+  int err = 0;
+  {
+    int32_t pch[17];
+    float *f = (float *)pch;
+    for(int k = 0; k < 4; k++) { f[k] = p->wb[k]; f[4 + k] = black[k]; f[8 + k] = white[k]; pch[12 + k] = 0; }
+    pch[12 + 2] = W; pch[12 + 3] = H;
+    pch[16] = p->filters;
+    or_buf_t c[2] = { raw, { half, Qw, Qh, 4, OR_F16, 1 } };
+    err |= or_node("denoise", "half", pch, sizeof(pch), dpar, sizeof(dpar), c, 2, Qw, Qh, 1);
+  }
+  for(int l = 0; l < 4; l++)
+  {
+    int32_t pc[20];
+    float *f = (float *)pc;
+    for(int k = 0; k < 4; k++) { f[k] = p->wb[k]; f[4 + k] = black[k]; f[8 + k] = white[k]; pc[12 + k] = 0; }
+    f[16] = p->noise_a; f[17] = p->noise_b; pc[18] = l; pc[19] = block;
+    or_buf_t c[2] = { { l == 0 ? half : s[l - 1], Qw, Qh, 4, OR_F16, 1 }, { s[l], Qw, Qh, 4, OR_F16, 1 } };
+    err |= or_node("denoise", l == 0 ? "downcov" : "down", pc, sizeof(pc), dpar, sizeof(dpar), c, 2, Qw, Qh, 1);
+  }
+  {
+    int32_t pc[19];
+    float *f = (float *)pc;
+    for(int k = 0; k < 4; k++) { f[k] = p->wb[k]; f[4 + k] = black[k]; f[8 + k] = white[k]; pc[12 + k] = 0; }
+    f[16] = p->noise_a; f[17] = p->noise_b; pc[18] = p->filters;
+    or_buf_t c[6] = {
+      { half, Qw, Qh, 4, OR_F16, 1 }, { s[0], Qw, Qh, 4, OR_F16, 1 }, { s[1], Qw, Qh, 4, OR_F16, 1 },
+      { s[2], Qw, Qh, 4, OR_F16, 1 }, { s[3], Qw, Qh, 4, OR_F16, 1 }, { asm_, Qw, Qh, 4, OR_F16, 1 } };
+    err |= or_node("denoise", "assemble", pc, sizeof(pc), dpar, sizeof(dpar), c, 6, Qw, Qh, 1);
+  }
+  {
+    int32_t pc[24];
+    float *f = (float *)pc;
+    for(int k = 0; k < 4; k++) { f[k] = p->wb[k]; f[4 + k] = black[k]; f[8 + k] = white[k]; pc[12 + k] = 0; }
+    pc[12 + 2] = W; pc[12 + 3] = H;
+    pc[16] = p->filters; f[17] = p->noise_a; f[18] = p->noise_b; pc[19] = 0;
+    f[20] = f[21] = f[22] = f[23] = 0.0f;
+    or_buf_t c[4] = { raw, { asm_, Qw, Qh, 4, OR_F16, 1 }, { half, Qw, Qh, 4, OR_F16, 1 }, { dn, W, H, 1, OR_F16, 1 } };
+    err |= or_node("denoise", "doub", pc, sizeof(pc), dpar, sizeof(dpar), c, 4, W, H, 1);
+  }
+  free(half); free(asm_);
+  for(int l = 0; l < 4; l++) free(s[l]);
+  if(st_denoise) memcpy(st_denoise, dn, sizeof(double) * W * H);
+
+  // demosaic: gauss -> splat -> fix ("down" is wired but never sampled)
+  double *cov = alloc_img(Qw, Qh, 4, 1), *green = alloc_img(W, H, 1, 1), *dm = alloc_img(W, H, 4, 1);
+  {
+    int32_t pc[5];
+    float *f = (float *)pc;
+    for(int k = 0; k < 4; k++) f[k] = p->wb[k];
+    pc[4] = 0; // demosaic sees the filters of its input
+    pc[4] = p->filters;
+    int32_t dmpar[2] = { p->dm_colour, 0 };
+    or_buf_t cin = { dn, W, H, 1, OR_F16, 1 };
+    or_buf_t cg[3] = { cin, cin, { cov, Qw, Qh, 4, OR_F16, 1 } };
+    err |= or_node("demosaic", "gauss", pc, sizeof(pc), dmpar, sizeof(dmpar), cg, 3, Qw, Qh, 1);
+    or_buf_t cs[3] = { cin, { cov, Qw, Qh, 4, OR_F16, 1 }, { green, W, H, 1, OR_F16, 1 } };
+    err |= or_node("demosaic", "splat", pc, sizeof(pc), dmpar, sizeof(dmpar), cs, 3, W, H, 1);
+    or_buf_t cf[4] = { cin, { green, W, H, 1, OR_F16, 1 }, { cov, Qw, Qh, 4, OR_F16, 1 }, { dm, W, H, 4, OR_F16, 1 } };
+    err |= or_node("demosaic", "fix", pc, sizeof(pc), dmpar, sizeof(dmpar), cf, 4, W, H, 1);
+  }
+  free(cov); free(green); free(dn);
+  if(st_demosaic) memcpy(st_demosaic, dm, sizeof(double) * W * H * 4);
+
+  // colour: commit_params (colour/main.c:219-365) for matrix=1, white=0
+  double *co = alloc_img(W, H, 4, 1);
+  {
+    float cp[242];
+    memset(cp, 0, sizeof(cp));
+    uint32_t *cu = (uint32_t *)cp;
+    float pwb[3] = { p->co_white[0], p->co_white[1], p->co_white[2] };
+    if(pwb[0] == 0.0f && pwb[1] == 0.0f && pwb[2] == 0.0f)
+    {
+      float w0[3] = { 0, 0, 0 }, w[3] = { p->wb[0], p->wb[1], p->wb[2] };
+      for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++) w0[j] += p->cam_to_rec2020[3 * j + i] / w[i];
+      w0[0] /= w0[1]; w0[2] /= w0[1]; w0[1] = 1.0f;
+      pwb[0] = 1 / w0[0]; pwb[1] = 1; pwb[2] = 1 / w0[2];
+    }
+    if(!(pwb[0] == pwb[0]) || pwb[0] == 0.0f || pwb[1] == 0.0f || pwb[2] == 0.0f) pwb[0] = pwb[1] = pwb[2] = 1.0f;
+    cp[0] = pwb[0] / pwb[1]; cp[1] = 1.0f; cp[2] = pwb[2] / pwb[1];
+    cp[3] = powf(2.0f, p->co_exposure);
+    for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++) cp[4 + 4 * i + j] = p->cam_to_rec2020[3 * j + i];
+    cp[226] = 1.0f;   // saturation
+    cu[229] = 0;      // primaries custom (i-raw)
+    cu[230] = 0;      // trc linear
+    int32_t pc[3] = { 0, 0, 0 };
+    or_buf_t c[2] = { { dm, W, H, 4, OR_F16, 1 }, { co, W, H, 4, OR_F16, 1 } };
+    err |= or_node("colour", "main", pc, sizeof(pc), cp, sizeof(cp), c, 2, W, H, 1);
+  }
+  free(dm);
+  if(st_colour) memcpy(st_colour, co, sizeof(double) * W * H * 4);
+
+  double *fc = alloc_img(W, H, 4, 1);
+  {
+    float fp[10] = { p->fc_light, p->fc_contrast, p->fc_bias, 0, 1.0f, 0, 0, 0, 0, 0 };
+    ((int32_t *)fp)[3] = p->fc_colour;
+    or_buf_t c[2] = { { co, W, H, 4, OR_F16, 1 }, { fc, W, H, 4, OR_F16, 1 } };
+    err |= or_node("filmcurv", "main", 0, 0, fp, sizeof(fp), c, 2, W, H, 1);
+  }
+  free(co);
+  if(st_filmcurv) memcpy(st_filmcurv, fc, sizeof(double) * W * H * 4);
+
+  // llap: curve, reduce on all levels, assemble coarse to fine, colour
+  {
+    const int ng = p->ll_num_gamma;
+    const int maxl = p->ll_max_levels;
+    float lp[4] = { p->ll_sigma, p->ll_shadows, p->ll_hilights, p->ll_clarity };
+    int lw[16], lh[16];
+    lw[0] = W; lh[0] = H;
+    int nl = maxl;
+    for(int l = 1; l < maxl; l++)
+    {
+      lw[l] = (lw[l - 1] - 1) / 2 + 1;
+      lh[l] = (lh[l - 1] - 1) / 2 + 1;
+      const int nw = (lw[l] - 1) / 2 + 1, nh = (lh[l] - 1) / 2 + 1;
+      if(nw <= 1 || nh <= 1) { nl = l + 1; break; }
+    }
+    double *pyr[16], *outl[16];
+    for(int l = 0; l < nl; l++) { pyr[l] = alloc_img(lw[l], lh[l], 1, ng + 1); outl[l] = 0; }
+    int32_t pcc[1] = { ng };
+    or_buf_t cc[2] = { { fc, W, H, 4, OR_F16, 1 }, { pyr[0], W, H, 1, OR_F16, ng + 1 } };
+    err |= or_node("llap", "curve", pcc, sizeof(pcc), lp, sizeof(lp), cc, 2, W, H, 1);
+    for(int l = 1; l < nl; l++)
+    {
+      or_buf_t cr[2] = { { pyr[l - 1], lw[l - 1], lh[l - 1], 1, OR_F16, ng + 1 },
+                         { pyr[l], lw[l], lh[l], 1, OR_F16, ng + 1 } };
+      err |= or_node("llap", "reduce", 0, 0, lp, sizeof(lp), cr, 2, lw[l], lh[l], ng + 1);
+    }
+    for(int l = nl - 1; l >= 1; l--)
+    {
+      outl[l - 1] = alloc_img(lw[l - 1], lh[l - 1], 1, 1);
+      int32_t pca[2] = { ng, l == nl - 1 ? 1 : 0 };
+      or_buf_t coarse = l == nl - 1 ? (or_buf_t){ pyr[0], W, H, 1, OR_F16, ng + 1 }
+                                    : (or_buf_t){ outl[l], lw[l], lh[l], 1, OR_F16, 1 };
+      or_buf_t ca[4] = { coarse, { pyr[l - 1], lw[l - 1], lh[l - 1], 1, OR_F16, ng + 1 },
+                         { pyr[l], lw[l], lh[l], 1, OR_F16, ng + 1 }, { outl[l - 1], lw[l - 1], lh[l - 1], 1, OR_F16, 1 } };
+      err |= or_node("llap", "assemble", pca, sizeof(pca), lp, sizeof(lp), ca, 4, lw[l - 1], lh[l - 1], 1);
+      if(l < nl - 1) { free(outl[l]); outl[l] = 0; }
+    }
+    or_buf_t cl[3] = { { outl[0], W, H, 1, OR_F16, 1 }, { fc, W, H, 4, OR_F16, 1 }, { out_rgba, W, H, 4, OR_F32, 1 } };
+    err |= or_node("llap", "colour", 0, 0, lp, sizeof(lp), cl, 3, W, H, 1);
+    free(outl[0]);
+    for(int l = 0; l < nl; l++) free(pyr[l]);
+  }
+  free(fc);
+  return err ? -1 : 0;
+}

@@ -1,0 +1,100 @@
+/*
+ * oracle.h -- CPU restatement of vkdt's raw-development node kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (vkdt_amd/, include/)
+ * links, loads or calls this code.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, and only as the checker.
+ *
+ * One source, two builds:
+ *   liboracle64.so  arithmetic in double ("FP64 oracle")
+ *   liboracle32.so  arithmetic in float  (same operation order as the GLSL)
+ * Buffers are always double in host memory; with storage mode on (the
+ * default) every f16 image store is rounded to the nearest binary16 value
+ * (round-to-nearest-even, overflow to inf), which is what the reference's
+ * R16F/RGBA16F storage images do.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_UI16 = 1, OR_F16 = 2, OR_F32 = 3 };
+
+typedef struct or_buf_t
+{
+  void    *data;          // uint16_t* for OR_UI16, double* otherwise
+  int32_t  wd, ht;        // texel extent
+  int32_t  chan;          // 1 or 4 (interleaved)
+  int32_t  format;        // OR_UI16, OR_F16 or OR_F32
+  int32_t  array_length;  // planes, >= 1, plane i at offset i*wd*ht*chan
+}
+or_buf_t;
+
+// storage rounding of f16 images (1 = binary16 round trip on store, 0 = keep full precision)
+void or_set_storage_f16(int on);
+// 8 for the float build, 16 for the double build
+int or_real_bytes(void);
+
+// run one node with the push-constant and params blobs exactly as the
+// module's create_nodes/commit_params produced them (std140 layouts of the
+// GLSL kernels).  returns 0 on success, <0 on unknown node or bad buffers.
+int or_node(
+    const char *module, const char *kernel,
+    const void *pc, int pc_size,
+    const void *params, int params_size,
+    const or_buf_t *conn, int nconn,
+    int wd, int ht, int dp);
+
+// half precision rounding helper (binary16, RNE), exposed for tests
+double or_f16_round(double x);
+// mirrored-repeat texel index as the sampler computes it (exposed for tests)
+int or_mirror(int i, int n);
+// bilinear sample of a single-channel double image at texture-space (u, v)
+double or_bilinear(const double *img, int wd, int ht, double u, double v);
+// textureGather component 0 of a ui16 image: (i0j1, i1j1, i1j0, i0j0)
+void or_gather(const uint16_t *img, int wd, int ht, double u, double v, double out[4]);
+// 2x2 symmetric eigen decomposition as the reference computes it
+void or_evd2x2(const double M[4], double eval[2], double evec0[2], double evec1[2]);
+
+// whole-graph restatement of i-raw -> denoise -> demosaic -> colour ->
+// filmcurv -> llap for one uncompressed CFA image (already block rounded,
+// CFA offset normalised).  parameters follow the modules' params files.
+typedef struct or_develop_t
+{
+  // image parameters as i-raw sets them
+  int32_t filters;            // 9 = x-trans, anything else non-zero = bayer rggb
+  float   black[4], white[4]; // raw units (0..65535)
+  float   wb[4];              // camera white balance, g-normalised
+  float   cam_to_rec2020[9];  // row major
+  float   noise_a, noise_b;   // vkdt units (gaussian, poissonian)
+  // denoise params
+  float   dn_strength, dn_luma, dn_detail, dn_edges[4];
+  // demosaic params
+  int32_t dm_colour;
+  // colour params
+  float   co_exposure;
+  float   co_white[3];        // 0,0,0 = use camera coefficients
+  // filmcurv params
+  float   fc_light, fc_contrast, fc_bias;
+  int32_t fc_colour;
+  // llap params
+  float   ll_sigma, ll_shadows, ll_hilights, ll_clarity;
+  int32_t ll_num_gamma;       // curve samples (10 in the reference)
+  int32_t ll_max_levels;      // pyramid depth cap (12 in the reference)
+}
+or_develop_t;
+
+// out_rgba: W*H*4 doubles (rgba f32 output of llap/colour).  stage buffers
+// are optional (may be 0): denoise W*H*1, demosaic/colour/filmcurv W*H*4.
+int or_develop(
+    const uint16_t *cfa, int W, int H,
+    const or_develop_t *p,
+    double *out_rgba,
+    double *st_denoise, double *st_demosaic,
+    double *st_colour, double *st_filmcurv);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,242 @@
+"""The node DAG of the raw-development graph, restated for node-level tests.
+
+Mirrors what the modules' create_nodes build for i-raw -> denoise ->
+demosaic -> colour -> filmcurv -> llap with strength > 0 and the splat
+demosaic (denoise/main.c:133-333, demosaic/main.c:59-203,
+colour/main.c:219-365, filmcurv/main.c:25-51, llap/main.c:14-114), so a test
+can run every node on the GPU through the C ABI and on the oracle with the
+same push-constant and params blobs.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from vkdt_amd import synth
+
+UI16, F16, F32 = 1, 2, 3
+
+# matrices.h: matrix_xyz_to_rec2020 (row major)
+XYZ_TO_REC2020 = np.array([
+    [1.71665119, -0.35567078, -0.25336628],
+    [-0.66668435, 1.61648124, 0.01576855],
+    [0.01763986, -0.04277061, 0.94210312]], dtype=np.float32)
+
+
+def mat3inv_f32(m):
+    """core/mat3.h:21-45 in float32."""
+    A = lambda y, x: np.float32(m[y - 1][x - 1])
+    det = (A(1, 1) * (A(3, 3) * A(2, 2) - A(3, 2) * A(2, 3))
+           - A(2, 1) * (A(3, 3) * A(1, 2) - A(3, 2) * A(1, 3))
+           + A(3, 1) * (A(2, 3) * A(1, 2) - A(2, 2) * A(1, 3)))
+    idet = np.float32(1.0) / det
+    B = np.zeros((3, 3), np.float32)
+    B[0, 0] = idet * (A(3, 3) * A(2, 2) - A(3, 2) * A(2, 3))
+    B[0, 1] = -idet * (A(3, 3) * A(1, 2) - A(3, 2) * A(1, 3))
+    B[0, 2] = idet * (A(2, 3) * A(1, 2) - A(2, 2) * A(1, 3))
+    B[1, 0] = -idet * (A(3, 3) * A(2, 1) - A(3, 1) * A(2, 3))
+    B[1, 1] = idet * (A(3, 3) * A(1, 1) - A(3, 1) * A(1, 3))
+    B[1, 2] = -idet * (A(2, 3) * A(1, 1) - A(2, 1) * A(1, 3))
+    B[2, 0] = idet * (A(3, 2) * A(2, 1) - A(3, 1) * A(2, 2))
+    B[2, 1] = -idet * (A(3, 2) * A(1, 1) - A(3, 1) * A(1, 2))
+    B[2, 2] = idet * (A(2, 2) * A(1, 1) - A(2, 1) * A(1, 2))
+    return B
+
+
+def mat3mul_f32(a, b):
+    out = np.zeros((3, 3), np.float32)
+    for k in range(3):
+        for i in range(3):
+            s = np.float32(0)
+            for j in range(3):
+                s = np.float32(s + np.float32(a[k][j]) * np.float32(b[j][i]))
+            out[k, i] = s
+    return out
+
+
+def cam_to_rec2020(xyz_to_cam):
+    """i-raw/main.c:228-258 for a D65 matrix."""
+    return mat3mul_f32(XYZ_TO_REC2020, mat3inv_f32(np.asarray(xyz_to_cam, np.float32)))
+
+
+@dataclass
+class DevelopParams:
+    kind: str = "bayer"
+    black: float = 512.0
+    white: float = 16383.0
+    wb: tuple = (1.0, 1.0, 1.0, 1.0)
+    noise_a: float = 0.0
+    noise_b: float = 0.0
+    strength: float = 0.5
+    luma: float = 0.6
+    detail: float = 1.0
+    edges: tuple = (0.0, 0.0, 0.0, 0.0)
+    dm_colour: int = 0
+    exposure: float = 0.0
+    co_white: tuple = (0.0, 0.0, 0.0)
+    light: float = 3.0
+    contrast: float = 1.2
+    bias: float = 0.0
+    fc_colour: int = 3
+    sigma: float = 0.12
+    shadows: float = 1.0
+    hilights: float = 1.0
+    clarity: float = 0.2
+    num_gamma: int = 10
+    max_levels: int = 12
+    cam: np.ndarray = field(default_factory=lambda: cam_to_rec2020(synth.xyz_to_cam()))
+
+    @property
+    def filters(self):
+        return synth.XTRANS_FILTERS if self.kind == "xtrans" else synth.BAYER_FILTERS
+
+
+def f32blob(vals):
+    return np.asarray(vals, dtype=np.float32)
+
+
+def mixed_blob(items):
+    """std140 scalar blob from (kind, value) pairs, kind 'f' or 'i'."""
+    out = np.zeros(len(items), np.uint32)
+    for i, (k, v) in enumerate(items):
+        out[i] = np.float32(v).view(np.uint32) if k == "f" else np.uint32(np.int64(v) & 0xffffffff)
+    return out
+
+
+def llap_levels(W, H, max_levels):
+    lw, lh = [W], [H]
+    nl = max_levels
+    for l in range(1, max_levels):
+        lw.append((lw[-1] - 1) // 2 + 1)
+        lh.append((lh[-1] - 1) // 2 + 1)
+        nw, nh = (lw[-1] - 1) // 2 + 1, (lh[-1] - 1) // 2 + 1
+        if nw <= 1 or nh <= 1:
+            nl = l + 1
+            break
+    return nl, lw[:nl], lh[:nl]
+
+
+def colour_params(p):
+    """colour/main.c:219-365 commit_params for matrix=1, mode=0."""
+    cp = np.zeros(242, np.float32)
+    cu = cp.view(np.uint32)
+    pwb = list(np.float32(v) for v in p.co_white)
+    cam = np.asarray(p.cam, np.float32)
+    if pwb[0] == 0 and pwb[1] == 0 and pwb[2] == 0:
+        w = [np.float32(v) for v in p.wb[:3]]
+        w0 = [np.float32(0)] * 3
+        for j in range(3):
+            for i in range(3):
+                w0[j] = np.float32(w0[j] + cam[j, i] / w[i])
+        w0[0] = np.float32(w0[0] / w0[1])
+        w0[2] = np.float32(w0[2] / w0[1])
+        w0[1] = np.float32(1)
+        pwb = [np.float32(1) / w0[0], np.float32(1), np.float32(1) / w0[2]]
+    if not (pwb[0] == pwb[0]) or pwb[0] == 0 or pwb[1] == 0 or pwb[2] == 0:
+        pwb = [np.float32(1)] * 3
+    cp[0] = pwb[0] / pwb[1]
+    cp[1] = 1
+    cp[2] = pwb[2] / pwb[1]
+    cp[3] = np.float32(2.0) ** np.float32(p.exposure)
+    for j in range(3):
+        for i in range(3):
+            cp[4 + 4 * i + j] = cam[j, i]
+    cp[224] = 0.0
+    cp[226] = 1.0
+    cu[229] = 0
+    cu[230] = 0
+    return cp
+
+
+def build(W, H, p):
+    """(buffers, nodes): buffers name -> (wd, ht, chan, fmt, planes);
+    nodes are dicts with module, kernel, pc, params, conn (buffer names), wd, ht, dp."""
+    xt = p.kind == "xtrans"
+    block = 3 if xt else 2
+    Qw, Qh = W // block, H // block
+    black = [np.float32(p.black) / np.float32(65535.0)] * 4
+    white = [np.float32(p.white) / np.float32(65535.0)] * 4
+    wb = list(p.wb)
+    bufs, nodes = {}, []
+
+    def buf(name, wd, ht, chan, fmt, planes=1):
+        bufs[name] = (wd, ht, chan, fmt, planes)
+
+    def add(module, kernel, pc, params, conn, wd, ht, dp=1):
+        nodes.append(dict(module=module, kernel=kernel, pc=pc, params=params, conn=conn, wd=wd, ht=ht, dp=dp))
+
+    buf("raw", W, H, 1, UI16)
+    dpar = mixed_blob([("f", p.strength), ("f", p.luma), ("f", p.detail), ("f", 0.0)]
+                      + [("f", e) for e in p.edges] + [("i", 1)])
+    rgbc = lambda crop: [("f", v) for v in wb] + [("f", v) for v in black] + [("f", v) for v in white] + \
+        [("i", v) for v in crop]
+    buf("half", Qw, Qh, 4, F16)
+    add("denoise", "half", mixed_blob(rgbc([0, 0, W, H]) + [("i", p.filters)]), dpar, ["raw", "half"], Qw, Qh)
+    for l in range(4):
+        buf(f"s{l}", Qw, Qh, 4, F16)
+        pc = mixed_blob(rgbc([0, 0, 0, 0]) + [("f", p.noise_a), ("f", p.noise_b), ("i", l), ("i", block)])
+        add("denoise", "downcov" if l == 0 else "down", pc, dpar, ["half" if l == 0 else f"s{l-1}", f"s{l}"], Qw, Qh)
+    buf("asm", Qw, Qh, 4, F16)
+    pc = mixed_blob(rgbc([0, 0, 0, 0]) + [("f", p.noise_a), ("f", p.noise_b), ("i", p.filters)])
+    add("denoise", "assemble", pc, dpar, ["half", "s0", "s1", "s2", "s3", "asm"], Qw, Qh)
+    buf("dn", W, H, 1, F16)
+    pc = mixed_blob(rgbc([0, 0, W, H]) + [("i", p.filters), ("f", p.noise_a), ("f", p.noise_b), ("i", 0)]
+                    + [("f", 0.0)] * 4)
+    add("denoise", "doub", pc, dpar, ["raw", "asm", "half", "dn"], W, H)
+
+    dmpc = mixed_blob([("f", v) for v in wb] + [("i", p.filters)])
+    dmpar = mixed_blob([("i", p.dm_colour), ("i", 0)])
+    buf("cov", Qw, Qh, 4, F16)
+    add("demosaic", "gauss", dmpc, dmpar, ["dn", "dn", "cov"], Qw, Qh)
+    buf("green", W, H, 1, F16)
+    add("demosaic", "splat", dmpc, dmpar, ["dn", "cov", "green"], W, H)
+    buf("dm", W, H, 4, F16)
+    add("demosaic", "fix", dmpc, dmpar, ["dn", "green", "cov", "dm"], W, H)
+
+    buf("co", W, H, 4, F16)
+    add("colour", "main", mixed_blob([("i", 0)] * 3), colour_params(p), ["dm", "co"], W, H)
+    buf("fc", W, H, 4, F16)
+    fpar = mixed_blob([("f", p.light), ("f", p.contrast), ("f", p.bias), ("i", p.fc_colour), ("f", 1.0),
+                       ("f", 0), ("f", 0), ("f", 0), ("f", 0), ("f", 0)])
+    add("filmcurv", "main", None, fpar, ["co", "fc"], W, H)
+
+    ng = p.num_gamma
+    lpar = f32blob([p.sigma, p.shadows, p.hilights, p.clarity])
+    nl, lw, lh = llap_levels(W, H, p.max_levels)
+    for l in range(nl):
+        buf(f"pyr{l}", lw[l], lh[l], 1, F16, ng + 1)
+    add("llap", "curve", mixed_blob([("i", ng)]), lpar, ["fc", "pyr0"], W, H)
+    for l in range(1, nl):
+        add("llap", "reduce", None, lpar, [f"pyr{l-1}", f"pyr{l}"], lw[l], lh[l], ng + 1)
+    for l in range(nl - 1, 0, -1):
+        buf(f"out{l-1}", lw[l - 1], lh[l - 1], 1, F16)
+        coarse = "pyr0" if l == nl - 1 else f"out{l}"
+        add("llap", "assemble", mixed_blob([("i", ng), ("i", 1 if l == nl - 1 else 0)]), lpar,
+            [coarse, f"pyr{l-1}", f"pyr{l}", f"out{l-1}"], lw[l - 1], lh[l - 1])
+    buf("final", W, H, 4, F32)
+    add("llap", "colour", None, lpar, ["out0", "fc", "final"], W, H)
+    return bufs, nodes
+
+
+def oracle_params(p):
+    """tests.oracle_py.OrDevelop for the same graph."""
+    from oracle_py import OrDevelop
+    o = OrDevelop()
+    o.filters = p.filters
+    for k in range(4):
+        o.black[k] = p.black
+        o.white[k] = p.white
+        o.wb[k] = p.wb[k]
+        o.dn_edges[k] = p.edges[k]
+    cam = np.asarray(p.cam, np.float32).ravel()
+    for k in range(9):
+        o.cam_to_rec2020[k] = float(cam[k])
+    o.noise_a, o.noise_b = p.noise_a, p.noise_b
+    o.dn_strength, o.dn_luma, o.dn_detail = p.strength, p.luma, p.detail
+    o.dm_colour = p.dm_colour
+    o.co_exposure = p.exposure
+    for k in range(3):
+        o.co_white[k] = p.co_white[k]
+    o.fc_light, o.fc_contrast, o.fc_bias, o.fc_colour = p.light, p.contrast, p.bias, p.fc_colour
+    o.ll_sigma, o.ll_shadows, o.ll_hilights, o.ll_clarity = p.sigma, p.shadows, p.hilights, p.clarity
+    o.ll_num_gamma, o.ll_max_levels = p.num_gamma, p.max_levels
+    return o

@@ -1,0 +1,109 @@
+"""ctypes binding of the oracle (oracle/oracle.h) for the tests.
+
+The oracle is the checker: tests compare the HIP path against it, nothing
+else in the repository calls it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle", "_build")
+
+OR_UI16, OR_F16, OR_F32 = 1, 2, 3
+
+
+class OrBuf(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("wd", C.c_int32), ("ht", C.c_int32), ("chan", C.c_int32),
+                ("format", C.c_int32), ("array_length", C.c_int32)]
+
+
+class OrDevelop(C.Structure):
+    _fields_ = [
+        ("filters", C.c_int32),
+        ("black", C.c_float * 4), ("white", C.c_float * 4), ("wb", C.c_float * 4),
+        ("cam_to_rec2020", C.c_float * 9),
+        ("noise_a", C.c_float), ("noise_b", C.c_float),
+        ("dn_strength", C.c_float), ("dn_luma", C.c_float), ("dn_detail", C.c_float), ("dn_edges", C.c_float * 4),
+        ("dm_colour", C.c_int32),
+        ("co_exposure", C.c_float), ("co_white", C.c_float * 3),
+        ("fc_light", C.c_float), ("fc_contrast", C.c_float), ("fc_bias", C.c_float), ("fc_colour", C.c_int32),
+        ("ll_sigma", C.c_float), ("ll_shadows", C.c_float), ("ll_hilights", C.c_float), ("ll_clarity", C.c_float),
+        ("ll_num_gamma", C.c_int32), ("ll_max_levels", C.c_int32),
+    ]
+
+
+_libs = {}
+
+
+def lib(bits=64):
+    if bits not in _libs:
+        path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
+        if not os.path.exists(path):
+            raise ImportError(f"oracle not built: {path}")
+        L = C.CDLL(path)
+        L.or_node.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                              C.POINTER(OrBuf), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_develop.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(OrDevelop), C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_f16_round.argtypes = [C.c_double]
+        L.or_f16_round.restype = C.c_double
+        L.or_mirror.argtypes = [C.c_int, C.c_int]
+        L.or_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.or_bilinear.restype = C.c_double
+        L.or_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        L.or_evd2x2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_set_storage_f16.argtypes = [C.c_int]
+        _libs[bits] = L
+    return _libs[bits]
+
+
+def as_double(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def orbuf(arr, fmt, chan, planes=1):
+    """OrBuf over a numpy array (uint16 for OR_UI16, float64 otherwise)."""
+    if fmt == OR_UI16:
+        assert arr.dtype == np.uint16 and arr.flags.c_contiguous
+    else:
+        assert arr.dtype == np.float64 and arr.flags.c_contiguous
+    if planes > 1:
+        ht, wd = arr.shape[1], arr.shape[2]
+    else:
+        ht, wd = arr.shape[0], arr.shape[1]
+    return OrBuf(arr.ctypes.data, wd, ht, chan, fmt, planes)
+
+
+def node(bits, module, kernel, pc, params, bufs, wd, ht, dp=1, storage_f16=True):
+    L = lib(bits)
+    L.or_set_storage_f16(1 if storage_f16 else 0)
+    conn = (OrBuf * len(bufs))(*bufs)
+    pcb = None if pc is None else np.ascontiguousarray(pc)
+    pab = None if params is None else np.ascontiguousarray(params)
+    rc = L.or_node(module.encode(), kernel.encode(),
+                   None if pcb is None else pcb.ctypes.data, 0 if pcb is None else pcb.nbytes,
+                   None if pab is None else pab.ctypes.data, 0 if pab is None else pab.nbytes,
+                   conn, len(bufs), wd, ht, dp)
+    if rc != 0:
+        raise RuntimeError(f"oracle {module}/{kernel} failed: {rc}")
+
+
+def develop(bits, cfa, p, stages=False, storage_f16=True):
+    """Whole graph on the CPU; returns rgba float64 H x W x 4 (and stages)."""
+    L = lib(bits)
+    L.or_set_storage_f16(1 if storage_f16 else 0)
+    H, W = cfa.shape
+    cfa = np.ascontiguousarray(cfa, dtype=np.uint16)
+    out = np.zeros((H, W, 4), np.float64)
+    st = {}
+    if stages:
+        st = {"denoise": np.zeros((H, W), np.float64), "demosaic": np.zeros((H, W, 4), np.float64),
+              "colour": np.zeros((H, W, 4), np.float64), "filmcurv": np.zeros((H, W, 4), np.float64)}
+    ptr = lambda k: st[k].ctypes.data if k in st else None
+    rc = L.or_develop(cfa.ctypes.data, W, H, C.byref(p), out.ctypes.data,
+                      ptr("denoise"), ptr("demosaic"), ptr("colour"), ptr("filmcurv"))
+    if rc != 0:
+        raise RuntimeError(f"oracle develop failed: {rc}")
+    return (out, st) if stages else out

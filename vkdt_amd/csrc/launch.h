@@ -1,0 +1,72 @@
+// launch.h -- host-side plumbing shared by the node launchers: the decoded
+// call, buffer checks and the thread-local error text.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/vkdt_hip.h"
+#include "common.h"
+
+struct node_call
+{
+  const void       *pc;
+  size_t            pc_size;
+  const void       *params;
+  size_t            params_size;
+  const vkdt_buf_t *conn;
+  int               nconn;
+  uint32_t          wd, ht, dp;
+  hipStream_t       stream;
+};
+
+typedef int (*launch_fn)(const node_call &);
+
+// records the error text for vkdt_hip_last_error() and returns `code`
+int vkdt_set_error(int code, const char *fmt, ...);
+
+// element-pitch device view of a connector
+static inline dimg to_dimg(const vkdt_buf_t &b)
+{
+  dimg d;
+  d.p = b.ptr;
+  d.wd = (int)b.wd;
+  d.ht = (int)b.ht;
+  d.pitch = (int)b.pitch;
+  d.chan = (int)b.chan;
+  const size_t esz = (b.format == VKDT_FMT_F32 ? 4 : 2) * (size_t)b.chan;
+  d.pstride = (long)(b.plane_stride / (esz / b.chan));  // in scalar elements
+  return d;
+}
+
+// host-side validation of one connector before any launch: the kernels
+// index within [0,wd) x [0,ht) x [0,array_length) and rely on these.
+int check_conn(const node_call &c, int idx, uint32_t fmt, uint32_t chan, uint32_t min_planes, const char *what);
+
+// pc/params blob size checks
+int check_blob(const node_call &c, size_t pc_min, size_t params_min, const char *what);
+
+static inline dim3 grid2d(uint32_t wd, uint32_t ht, uint32_t dp)
+{
+  return dim3((wd + BX - 1) / BX, (ht + BY - 1) / BY, dp ? dp : 1);
+}
+
+int launch_post(const char *what);
+
+// launchers, one per reference kernel
+int launch_denoise_noop(const node_call &);
+int launch_denoise_half(const node_call &);
+int launch_denoise_downcov(const node_call &);
+int launch_denoise_down(const node_call &);
+int launch_denoise_assemble(const node_call &);
+int launch_denoise_doub(const node_call &);
+int launch_demosaic_down(const node_call &);
+int launch_demosaic_gauss(const node_call &);
+int launch_demosaic_splat(const node_call &);
+int launch_demosaic_fix(const node_call &);
+int launch_colour_main(const node_call &);
+int launch_filmcurv_main(const node_call &);
+int launch_llap_curve(const node_call &);
+int launch_llap_reduce(const node_call &);
+int launch_llap_assemble(const node_call &);
+int launch_llap_colour(const node_call &);

@@ -1,0 +1,170 @@
+"""ctypes binding of the C ABI in include/vkdt_hip.h (libvkdt_hip.so).
+
+This is the same boundary the host runtime (libvkdt_pipe.so) links against;
+Python uses it for node-level tests and the benchmark.  Loading fails loudly
+when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import lib_path
+
+OK, INCOMPLETE, OOM, DEVICE = 0, -1, -2, -3
+FMT_UI16, FMT_F16, FMT_F32 = 1, 2, 3
+
+
+class Buf(C.Structure):
+    """vkdt_buf_t"""
+    _fields_ = [
+        ("ptr", C.c_void_p),
+        ("wd", C.c_uint32), ("ht", C.c_uint32),
+        ("pitch", C.c_uint32), ("chan", C.c_uint32),
+        ("format", C.c_uint32), ("array_length", C.c_uint32),
+        ("plane_stride", C.c_uint64),
+    ]
+
+
+def token(s):
+    """dt_token(): up to 8 chars packed little endian into a uint64."""
+    b = s.encode()[:8]
+    return int.from_bytes(b.ljust(8, b"\0"), "little")
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = lib_path("libvkdt_hip.so")
+        if not os.path.exists(path):
+            raise ImportError(f"libvkdt_hip.so not built ({path}); run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp, sz, u32, u64, i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
+        L.vkdt_hip_abi_version.restype = i32
+        L.vkdt_hip_init.argtypes = [i32]
+        L.vkdt_hip_node_launch.argtypes = [u64, u64, vp, sz, vp, sz, C.POINTER(Buf), i32, u32, u32, u32, vp]
+        L.vkdt_hip_node_supported.argtypes = [u64, u64]
+        L.vkdt_hip_alloc.argtypes = [C.POINTER(vp), sz]
+        L.vkdt_hip_free.argtypes = [vp]
+        L.vkdt_hip_host_alloc.argtypes = [C.POINTER(vp), sz]
+        L.vkdt_hip_host_free.argtypes = [vp]
+        L.vkdt_hip_upload.argtypes = [vp, vp, sz, vp]
+        L.vkdt_hip_download.argtypes = [vp, vp, sz, vp]
+        L.vkdt_hip_stream_create.argtypes = [C.POINTER(vp)]
+        L.vkdt_hip_stream_destroy.argtypes = [vp]
+        L.vkdt_hip_stream_sync.argtypes = [vp]
+        L.vkdt_hip_event_create.argtypes = [C.POINTER(vp)]
+        L.vkdt_hip_event_destroy.argtypes = [vp]
+        L.vkdt_hip_event_record.argtypes = [vp, vp]
+        L.vkdt_hip_event_elapsed_ms.argtypes = [C.POINTER(C.c_float), vp, vp]
+        L.vkdt_hip_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != OK:
+        msg = lib().vkdt_hip_last_error().decode(errors="replace")
+        raise HipError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def init(dev=0):
+    check(lib().vkdt_hip_init(dev), "vkdt_hip_init")
+
+
+class DeviceBuffer:
+    """A device allocation with the texel layout of one connector."""
+
+    def __init__(self, wd, ht, chan, fmt, planes=1):
+        self.wd, self.ht, self.chan, self.fmt, self.planes = wd, ht, chan, fmt, planes
+        self.esize = (4 if fmt == FMT_F32 else 2) * chan
+        self.plane_bytes = wd * ht * self.esize
+        self.nbytes = self.plane_bytes * planes
+        p = C.c_void_p()
+        check(lib().vkdt_hip_alloc(C.byref(p), self.nbytes), "alloc")
+        self.ptr = p.value
+
+    def buf(self):
+        return Buf(self.ptr, self.wd, self.ht, self.wd, self.chan, self.fmt, self.planes, self.plane_bytes)
+
+    def dtype(self):
+        return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32}[self.fmt]
+
+    def shape(self):
+        s = (self.planes, self.ht, self.wd) if self.planes > 1 else (self.ht, self.wd)
+        return s + ((self.chan,) if self.chan > 1 else ())
+
+    def upload(self, arr, stream=None):
+        a = np.ascontiguousarray(arr, dtype=self.dtype())
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().vkdt_hip_upload(self.ptr, a.ctypes.data, a.nbytes, stream), "upload")
+        sync(stream)
+
+    def download(self, stream=None):
+        a = np.empty(self.shape(), dtype=self.dtype())
+        check(lib().vkdt_hip_download(a.ctypes.data, self.ptr, a.nbytes, stream), "download")
+        sync(stream)
+        return a
+
+    def free(self):
+        if self.ptr:
+            check(lib().vkdt_hip_free(self.ptr), "free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sync(stream=None):
+    check(lib().vkdt_hip_stream_sync(stream), "sync")
+
+
+def launch(module, kernel, pc, params, bufs, wd, ht, dp=1, stream=None):
+    """vkdt_hip_node_launch with numpy blobs and DeviceBuffer/Buf connectors."""
+    conn = (Buf * max(1, len(bufs)))()
+    for i, b in enumerate(bufs):
+        conn[i] = b.buf() if isinstance(b, DeviceBuffer) else (b if b is not None else Buf())
+    pcb = np.ascontiguousarray(pc) if pc is not None else None
+    pab = np.ascontiguousarray(params) if params is not None else None
+    rc = lib().vkdt_hip_node_launch(
+        token(module), token(kernel),
+        pcb.ctypes.data if pcb is not None else None, pcb.nbytes if pcb is not None else 0,
+        pab.ctypes.data if pab is not None else None, pab.nbytes if pab is not None else 0,
+        conn, len(bufs), wd, ht, dp, stream)
+    check(rc, f"{module}/{kernel}")
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().vkdt_hip_event_create(C.byref(p)), "event")
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        check(lib().vkdt_hip_event_record(self.ptr, stream), "record")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(lib().vkdt_hip_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr), "elapsed")
+        return ms.value
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().vkdt_hip_stream_create(C.byref(p)), "stream")
+        self.ptr = p.value
+
+    def sync(self):
+        sync(self.ptr)

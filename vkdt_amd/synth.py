@@ -1,0 +1,213 @@
+"""Seeded synthetic raw scenes and the uncompressed DNGs that carry them.
+
+Scene model (the workloads of the raw-development benchmark):
+  * a smooth background whose luminance spans ``stops`` stops log-uniformly,
+    with smoothly varying per-channel gains,
+  * ``nrect`` axis-aligned rectangles of log-uniform radiance over the same
+    range and random colour,
+  * scaled so that ``clip_frac`` of the CFA sites exceed the white level,
+  * Poisson-Gaussian noise on the white-normalised signal x:
+    var = a * x + b, then quantised to the raw integer range and clipped
+    at [0, white].
+
+vkdt's noise model evaluates sigma = sqrt(noise_a + v * noise_b) on
+black-subtracted UNORM16 values (denoise/noise.glsl:9) with noise_a the
+gaussian and noise_b the poissonian term (i-raw/readme.md:9-10), so the
+matching graph parameters are ``noise_params(a, b, black, white)``.
+"""
+import numpy as np
+
+# canonical x-trans layout after i-raw's offset normalisation
+# (demosaic/splat.comp:103-110): 0 red, 1 green, 2 blue
+XTRANS = np.array([
+    [1, 0, 1, 1, 2, 1],
+    [2, 1, 2, 0, 1, 0],
+    [1, 0, 1, 1, 2, 1],
+    [1, 2, 1, 1, 0, 1],
+    [0, 1, 0, 2, 1, 2],
+    [1, 2, 1, 1, 0, 1],
+], dtype=np.uint8)
+BAYER_RGGB = np.array([[0, 1], [1, 2]], dtype=np.uint8)
+
+BAYER_FILTERS = 0x49494949   # i-raw's "any bayer" marker after RGGB normalisation
+XTRANS_FILTERS = 9
+
+# fixed camera -> XYZ(D65) matrix of the synthetic camera (row major); the
+# DNG stores its inverse as ColorMatrix1 with CalibrationIlluminant1 = D65
+CAM_TO_XYZ = np.array([
+    [0.6280, 0.2310, 0.1020],
+    [0.2650, 0.6850, 0.0500],
+    [0.0230, 0.1180, 0.9550],
+], dtype=np.float64)
+
+
+def xyz_to_cam():
+    return np.linalg.inv(CAM_TO_XYZ)
+
+
+def cfa_pattern(kind):
+    return XTRANS if kind == "xtrans" else BAYER_RGGB
+
+
+def channel_map(W, H, kind):
+    """CFA colour (0 r, 1 g, 2 b) of every site, uint8 H x W."""
+    p = cfa_pattern(kind)
+    n = p.shape[0]
+    reps = (-(-H // n), -(-W // n))
+    return np.tile(p, reps)[:H, :W]
+
+
+def noise_params(a, b, black=512.0, white=16383.0):
+    """(noise_a, noise_b) in vkdt units for var = a*x + b on white-normalised x."""
+    s = (white - black) / 65535.0
+    return b * s * s, a * s * s
+
+
+def radiance(W, H, seed, kind="bayer", nrect=200, stops=12.0, origin=(0, 0), canvas=None):
+    """Noise-free CFA radiance (float32 H x W, arbitrary scale).
+
+    ``origin``/``canvas`` place the W x H frame inside a larger scene of
+    extent ``canvas`` (for video streams that drift across one scene)."""
+    rng = np.random.default_rng(seed)
+    CW, CH = canvas if canvas is not None else (W, H)
+    ox, oy = origin
+    xs = (np.arange(W, dtype=np.float32) + ox) / np.float32(CW)
+    ys = (np.arange(H, dtype=np.float32) + oy) / np.float32(CH)
+    ph = rng.uniform(0, 2 * np.pi, size=6).astype(np.float32)
+    u = (0.45 * xs[None, :] + 0.35 * ys[:, None]
+         + 0.1 * np.sin(3.1 * xs[None, :] + ph[0]) * np.cos(2.3 * ys[:, None] + ph[1]))
+    u = (u - u.min()) / max(float(u.max() - u.min()), 1e-6)
+    lum = np.exp2(-stops * (1.0 - u)).astype(np.float32)
+    chan = channel_map(W, H, kind)
+    gains = np.stack([
+        0.8 + 0.3 * np.sin(2.0 * xs[None, :] + ph[2]) * np.cos(1.7 * ys[:, None] + ph[3]),
+        np.ones((1, 1), np.float32) + 0 * xs[None, :] + 0 * ys[:, None],
+        0.8 + 0.3 * np.cos(1.3 * xs[None, :] + ph[4]) * np.sin(2.9 * ys[:, None] + ph[5]),
+    ]).astype(np.float32)
+    rad = lum * np.take_along_axis(gains, chan[None].astype(np.intp), 0)[0]
+    for _ in range(nrect):
+        rw = int(rng.uniform(0.03, 0.25) * CW)
+        rh = int(rng.uniform(0.03, 0.25) * CH)
+        x0 = int(rng.uniform(0, CW - rw)) - ox
+        y0 = int(rng.uniform(0, CH - rh)) - oy
+        level = np.float32(np.exp2(-stops * rng.uniform(0.0, 1.0)))
+        col = rng.uniform(0.25, 1.0, size=3).astype(np.float32)
+        col /= col.max()
+        xa, xb = max(x0, 0), min(x0 + rw, W)
+        ya, yb = max(y0, 0), min(y0 + rh, H)
+        if xa >= xb or ya >= yb:
+            continue
+        rad[ya:yb, xa:xb] = level * col[chan[ya:yb, xa:xb]]
+    return rad
+
+
+def mosaic(W, H, seed, kind="bayer", a=2e-4, b=1e-6, black=512, white=16383,
+           nrect=200, stops=12.0, clip_frac=0.02, origin=(0, 0), canvas=None, scale=None):
+    """uint16 CFA of a seeded scene with Poisson-Gaussian noise.
+
+    Returns (cfa, scale) where ``scale`` is the radiance that maps to the
+    white level (pass it back in to keep exposure fixed across video frames)."""
+    rad = radiance(W, H, seed, kind, nrect, stops, origin, canvas)
+    if scale is None:
+        k = int((1.0 - clip_frac) * rad.size)
+        scale = float(np.partition(rad.ravel(), k)[k])
+    x = rad / np.float32(scale)
+    rng = np.random.default_rng(seed ^ 0x5EED5EED)
+    noise = rng.standard_normal(size=x.shape, dtype=np.float32)
+    x = x + noise * np.sqrt(np.float32(a) * np.maximum(x, 0) + np.float32(b))
+    raw = np.rint(black + x * np.float32(white - black))
+    return np.clip(raw, 0, white).astype(np.uint16), scale
+
+
+# ---------------------------------------------------------------------------
+# uncompressed DNG writer (little endian TIFF, one raw IFD)
+# ---------------------------------------------------------------------------
+
+def _rational(v, signed):
+    from fractions import Fraction
+    f = Fraction(float(v)).limit_denominator(1_000_000)
+    if not signed and f < 0:
+        raise ValueError("negative unsigned rational")
+    return f.numerator, f.denominator
+
+
+def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(1.0, 1.0, 1.0),
+              color_matrix=None, make="vkdt-synth", model="synthetic"):
+    """Write ``cfa`` (uint16 H x W) as an uncompressed CFA DNG."""
+    import struct
+    H, W = cfa.shape
+    pat = cfa_pattern(kind)
+    n = pat.shape[0]
+    cm = xyz_to_cam() if color_matrix is None else np.asarray(color_matrix, np.float64)
+    entries = []   # (tag, type, count, payload bytes)
+
+    def add(tag, typ, values):
+        fmt = {1: "B", 2: "s", 3: "H", 4: "I", 5: "II", 7: "B", 10: "ii", 12: "d"}[typ]
+        if typ == 2:
+            data = values.encode() + b"\0"
+            cnt = len(data)
+        elif typ in (5, 10):
+            data = b"".join(struct.pack("<" + fmt, *_rational(v, typ == 10)) for v in values)
+            cnt = len(values)
+        else:
+            data = struct.pack("<%d%s" % (len(values), fmt), *values)
+            cnt = len(values)
+        entries.append((tag, typ, cnt, data))
+
+    strip_bytes = W * H * 2
+    add(254, 4, [0])                      # NewSubFileType: main image
+    add(256, 4, [W])                      # ImageWidth
+    add(257, 4, [H])                      # ImageLength
+    add(258, 3, [16])                     # BitsPerSample
+    add(259, 3, [1])                      # Compression: none
+    add(262, 3, [32803])                  # PhotometricInterpretation: CFA
+    add(271, 2, make)                     # Make
+    add(272, 2, model)                    # Model
+    add(273, 4, [0])                      # StripOffsets (patched)
+    add(277, 3, [1])                      # SamplesPerPixel
+    add(278, 4, [H])                      # RowsPerStrip
+    add(279, 4, [strip_bytes])            # StripByteCounts
+    add(284, 3, [1])                      # PlanarConfiguration
+    add(33421, 3, [n, n])                 # CFARepeatPatternDim
+    add(33422, 1, [int(v) for v in pat.ravel()])  # CFAPattern
+    add(50706, 1, [1, 4, 0, 0])           # DNGVersion
+    add(50708, 2, f"{make} {model}")      # UniqueCameraModel
+    add(50714, 4, [int(black)])           # BlackLevel
+    add(50717, 4, [int(white)])           # WhiteLevel
+    add(50721, 10, [float(v) for v in cm.ravel()])  # ColorMatrix1
+    add(50728, 5, [float(v) for v in as_shot_neutral])  # AsShotNeutral
+    add(50778, 3, [21])                   # CalibrationIlluminant1: D65
+    entries.sort(key=lambda e: e[0])
+
+    ifd_off = 8
+    ifd_size = 2 + 12 * len(entries) + 4
+    data_off = ifd_off + ifd_size
+    blobs = []
+    cur = data_off
+    fixed = []
+    for tag, typ, cnt, data in entries:
+        if len(data) > 4:
+            pad = cur & 1
+            cur += pad
+            blobs.append((cur, data))
+            fixed.append((tag, typ, cnt, None, cur))
+            cur += len(data)
+        else:
+            fixed.append((tag, typ, cnt, data.ljust(4, b"\0"), None))
+    strip_off = (cur + 15) & ~15
+    out = bytearray(b"II*\0" + struct.pack("<I", ifd_off))
+    out += struct.pack("<H", len(fixed))
+    for tag, typ, cnt, inline, off in fixed:
+        if tag == 273:
+            inline = struct.pack("<I", strip_off)
+        out += struct.pack("<HHI", tag, typ, cnt)
+        out += inline if inline is not None else struct.pack("<I", off)
+    out += struct.pack("<I", 0)
+    for off, data in blobs:
+        out += b"\0" * (off - len(out))
+        out += data
+    out += b"\0" * (strip_off - len(out))
+    with open(path, "wb") as f:
+        f.write(out)
+        f.write(np.ascontiguousarray(cfa, dtype="<u2").tobytes())
+    return path
