@@ -79,7 +79,7 @@ def build_pipe():
     if os.path.exists(cli):
         os.makedirs(BIN, exist_ok=True)
         exe = os.path.join(BIN, "vkdt-cli")
-        run(["gcc"] + C_FLAGS + inc + ["-o", exe + ".tmp", cli, "-L" + LIB, "-lvkdt_pipe",
+        run(["gcc"] + C_FLAGS + inc + ["-o", exe + ".tmp", cli, "-L" + LIB, "-lvkdt_pipe", "-lvkdt_hip",
              "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
         os.replace(exe + ".tmp", exe)
     return out

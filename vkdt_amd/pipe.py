@@ -1,0 +1,217 @@
+"""ctypes binding of the host graph runtime (libvkdt_pipe.so).
+
+A Graph reads a vkdt .cfg, replaces its display by an output module and
+runs it: the same code path as ``vkdt_amd/bin/vkdt-cli``.  Node-level
+accessors expose the DAG the modules' create_nodes built (module/kernel
+tokens, push constants, connectors) for tests and the benchmark.
+"""
+import ctypes as C
+import os
+
+from . import PKG_DIR, lib_path
+from .hip import Buf, check as hip_check, lib as hip_lib
+
+RUN_ROI, RUN_CREATE, RUN_ALLOC, RUN_RECORD, RUN_UPLOAD, RUN_DOWNLOAD, RUN_WAIT = (1 << i for i in range(7))
+RUN_ALL = 0xffffffff
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = lib_path("libvkdt_pipe.so")
+        if not os.path.exists(path):
+            raise ImportError(f"libvkdt_pipe.so not built ({path}); run __graft_entry__.build()")
+        hip_lib()   # the runtime links the C ABI library
+        L = C.CDLL(path)
+        vp, i32, u32, u64, cp = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_char_p
+        L.dt_pipe_global_init.argtypes = [cp]
+        L.dt_graph_new.restype = vp
+        L.dt_graph_free.argtypes = [vp]
+        L.dt_graph_read_config_ascii.argtypes = [vp, cp]
+        L.dt_graph_read_config_line.argtypes = [vp, cp]
+        L.dt_graph_run.argtypes = [vp, u32]
+        L.dt_graph_plan.argtypes = [vp, u32]
+        L.dt_graph_replace_display.argtypes = [vp, u64, u64]
+        L.dt_graph_disconnect_display_modules.argtypes = [vp]
+        L.dt_graph_last_error.argtypes = [vp]
+        L.dt_graph_last_error.restype = cp
+        L.dt_graph_set_perf.argtypes = [vp, i32]
+        L.dt_graph_set_skip_dead.argtypes = [vp, i32]
+        L.dt_graph_num_nodes.argtypes = [vp]
+        L.dt_graph_num_exec.argtypes = [vp]
+        L.dt_graph_exec_node.argtypes = [vp, i32]
+        L.dt_graph_exec_ms.argtypes = [vp, i32]
+        L.dt_graph_exec_ms.restype = C.c_float
+        L.dt_graph_stream.argtypes = [vp]
+        L.dt_graph_stream.restype = vp
+        L.dt_graph_arena_bytes.argtypes = [vp]
+        L.dt_graph_arena_bytes.restype = u64
+        L.dt_graph_node_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32),
+                                         C.POINTER(vp), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)]
+        L.dt_graph_node_params.argtypes = [vp, i32, C.POINTER(u64)]
+        L.dt_graph_node_params.restype = vp
+        L.dt_graph_node_conn.argtypes = [vp, i32, i32, C.POINTER(u64), C.POINTER(i32), C.POINTER(Buf)]
+        L.dt_graph_module.argtypes = [vp, cp, cp]
+        L.dt_graph_set_param.argtypes = [vp, i32, cp, cp]
+        L.dt_graph_node_buffer.argtypes = [vp, i32, i32]
+        L.dt_graph_node_buffer.restype = vp
+        f4, u4 = C.c_float * 4, C.c_uint32 * 4
+        L.dt_graph_img_param_get.argtypes = [vp, i32, f4, f4, f4, C.c_float * 9, C.POINTER(u32), u4,
+                                             C.c_float * 2, C.c_uint32 * 2]
+        L.dt_graph_commit.argtypes = [vp]
+        rc = L.dt_pipe_global_init(PKG_DIR.encode())
+        if rc != 0:
+            raise ImportError(f"no module classes found under {PKG_DIR}/modules")
+        _lib = L
+    return _lib
+
+
+def tok_str(t):
+    return int(t).to_bytes(8, "little").rstrip(b"\0").decode(errors="replace")
+
+
+def token(s):
+    return int.from_bytes(s.encode()[:8].ljust(8, b"\0"), "little")
+
+
+class PipeError(RuntimeError):
+    pass
+
+
+class Graph:
+    def __init__(self, cfg=None, lines=None):
+        self.L = lib()
+        self.g = self.L.dt_graph_new()
+        if not self.g:
+            raise PipeError("dt_graph_new failed")
+        if cfg is not None and self.L.dt_graph_read_config_ascii(self.g, cfg.encode()):
+            raise PipeError(f"reading {cfg}: {self.error()}")
+        for l in lines or []:
+            if self.L.dt_graph_read_config_line(self.g, l.encode()) < 0:
+                raise PipeError(f"cfg line '{l}': {self.error()}")
+
+    def error(self):
+        return self.L.dt_graph_last_error(self.g).decode(errors="replace")
+
+    def export(self, output_module="o-pfm", filename=None, inst="main"):
+        m = self.L.dt_graph_replace_display(self.g, token(inst), token(output_module))
+        if m < 0:
+            raise PipeError(f"replace display:{inst} failed ({m}): {self.error()}")
+        self.L.dt_graph_disconnect_display_modules(self.g)
+        if filename is not None:
+            self.set_param(output_module, inst, "filename", filename)
+        return m
+
+    def set_param(self, module, inst, param, value):
+        m = self.L.dt_graph_module(self.g, module.encode(), inst.encode())
+        if m < 0:
+            raise PipeError(f"no module {module}:{inst}")
+        if isinstance(value, (list, tuple)):
+            value = ":".join(str(v) for v in value)
+        rc = self.L.dt_graph_set_param(self.g, m, param.encode(), str(value).encode())
+        if rc != 0:
+            raise PipeError(f"set {module}:{inst}:{param}={value} failed ({rc})")
+
+    def img_param(self, module, inst):
+        """metadata the module publishes after the roi pass"""
+        m = self.L.dt_graph_module(self.g, module.encode(), inst.encode())
+        if m < 0:
+            raise PipeError(f"no module {module}:{inst}")
+        f4 = C.c_float * 4
+        black, white, wb, cam = f4(), f4(), f4(), (C.c_float * 9)()
+        filters, crop, noise, roi = C.c_uint32(), (C.c_uint32 * 4)(), (C.c_float * 2)(), (C.c_uint32 * 2)()
+        self.L.dt_graph_img_param_get(self.g, m, black, white, wb, cam, C.byref(filters), crop, noise, roi)
+        return dict(black=list(black), white=list(white), wb=list(wb), cam=list(cam), filters=filters.value,
+                    crop=list(crop), noise=list(noise), full=(roi[0], roi[1]))
+
+    def commit(self):
+        self.L.dt_graph_commit(self.g)
+
+    def plan(self, run=RUN_ALL):
+        if self.L.dt_graph_plan(self.g, run):
+            raise PipeError(f"plan failed: {self.error()}")
+
+    def run(self, run=RUN_ALL):
+        rc = self.L.dt_graph_run(self.g, run)
+        if rc:
+            raise PipeError(f"run failed ({rc}): {self.error()}")
+
+    def set_perf(self, on=True):
+        self.L.dt_graph_set_perf(self.g, 1 if on else 0)
+
+    def set_skip_dead(self, on=True):
+        self.L.dt_graph_set_skip_dead(self.g, 1 if on else 0)
+
+    def stream(self):
+        return self.L.dt_graph_stream(self.g)
+
+    def arena_bytes(self):
+        return self.L.dt_graph_arena_bytes(self.g)
+
+    def exec_nodes(self):
+        return [self.L.dt_graph_exec_node(self.g, k) for k in range(self.L.dt_graph_num_exec(self.g))]
+
+    def exec_ms(self):
+        return [self.L.dt_graph_exec_ms(self.g, k) for k in range(self.L.dt_graph_num_exec(self.g))]
+
+    def node(self, n):
+        name, kern = C.c_uint64(), C.c_uint64()
+        dim = (C.c_uint32 * 3)()
+        pc, pcs = C.c_void_p(), C.c_uint64()
+        nconn, culled = C.c_int(), C.c_int()
+        if self.L.dt_graph_node_info(self.g, n, C.byref(name), C.byref(kern), dim, C.byref(pc), C.byref(pcs),
+                                     C.byref(nconn), C.byref(culled)):
+            raise PipeError(f"no node {n}")
+        psz = C.c_uint64()
+        pp = self.L.dt_graph_node_params(self.g, n, C.byref(psz))
+        conns = []
+        for c in range(nconn.value):
+            tk = (C.c_uint64 * 4)()
+            link = (C.c_int * 2)()
+            b = Buf()
+            self.L.dt_graph_node_conn(self.g, n, c, tk, link, C.byref(b))
+            conns.append(dict(name=tok_str(tk[0]), type=tok_str(tk[1]), chan=tok_str(tk[2]),
+                              format=tok_str(tk[3]), link=(link[0], link[1]), buf=b))
+        return dict(id=n, module=tok_str(name.value), kernel=tok_str(kern.value), wd=dim[0], ht=dim[1], dp=dim[2],
+                    pc=C.string_at(pc.value, pcs.value) if pcs.value else b"",
+                    params=C.string_at(pp, psz.value) if pp and psz.value else b"",
+                    conns=conns, culled=bool(culled.value))
+
+    def nodes(self):
+        return [self.node(n) for n in range(self.L.dt_graph_num_nodes(self.g))]
+
+    def close(self):
+        if self.g:
+            self.L.dt_graph_free(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def raw_graph_cfg(dng, strength=0.5, noise_a=0.0, noise_b=0.0, clarity=0.2, numgamma=10, levels=12,
+                  extra=()):
+    """cfg lines of the raw-development graph i-raw -> ... -> llap -> display."""
+    lines = [
+        "module:i-raw:main", "module:denoise:01", "module:demosaic:01", "module:colour:01",
+        "module:filmcurv:01", "module:llap:01", "module:display:main",
+        "connect:i-raw:main:output:denoise:01:input",
+        "connect:denoise:01:output:demosaic:01:input",
+        "connect:demosaic:01:output:colour:01:input",
+        "connect:colour:01:output:filmcurv:01:input",
+        "connect:filmcurv:01:output:llap:01:input",
+        "connect:llap:01:output:display:main:input",
+        f"param:i-raw:main:filename:{dng}",
+        f"param:i-raw:main:noise a:{noise_a!r}",
+        f"param:i-raw:main:noise b:{noise_b!r}",
+        f"param:denoise:01:strength:{strength}",
+        f"param:llap:01:clarity:{clarity}",
+        f"param:llap:01:numgamma:{numgamma}",
+        f"param:llap:01:levels:{levels}",
+    ]
+    return lines + list(extra)

@@ -1,0 +1,89 @@
+/*
+ * vkdt-cli for the raw-development path: read a .cfg graph, replace its
+ * display:main by an output module, run it on the GPU and write the file.
+ *
+ * Arguments as src/cli/main.c:37-159 for what the path supports:
+ *   -g <graph.cfg>            graph to process
+ *   --format o-pfm            output module (o-pfm only)
+ *   --filename <base>         output file name without extension
+ *   --colour-prim bt2020      only linear rec2020 output is produced
+ *   --colour-trc linear
+ *   -d perf                   per-node and total GPU time
+ *   --device <n>              HIP device
+ */
+#include "api.h"
+#include "vkdt_hip.h"
+
+#include <libgen.h>
+#include <stdlib.h>
+#include <time.h>
+#include <unistd.h>
+
+static double now_ms(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+int main(int argc, char *argv[])
+{
+  const char *cfg = 0, *fmt = "o-pfm", *fname = "output", *prim = "bt2020", *trc = "linear";
+  int perf = 0, dev = 0;
+  for(int i = 1; i < argc; i++)
+  {
+    if(!strcmp(argv[i], "-g") && i + 1 < argc) cfg = argv[++i];
+    else if(!strcmp(argv[i], "--format") && i + 1 < argc) fmt = argv[++i];
+    else if(!strcmp(argv[i], "--filename") && i + 1 < argc) fname = argv[++i];
+    else if(!strcmp(argv[i], "--colour-prim") && i + 1 < argc) prim = argv[++i];
+    else if(!strcmp(argv[i], "--colour-trc") && i + 1 < argc) trc = argv[++i];
+    else if(!strcmp(argv[i], "-d") && i + 1 < argc) { if(!strcmp(argv[++i], "perf") || !strcmp(argv[i], "all")) perf = 1; }
+    else if(!strcmp(argv[i], "--device") && i + 1 < argc) dev = atoi(argv[++i]);
+    else if(!strcmp(argv[i], "--version")) { printf("vkdt-cli (raw-development path, HIP gfx950)\n"); return 0; }
+  }
+  if(!cfg)
+  {
+    fprintf(stderr, "usage: %s -g <graph.cfg> [--format o-pfm] [--filename <base>] [-d perf]\n", argv[0]);
+    return 1;
+  }
+  if(strcmp(fmt, "o-pfm")) { fprintf(stderr, "[cli] output format %s not supported (o-pfm only)\n", fmt); return 1; }
+  if(strcmp(prim, "bt2020") || strcmp(trc, "linear"))
+  { fprintf(stderr, "[cli] only --colour-prim bt2020 --colour-trc linear is supported\n"); return 1; }
+
+  char exe[4096] = {0}, base[4096];
+  if(readlink("/proc/self/exe", exe, sizeof(exe) - 1) <= 0) snprintf(exe, sizeof(exe), "%s", argv[0]);
+  snprintf(base, sizeof(base), "%s/..", dirname(exe));   // <pkg>/bin/.. holds modules/
+  if(dt_pipe_global_init(base)) { fprintf(stderr, "[cli] no modules under %s\n", base); return 1; }
+  if(vkdt_hip_init(dev)) { fprintf(stderr, "[cli] %s\n", vkdt_hip_last_error()); return 1; }
+
+  dt_graph_t *g = malloc(sizeof(dt_graph_t));
+  if(!g || dt_graph_init(g)) return 1;
+  g->perf = perf;
+  int ret = 1;
+  if(dt_graph_read_config_ascii(g, cfg)) goto out;
+  const int m = dt_graph_replace_display(g, dt_token("main"), dt_token("o-pfm"));
+  if(m < 0) { fprintf(stderr, "[cli] graph has no connected display:main\n"); goto out; }
+  dt_graph_disconnect_display_modules(g);
+  dt_module_set_param_string(g->module + m, dt_token("filename"), fname);
+  const double t0 = now_ms();
+  if(dt_graph_run(g, s_graph_run_all)) { fprintf(stderr, "[cli] run failed: %s\n", dt_graph_last_error(g)); goto out; }
+  const double t1 = now_ms();
+  if(perf)
+  {
+    float total = 0.0f;
+    for(int k = 0; k < g->num_exec; k++)
+    {
+      const dt_node_t *n = g->node + g->exec[k];
+      total += g->node_ms[k];
+      fprintf(stderr, "[perf] %" PRItkn "\t%" PRItkn "\t: %8.3f ms\n", dt_token_str(n->name), dt_token_str(n->kernel),
+              g->node_ms[k]);
+    }
+    fprintf(stderr, "[perf] total time:\t%8.3f ms (end to end %.1f ms)\n", total, t1 - t0);
+  }
+  ret = 0;
+out:
+  dt_graph_cleanup(g);
+  free(g);
+  dt_pipe_global_cleanup();
+  return ret;
+}
