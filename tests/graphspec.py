@@ -140,10 +140,14 @@ def colour_params(p):
     for j in range(3):
         for i in range(3):
             cp[4 + 4 * i + j] = cam[j, i]
-    cp[224] = 0.0
-    cp[226] = 1.0
-    cu[229] = 0
-    cu[230] = 0
+    cp[224] = 0.0          # clut anchor from the Kelvin slider: unused by the matrix path
+    cp[226] = 1.0          # saturation
+    cu[229] = 0            # primaries: custom (camera matrix from i-raw)
+    cu[230] = 0            # trc: linear
+    w = [np.float32(v) for v in p.wb[:3]]
+    if not (w[0] > 0 and w[1] > 0 and w[2] > 0):
+        w = [np.float32(1)] * 3
+    cp[232:236] = [w[0] / w[1], 1.0, w[2] / w[1], 1.0]   # as-shot wb, g-normalised
     return cp
 
 

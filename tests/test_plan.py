@@ -1,0 +1,121 @@
+"""The host runtime on the CPU: cfg parsing, module -> node expansion, roi
+passes, culling and the push-constant / uniform blobs every node hands to
+the HIP C ABI, compared with an independent restatement of the modules'
+create_nodes (graphspec).  No device is touched (dt_graph_plan)."""
+import numpy as np
+import pytest
+
+import graphspec as gs
+from vkdt_amd import pipe, synth
+
+
+def planned(tmp_path, kind, W, H, extra=()):
+    cfa, _ = synth.mosaic(W, H, 1, kind)
+    path = str(tmp_path / "p.dng")
+    synth.write_dng(path, cfa, kind, as_shot_neutral=(0.45, 1.0, 0.7))
+    na, nb = synth.noise_params(2e-4, 1e-6)
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(path, noise_a=na, noise_b=nb, extra=extra))
+    g.export("o-pfm", str(tmp_path / "out"))
+    g.plan()
+    g.commit()
+    return g, np.float32(na), np.float32(nb)
+
+
+@pytest.mark.parametrize("kind,W,H", [("bayer", 200, 150), ("xtrans", 201, 150)])
+def test_nodes_match_restatement(tmp_path, kind, W, H):
+    g, na, nb = planned(tmp_path, kind, W, H)
+    ip = g.img_param("i-raw", "main")
+    W, H = ip["full"]
+    p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb, cam=np.array(ip["cam"], np.float32).reshape(3, 3),
+                         wb=tuple(ip["wb"][:3]) + (ip["wb"][3],))
+    _, ref = gs.build(W, H, p)
+    nodes = {n["id"]: n for n in g.nodes()}
+    live = [nodes[k] for k in g.exec_nodes() if nodes[k]["module"] not in ("i-raw", "o-pfm")]
+    assert [(n["module"], n["kernel"]) for n in live] == [(r["module"], r["kernel"]) for r in ref]
+    for n, r in zip(live, ref):
+        what = f"{n['module']}/{n['kernel']}"
+        assert (n["wd"], n["ht"], n["dp"]) == (r["wd"], r["ht"], r["dp"]), what
+        if r["pc"] is not None:
+            rb = np.ascontiguousarray(r["pc"]).tobytes()
+            assert n["pc"][:len(rb)] == rb, what
+        if r["params"] is not None:
+            rb = np.ascontiguousarray(r["params"]).tobytes()
+            nb = n["params"][:len(rb)]
+            if n["module"] == "colour":
+                # word 224 is the clut anchor of the Kelvin slider (tanf/asinhf of the
+                # temp param), unused by the matrix path: not restated
+                rb, nb = rb[:224 * 4] + rb[225 * 4:], nb[:224 * 4] + nb[225 * 4:]
+            assert nb == rb, what
+
+
+def test_culling_and_dead_stores(tmp_path):
+    g, _, _ = planned(tmp_path, "bayer", 64, 48)
+    nodes = g.nodes()
+    culled = {(n["module"], n["kernel"]) for n in nodes if n["culled"]}
+    # filmcurv's histogram and curve display feed nothing; demosaic/down feeds an unsampled binding
+    assert culled == {("OpenDRT", "hist"), ("filmcurv", "dspy"), ("demosaic", "down")}
+    dc = [n for n in nodes if n["kernel"] == "downcov"][0]
+    assert dc["conns"][2]["name"] == "cov"
+    # llap pyramid (llap/main.c:57-97): a level is added while the one after it
+    # stays wider and higher than 1 texel
+    red = [n for n in nodes if n["kernel"] == "reduce"]
+    assert [(n["wd"], n["ht"]) for n in red] == [(32, 24), (16, 12), (8, 6), (4, 3), (2, 2)]
+    assert all(n["dp"] == 11 for n in red)
+
+
+def test_llap_parameters_change_the_pyramid(tmp_path):
+    g, _, _ = planned(tmp_path, "xtrans", 300, 240, extra=["param:llap:01:numgamma:8", "param:llap:01:levels:6"])
+    red = [n for n in g.nodes() if n["kernel"] == "reduce"]
+    assert len(red) == 5 and all(n["dp"] == 9 for n in red)
+    asm = [n for n in g.nodes() if n["module"] == "llap" and n["kernel"] == "assemble"]
+    first = [np.frombuffer(n["pc"], np.int32)[1] for n in asm]
+    assert sum(first) == 1 and np.frombuffer(asm[-1]["pc"], np.int32)[1] == 1
+
+
+def test_sink_is_f32_and_llap_output_follows(tmp_path):
+    g, _, _ = planned(tmp_path, "bayer", 64, 48)
+    col = [n for n in g.nodes() if n["module"] == "llap" and n["kernel"] == "colour"][0]
+    assert col["conns"][2]["format"] == "f32"
+
+
+def test_errors_are_reported(tmp_path):
+    with pytest.raises(pipe.PipeError):
+        pipe.Graph(lines=["module:nosuchmod:01"])
+    # missing raw file: the sink has no size
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(str(tmp_path / "missing.dng")))
+    g.export("o-pfm", str(tmp_path / "o"))
+    with pytest.raises(pipe.PipeError, match="uninited size"):
+        g.plan()
+    # unconnected input
+    cfa = np.zeros((32, 32), np.uint16)
+    path = str(tmp_path / "z.dng")
+    synth.write_dng(path, cfa)
+    lines = [l for l in pipe.raw_graph_cfg(path) if "demosaic:01:output" not in l]
+    lines.append("connect:demosaic:01:output:display:main:input")   # colour... left dangling
+    g2 = pipe.Graph(lines=lines)
+    g2.export("o-pfm", str(tmp_path / "o2"))
+    g2.plan()   # colour/filmcurv/llap are not reachable from the sink: fine
+    names = {n["module"] for n in g2.nodes()}
+    assert "llap" not in names
+    # a format mismatch is a connect error (11)
+    g3 = pipe.Graph(lines=["module:i-raw:main", "module:llap:01",
+                           "connect:i-raw:main:output:llap:01:input"])
+    assert "11" in g3.error() or "failed" in g3.error()
+
+
+def test_cfg_file_round(tmp_path):
+    """a .cfg on disk with comments, the default-darkroom style module lines
+    with gui coordinates, and params with spaces in their names"""
+    cfa, _ = synth.mosaic(64, 48, 2, "bayer")
+    dng = str(tmp_path / "c.dng")
+    synth.write_dng(dng, cfa)
+    lines = pipe.raw_graph_cfg("c.dng", noise_a=1e-7, noise_b=2e-5)
+    lines = [l + ":43:400" if l.startswith("module:") else l for l in lines]
+    cfg = tmp_path / "g.cfg"
+    cfg.write_text("# a comment\n" + "\n".join(lines) + "\n")
+    g = pipe.Graph(cfg=str(cfg))
+    g.export("o-pfm", str(tmp_path / "o"))
+    g.plan()
+    ip = g.img_param("i-raw", "main")
+    assert ip["full"] == (64, 48)
+    assert np.allclose(ip["noise"], [1e-7, 2e-5], rtol=1e-6)

@@ -61,6 +61,7 @@ def lib():
         L.dt_graph_img_param_get.argtypes = [vp, i32, f4, f4, f4, C.c_float * 9, C.POINTER(u32), u4,
                                              C.c_float * 2, C.c_uint32 * 2]
         L.dt_graph_commit.argtypes = [vp]
+        L.dt_graph_read_source.argtypes = [vp, i32, vp]
         rc = L.dt_pipe_global_init(PKG_DIR.encode())
         if rc != 0:
             raise ImportError(f"no module classes found under {PKG_DIR}/modules")
@@ -128,6 +129,15 @@ class Graph:
 
     def commit(self):
         self.L.dt_graph_commit(self.g)
+
+    def read_source(self, module, inst, nbytes):
+        """host side of a source upload: the bytes read_source stages"""
+        import numpy as np
+        m = self.L.dt_graph_module(self.g, module.encode(), inst.encode())
+        buf = np.zeros(nbytes, np.uint8)
+        if self.L.dt_graph_read_source(self.g, m, buf.ctypes.data):
+            raise PipeError(f"read_source {module}:{inst} failed: {self.error()}")
+        return buf
 
     def plan(self, run=RUN_ALL):
         if self.L.dt_graph_plan(self.g, run):

@@ -126,6 +126,15 @@ int dt_graph_img_param_get(const dt_graph_t *g, int m, float black[4], float whi
   return 0;
 }
 
+// host part of a source module's upload: read_source into caller memory
+// (the staging copy that precedes the device transfer)
+int dt_graph_read_source(dt_graph_t *g, int m, void *buf)
+{
+  if(m < 0 || m >= g->num_modules || !g->module[m].so->read_source) return 1;
+  dt_read_source_params_t p = { 0 };
+  return g->module[m].so->read_source(g->module + m, buf, &p);
+}
+
 // run every module's commit_params (the uniform upload) without the device
 void dt_graph_commit(dt_graph_t *g)
 {

@@ -132,11 +132,16 @@ def _rational(v, signed):
 
 
 def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(1.0, 1.0, 1.0),
-              color_matrix=None, make="vkdt-synth", model="synthetic"):
-    """Write ``cfa`` (uint16 H x W) as an uncompressed CFA DNG."""
+              color_matrix=None, make="vkdt-synth", model="synthetic", pattern=None, illuminant=21,
+              extra_tags=()):
+    """Write ``cfa`` (uint16 H x W) as an uncompressed CFA DNG.
+
+    ``pattern`` overrides the CFA layout (a 2x2 or 6x6 array of 0/1/2),
+    ``illuminant`` is CalibrationIlluminant1 (21 = D65), ``extra_tags`` are
+    (tag, type, values) triples appended to the IFD (for negative tests)."""
     import struct
     H, W = cfa.shape
-    pat = cfa_pattern(kind)
+    pat = cfa_pattern(kind) if pattern is None else np.asarray(pattern, np.uint8)
     n = pat.shape[0]
     cm = xyz_to_cam() if color_matrix is None else np.asarray(color_matrix, np.float64)
     entries = []   # (tag, type, count, payload bytes)
@@ -176,7 +181,10 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
     add(50717, 4, [int(white)])           # WhiteLevel
     add(50721, 10, [float(v) for v in cm.ravel()])  # ColorMatrix1
     add(50728, 5, [float(v) for v in as_shot_neutral])  # AsShotNeutral
-    add(50778, 3, [21])                   # CalibrationIlluminant1: D65
+    add(50778, 3, [int(illuminant)])      # CalibrationIlluminant1 (21 = D65)
+    for tag, typ, vals in extra_tags:
+        entries[:] = [e for e in entries if e[0] != tag]
+        add(tag, typ, vals)
     entries.sort(key=lambda e: e[0])
 
     ifd_off = 8
