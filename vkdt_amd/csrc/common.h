@@ -42,6 +42,19 @@ DEV int mirror(int i, int n)
   return t < n ? t : 2 * n - 1 - t;
 }
 
+// hardware transcendentals (v_exp_f32, v_log_f32, v_rcp_f32, v_sqrt_f32,
+// v_rsq_f32: about 1 ulp).  The reference's GLSL exp/pow/division carry no
+// stronger guarantee (SPIR-V precision of exp2/log2 is 3 ulp); every use
+// is covered by the per-node parity tests against the float oracle.
+DEV float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+DEV float flog2(float x) { return __builtin_amdgcn_logf(x); }
+DEV float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+// x^y for x >= 0 (0^y = 0 for y > 0)
+DEV float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+
 DEV float h2f(__half h) { return __half2float(h); }
 DEV __half f2h(float f) { return __float2half_rn(f); }
 

@@ -41,7 +41,7 @@ struct fc_args { float il, k, bias; int mode; };
 
 DEV float weibull(float x, float il, float k)
 { // filmcurv/params.glsl:16-20
-  return 1.0f - expf(-powf(fmaxf(x, 1e-7f) * il, k));
+  return 1.0f - fexp(-fpow(fmaxf(x, 1e-7f) * il, k));
 }
 
 // shared.glsl:371-387
@@ -52,7 +52,7 @@ DEV void adjust_colour_dng(float c0[3], float c1[3])
   if(c0[2] > c0[1]) { t = c0[2]; c0[2] = c0[1]; c0[1] = t; t = c1[2]; c1[2] = c1[1]; c1[1] = t; fx = true; }
   if(c0[1] > c0[0]) { t = c0[0]; c0[0] = c0[1]; c0[1] = t; t = c1[0]; c1[0] = c1[1]; c1[1] = t; fy = true; }
   if(c0[2] > c0[1]) { t = c0[2]; c0[2] = c0[1]; c0[1] = t; t = c1[2]; c1[2] = c1[1]; c1[1] = t; fz = true; }
-  c1[1] = mixf(c1[2], c1[0], (c0[1] - c0[2] + 1e-6f) / (c0[0] - c0[2] + 1e-6f));
+  c1[1] = mixf(c1[2], c1[0], (c0[1] - c0[2] + 1e-6f) * frcp(c0[0] - c0[2] + 1e-6f));
   if(fz) { t = c1[2]; c1[2] = c1[1]; c1[1] = t; }
   if(fy) { t = c1[0]; c1[0] = c1[1]; c1[1] = t; }
   if(fx) { t = c1[2]; c1[2] = c1[1]; c1[1] = t; }

@@ -4,6 +4,11 @@
 // Inputs: the denoised mosaic, one f16 channel W x H.  gauss writes an rgba
 // f16 covariance (eigenvalues, major axis) per CFA block; splat writes the
 // interpolated green as one f16 channel; fix writes rgba f16 W x H.
+//
+// splat only stores the green sum of its splats and fix only weights the
+// red and blue sites (green sites copy the centre green), so both kernels
+// evaluate gaussian weights for exactly the taps whose sums reach the
+// output: the same sums in the same order as the reference's full loops.
 #include "launch.h"
 
 struct dm_args
@@ -30,12 +35,15 @@ __global__ void __launch_bounds__(256) k_dm_down(dimg in, dimg out, dm_args a)
   {
     const float c0 = fetch_h1(in, 2 * x, 2 * y), c1 = fetch_h1(in, 2 * x, 2 * y + 1);
     const float c2 = fetch_h1(in, 2 * x + 1, 2 * y), c3 = fetch_h1(in, 2 * x + 1, 2 * y + 1);
-    lum = (c0 + c1 + c2 + c3) / 4.0f;
+    lum = (c0 + c1 + c2 + c3) * 0.25f;
   }
   st_h1(out, x, y, lum);
 }
 
 // ---- gauss (gauss.comp:19-126) ---------------------------------------------
+// The covariance statistics decide between two tensors and feed a 2x2
+// eigen solve whose discriminant vanishes on flat or clipped blocks: they
+// keep IEEE division so that decision follows the written arithmetic.
 
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
@@ -44,7 +52,7 @@ __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
   if(x >= out.wd || y >= out.ht) return;
   constexpr int n0 = XT ? 0 : -1, n1 = 3, bs = XT ? 3 : 2;
   constexpr int NS = XT ? 5 : 8;
-  float px[NS];
+  float px[NS], rp[NS];
   int pi[NS], pj[NS];
   int n = 0;
 #pragma unroll
@@ -54,6 +62,7 @@ __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
     {
       if(XT ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
       px[n] = centre_h1(orig, bs * x + i, bs * y + j);
+      rp[n] = 1.0f / px[n];
       pi[n] = i; pj[n] = j;
       n++;
     }
@@ -64,7 +73,7 @@ __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
     mw0 += (float)pi[k] * px[k]; mw1 += (float)pj[k] * px[k];
     smw += px[k];
     mb0 += (float)pi[k] / px[k]; mb1 += (float)pj[k] / px[k];
-    smb += 1.0f / px[k];
+    smb += rp[k];
   }
   mw0 /= smw; mw1 /= smw;
   mb0 /= smb; mb1 /= smb;
@@ -107,110 +116,81 @@ __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
 
 // ---- splat (splat.comp:16-140) ---------------------------------------------
 
+// gaussian weight of offset (i,j) under cov = (eval, major axis):
+// exp(-0.5 |E o|^2_eval) clamped to [lo, 1], eval given as -0.5/eval
+DEV float gauss_w(float4 cov, float r0, float r1, int i, int j, float lo)
+{
+  const float ofx = cov.z * (float)i + cov.w * (float)j;
+  const float ofy = -cov.w * (float)i + cov.z * (float)j;
+  return clampf(fexp(ofx * ofx * r0 + ofy * ofy * r1), lo, 1.0f);
+}
+
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
   const float4 cov = XT ? fetch_h4(ga, x / 3, y / 3) : fetch_h4(ga, (x + 1) / 2, (y + 1) / 2);
-  const float ex = clampf(cov.x, 0.01f, 25.0f), ey = clampf(cov.y, 0.01f, 25.0f);
+  const float r0 = -0.5f * frcp(clampf(cov.x, 0.01f, 25.0f)), r1 = -0.5f * frcp(clampf(cov.y, 0.01f, 25.0f));
   constexpr int r = XT ? 2 : 1;
-  float rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  float g = 0.0f, w = 0.0f;
 #pragma unroll
   for(int j = -r; j <= r; j++)
 #pragma unroll
     for(int i = -r; i <= r; i++)
     {
+      // only green sites contribute to the stored channel
+      const bool green = XT ? xtrans_colour(x + i + 6, y + j + 6) == 1 : (((x + i + y + j) & 1) == 1);
+      if(!green) continue;
       int px = x + i, py = y + j;
       if(px < 0) px += 6;
       if(py < 0) py += 6;
       if(px >= in.wd) px -= 6;
       if(py >= in.ht) py -= 6;
-      float col = fetch_h1(in, px, py);
-      const float ofx = cov.z * (float)i + cov.w * (float)j;
-      const float ofy = -cov.w * (float)i + cov.z * (float)j;
-      float wt = clampf(expf(-0.5f * ((ofx / ex) * ofx + (ofy / ey) * ofy)), 1e-4f, 1.0f);
-      if(i == 0 && j == 0) wt = 666.0f;
-      if(XT)
-      {
-        const int cl = xtrans_colour(x + i + 6, y + j + 6);
-        const int k = cl == 1 ? 1 : (cl == 2 ? 2 : 0);
-        rgb[k] += col * wt;
-        w[k] += wt;
-      }
-      else
-      {
-        const int qx = x + i, qy = y + j;
-        col *= wt;
-        if(((qx & 1) == 0 && (qy & 1) == 1) || ((qx & 1) == 1 && (qy & 1) == 0)) { rgb[1] += col; w[1] += wt; }
-        else if((qx & 1) == 0 && (qy & 1) == 0) { rgb[0] += col; w[0] += wt; }
-        else { rgb[2] += col; w[2] += wt; }
-      }
+      const float col = fetch_h1(in, px, py);
+      const float wt = (i == 0 && j == 0) ? 666.0f : gauss_w(cov, r0, r1, i, j, 1e-4f);
+      g += col * wt;
+      w += wt;
     }
-  st_h1(out, x, y, rgb[1] / fmaxf(1e-8f, w[1]));
+  st_h1(out, x, y, g * frcp(fmaxf(1e-8f, w)));
 }
 
 // ---- fix (fix.comp:21-135) -------------------------------------------------
-
-DEV float fix_eval(float ex, float ey, float cz, float cw, int i, int j)
-{
-  const float ofx = cz * (float)i + cw * (float)j;
-  const float ofy = -cw * (float)i + cz * (float)j;
-  return clampf(expf(-0.5f * ((ofx / ex) * ofx + (ofy / ey) * ofy)), 1e-3f, 1.0f);
-}
 
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg out, int r)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  float rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
   const float gc = centre_h1(gr, x, y);
   float4 cov = XT ? fetch_h4(cv, (x + 1) / 3, (y + 1) / 3) : fetch_h4(cv, (x + 1) / 2, (y + 1) / 2);
   const float hi = XT ? 10.0f : 49.0f, sc = XT ? 3.0f : 2.0f;
-  cov.x = clampf(cov.x, 1.0f, hi);
-  cov.y = clampf(cov.y, 1.0f, hi);
-  const float ex = sc * cov.x, ey = sc * cov.y;
+  const float r0 = -0.5f * frcp(sc * clampf(cov.x, 1.0f, hi)), r1 = -0.5f * frcp(sc * clampf(cov.y, 1.0f, hi));
+  const float gcn = 1e-4f + gc;
+  bool any_green = false;
   for(int j = -r; j <= r; j++)
     for(int i = -r; i <= r; i++)
     {
-      const float gh = centre_h1(gr, x + i, y + j);
-      const float col = centre_h1(in, x + i, y + j);
+      const int px = x + i, py = y + j;   // may be negative: truncating division as in GLSL
+      int k;
       if(XT)
       {
-        const float corr = (1e-4f + gc) / (1e-4f + gh);
-        const int px = x + i, py = y + j;   // may be negative: truncating division as in GLSL
         const int blue_top = ((px / 3 + py / 3) & 1) > 0;
         const int mx = px - (px / 3) * 3, my = py - (py / 3) * 3;
-        if(((mx + my) & 1) == 0) { rgb[1] = gc; w[1] = 1.0f; }
-        else
-        {
-          const float wt = fix_eval(ex, ey, cov.z, cov.w, i, j);
-          const int k = (blue_top ^ (my == 1)) ? 2 : 0;
-          rgb[k] += col * corr * wt;
-          w[k] += wt;
-        }
+        k = (((mx + my) & 1) == 0) ? 1 : ((blue_top ^ (my == 1)) ? 2 : 0);
       }
-      else
-      {
-        const int ox = x + i, oy = y + j;
-        if((ox & 1) == 0 && (oy & 1) == 0)
-        {
-          const float wt = fix_eval(ex, ey, cov.z, cov.w, i, j);
-          rgb[0] += col * (1e-4f + gc) / (1e-4f + gh) * wt;
-          w[0] += wt;
-        }
-        else if(((ox & 1) == 0 && (oy & 1) == 1) || ((ox & 1) == 1 && (oy & 1) == 0)) { rgb[1] = gc; w[1] = 1.0f; }
-        else
-        {
-          const float wt = fix_eval(ex, ey, cov.z, cov.w, i, j);
-          rgb[2] += col * (1e-4f + gc) / (1e-4f + gh) * wt;
-          w[2] += wt;
-        }
-      }
+      else k = (((px + py) & 1) == 1) ? 1 : ((px & 1) == 0 ? 0 : 2);
+      if(k == 1) { any_green = true; continue; }
+      const float gh = centre_h1(gr, px, py);
+      const float col = centre_h1(in, px, py);
+      const float wt = gauss_w(cov, r0, r1, i, j, 1e-3f);
+      const float v = col * gcn * frcp(1e-4f + gh) * wt;
+      if(k == 0) { sr += v; wr += wt; }
+      else       { sb += v; wb += wt; }
     }
-  st_h4(out, x, y, make_float4(rgb[0] / fmaxf(1e-8f, w[0]), rgb[1] / fmaxf(1e-8f, w[1]),
-                               rgb[2] / fmaxf(1e-8f, w[2]), 1.0f));
+  const float g = any_green ? gc : 0.0f;
+  st_h4(out, x, y, make_float4(sr * frcp(fmaxf(1e-8f, wr)), g, sb * frcp(fmaxf(1e-8f, wb)), 1.0f));
 }
 
 // ---- host launchers --------------------------------------------------------
