@@ -9,7 +9,7 @@ noise a=2e-4 b=1e-6) through the graph i-raw -> denoise -> demosaic -> colour
 launch per node).  The DNG is decoded and uploaded once before timing; a
 step is one pass of every node over the resident raw (no host transfer).
 
-Prints one JSON line (see README of the driver contract): value is Mpix/s
+Prints one JSON line: value is Mpix/s
 (CFA pixels after i-raw block rounding per second), roofline is for the
 node kernel with the largest share of the step, measured with HIP events
 on the graph's stream inside the timed region, cpu_baseline times the
@@ -69,6 +69,18 @@ def cpu_baseline(seconds_hint=True):
             "sample": f"oracle (FP64 scalar C restatement, 1 thread) on the 2048x1536 Bayer case, {dt:.1f} s"}
 
 
+def calibrate_copy():
+    """a copy kernel of known traffic: 1 GiB read + 1 GiB written per launch"""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,10 +89,17 @@ def main():
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", default="", help="write per-kernel timings as JSON to this path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB (5 times), the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(images shard across GPUs with no communication, see DESIGN.md)", file=sys.stderr)
+              "(images are independent units and would shard across GPUs with no communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

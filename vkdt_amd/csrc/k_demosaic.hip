@@ -10,6 +10,7 @@
 // evaluate gaussian weights for exactly the taps whose sums reach the
 // output: the same sums in the same order as the reference's full loops.
 #include "launch.h"
+#include "ops.h"
 
 struct dm_args
 {
@@ -40,89 +41,12 @@ __global__ void __launch_bounds__(256) k_dm_down(dimg in, dimg out, dm_args a)
   st_h1(out, x, y, lum);
 }
 
-// ---- gauss (gauss.comp:19-126) ---------------------------------------------
-// The covariance statistics decide between two tensors and feed a 2x2
-// eigen solve whose discriminant vanishes on flat or clipped blocks: they
-// keep IEEE division so that decision follows the written arithmetic.
-
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  constexpr int n0 = XT ? 0 : -1, n1 = 3, bs = XT ? 3 : 2;
-  constexpr int NS = XT ? 5 : 8;
-  float px[NS], rp[NS];
-  int pi[NS], pj[NS];
-  int n = 0;
-#pragma unroll
-  for(int j = n0; j < n1; j++)
-#pragma unroll
-    for(int i = n0; i < n1; i++)
-    {
-      if(XT ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
-      px[n] = centre_h1(orig, bs * x + i, bs * y + j);
-      rp[n] = 1.0f / px[n];
-      pi[n] = i; pj[n] = j;
-      n++;
-    }
-  float mw0 = 0, mw1 = 0, mb0 = 0, mb1 = 0, smw = 0, smb = 0;
-#pragma unroll
-  for(int k = 0; k < NS; k++)
-  {
-    mw0 += (float)pi[k] * px[k]; mw1 += (float)pj[k] * px[k];
-    smw += px[k];
-    mb0 += (float)pi[k] / px[k]; mb1 += (float)pj[k] / px[k];
-    smb += rp[k];
-  }
-  mw0 /= smw; mw1 /= smw;
-  mb0 /= smb; mb1 /= smb;
-  float Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0;
-#pragma unroll
-  for(int k = 0; k < NS; k++)
-  {
-    float p2 = px[k] * px[k];
-    float dx = (float)pi[k] - mw0, dy = (float)pj[k] - mw1;
-    Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
-    sw += p2;
-    dx = (float)pi[k] - mb0; dy = (float)pj[k] - mb1;
-    p2 = 1.0f / p2;
-    Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
-    sb += p2;
-  }
-#pragma unroll
-  for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
-  const bool usew = det2(Sw) < det2(Sb);
-  float S[4];
-#pragma unroll
-  for(int k = 0; k < 4; k++) S[k] = usew ? Sw[k] : Sb[k];
-  float ev[2], e0[2], e1[2];
-  evd2x2(ev, e0, e1, S);
-  if(!XT)
-  {
-    ev[0] *= 0.2f; ev[1] *= 0.2f;
-    if(fabsf(e0[0]) > fabsf(e0[1])) { e0[0] = 1.0f; e0[1] = 0.0f; }
-    else                            { e0[0] = 0.0f; e0[1] = 1.0f; }
-  }
-  else
-  {
-    ev[0] *= 0.4f; ev[1] *= 0.4f;
-    if     (fabsf(e0[0]) > 2.0f * fabsf(e0[1])) { e0[0] = 1.0f; e0[1] = 0.0f; }
-    else if(fabsf(e0[1]) > 2.0f * fabsf(e0[0])) { e0[0] = 0.0f; e0[1] = 1.0f; }
-    else { ev[0] = 0.1f; ev[1] = 0.1f; }
-  }
-  st_h4(out, x, y, make_float4(ev[0], ev[1], e0[0], e0[1]));
-}
-
-// ---- splat (splat.comp:16-140) ---------------------------------------------
-
-// gaussian weight of offset (i,j) under cov = (eval, major axis):
-// exp(-0.5 |E o|^2_eval) clamped to [lo, 1], eval given as -0.5/eval
-DEV float gauss_w(float4 cov, float r0, float r1, int i, int j, float lo)
-{
-  const float ofx = cov.z * (float)i + cov.w * (float)j;
-  const float ofy = -cov.w * (float)i + cov.z * (float)j;
-  return clampf(fexp(ofx * ofx * r0 + ofy * ofy * r1), lo, 1.0f);
+  st_h4(out, x, y, dm_gauss_px<XT>(orig, x, y));
 }
 
 template<int XT>
@@ -130,67 +54,15 @@ __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  const float4 cov = XT ? fetch_h4(ga, x / 3, y / 3) : fetch_h4(ga, (x + 1) / 2, (y + 1) / 2);
-  const float r0 = -0.5f * frcp(clampf(cov.x, 0.01f, 25.0f)), r1 = -0.5f * frcp(clampf(cov.y, 0.01f, 25.0f));
-  constexpr int r = XT ? 2 : 1;
-  float g = 0.0f, w = 0.0f;
-#pragma unroll
-  for(int j = -r; j <= r; j++)
-#pragma unroll
-    for(int i = -r; i <= r; i++)
-    {
-      // only green sites contribute to the stored channel
-      const bool green = XT ? xtrans_colour(x + i + 6, y + j + 6) == 1 : (((x + i + y + j) & 1) == 1);
-      if(!green) continue;
-      int px = x + i, py = y + j;
-      if(px < 0) px += 6;
-      if(py < 0) py += 6;
-      if(px >= in.wd) px -= 6;
-      if(py >= in.ht) py -= 6;
-      const float col = fetch_h1(in, px, py);
-      const float wt = (i == 0 && j == 0) ? 666.0f : gauss_w(cov, r0, r1, i, j, 1e-4f);
-      g += col * wt;
-      w += wt;
-    }
-  st_h1(out, x, y, g * frcp(fmaxf(1e-8f, w)));
+  st_h1(out, x, y, dm_splat_px<XT>(in, ga, x, y));
 }
-
-// ---- fix (fix.comp:21-135) -------------------------------------------------
 
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg out, int r)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
-  const float gc = centre_h1(gr, x, y);
-  float4 cov = XT ? fetch_h4(cv, (x + 1) / 3, (y + 1) / 3) : fetch_h4(cv, (x + 1) / 2, (y + 1) / 2);
-  const float hi = XT ? 10.0f : 49.0f, sc = XT ? 3.0f : 2.0f;
-  const float r0 = -0.5f * frcp(sc * clampf(cov.x, 1.0f, hi)), r1 = -0.5f * frcp(sc * clampf(cov.y, 1.0f, hi));
-  const float gcn = 1e-4f + gc;
-  bool any_green = false;
-  for(int j = -r; j <= r; j++)
-    for(int i = -r; i <= r; i++)
-    {
-      const int px = x + i, py = y + j;   // may be negative: truncating division as in GLSL
-      int k;
-      if(XT)
-      {
-        const int blue_top = ((px / 3 + py / 3) & 1) > 0;
-        const int mx = px - (px / 3) * 3, my = py - (py / 3) * 3;
-        k = (((mx + my) & 1) == 0) ? 1 : ((blue_top ^ (my == 1)) ? 2 : 0);
-      }
-      else k = (((px + py) & 1) == 1) ? 1 : ((px & 1) == 0 ? 0 : 2);
-      if(k == 1) { any_green = true; continue; }
-      const float gh = centre_h1(gr, px, py);
-      const float col = centre_h1(in, px, py);
-      const float wt = gauss_w(cov, r0, r1, i, j, 1e-3f);
-      const float v = col * gcn * frcp(1e-4f + gh) * wt;
-      if(k == 0) { sr += v; wr += wt; }
-      else       { sb += v; wb += wt; }
-    }
-  const float g = any_green ? gc : 0.0f;
-  st_h4(out, x, y, make_float4(sr * frcp(fmaxf(1e-8f, wr)), g, sb * frcp(fmaxf(1e-8f, wb)), 1.0f));
+  st_h4(out, x, y, dm_fix_px<XT>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
 }
 
 // ---- host launchers --------------------------------------------------------
