@@ -50,6 +50,19 @@ def node_bytes(n):
     return total
 
 
+def measured_traffic(tag, node):
+    """HBM bytes per launch of `node` from the committed FETCH_SIZE/WRITE_SIZE
+    summary of this workload (counters scaled by a copy kernel of known size)"""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == node:
+            return int(f[2])
+    return None
+
+
 def cpu_baseline(seconds_hint=True):
     """the oracle (FP64 scalar C, one thread) on the 2048 x 1536 Bayer case"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -154,7 +167,8 @@ def main():
     achieved = e["bytes"] / (e["ms"] * 1e-3) / 1e9      # GB/s over its launches
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": int(e["bytes"] / e["launches"]),
+            "traffic": measured_traffic(f"plain_24mp_{args.kind}", dom),
+            "bytes_per_launch": int(e["bytes"] / e["launches"]),
             "avg_launch_ms": round(e["ms"] / e["launches"], 4)}
     graph_roof = {"algorithmic_bytes": graph_bytes, "bytes_per_px": round(graph_bytes / P, 1),
                   "gpu_ms": round(gpu_ms, 4), "achieved": round(graph_bytes / (gpu_ms * 1e-3) / 1e9, 1),

@@ -217,5 +217,21 @@ DEV int sorted1(int x, int n) { return x / 2 + (x & 1) * ((n + 1) / 2); }
 // grid helpers: 64x4 work groups (one wave per row segment)
 #define BX 64
 #define BY 4
-DEV int tid_x() { return blockIdx.x * BX + threadIdx.x; }
-DEV int tid_y() { return blockIdx.y * BY + threadIdx.y; }
+// XCD-aware tile order.  Work groups are handed to the 8 XCDs round robin
+// by linear id, and each XCD has its own L2: consecutive ids therefore land
+// on different L2s, and the halo rows a stencil tile shares with its
+// neighbours are fetched once per XCD.  xcd_block() renumbers so that XCD
+// k runs one contiguous run of tiles (raster order, k-th eighth of the
+// grid): vertical neighbours share an L2.  A bijection on the grid.
+#define NUM_XCD 8
+DEV int2 xcd_block()
+{
+  const int b = blockIdx.x + blockIdx.y * gridDim.x;
+  const int n = gridDim.x * gridDim.y;
+  const int q = n / NUM_XCD, r = n - q * NUM_XCD;
+  const int k = b % NUM_XCD, i = b / NUM_XCD;
+  const int t = k * q + min(k, r) + i;
+  return make_int2(t % gridDim.x, t / gridDim.x);
+}
+DEV int tid_x() { return xcd_block().x * BX + threadIdx.x; }
+DEV int tid_y() { return xcd_block().y * BY + threadIdx.y; }

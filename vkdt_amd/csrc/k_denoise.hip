@@ -56,13 +56,15 @@ __global__ void __launch_bounds__(256) k_dn_half(dimg in, dimg out, dn_args a)
 __global__ void __launch_bounds__(256) k_dn_downcov(dimg in, dimg out, dimg cov, dn_args a)
 {
   __shared__ dn_cov_smem s;
-  dn_downcov_tile(in, out, cov, a, blockIdx.x * BX, blockIdx.y * BY, s);
+  const int2 t = xcd_block();
+  dn_downcov_tile(in, out, cov, a, t.x * BX, t.y * BY, s);
 }
 
 __global__ void __launch_bounds__(256) k_dn_down(dimg in, dimg out, dn_args a)
 {
   __shared__ float4 tile[TN];
-  dn_down_tile(in, out, a, blockIdx.x * BX, blockIdx.y * BY, tile);
+  const int2 t = xcd_block();
+  dn_down_tile(in, out, a, t.x * BX, t.y * BY, tile);
 }
 
 __global__ void __launch_bounds__(256) k_dn_assemble(dimg s0, dimg s1, dimg s2, dimg s3, dimg s4, dimg out,

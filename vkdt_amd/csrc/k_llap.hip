@@ -14,22 +14,44 @@
 #include "launch.h"
 #include "ops.h"
 
-__global__ void __launch_bounds__(256) k_ll_curve(dimg in, dimg out, ll_args a)
+// four adjacent texels per thread; with 8-byte aligned output rows and
+// planes (`vec`) each plane's four values leave in one store
+__global__ void __launch_bounds__(256) k_ll_curve(dimg in, dimg out, ll_args a, int vec)
 {
-  const int x = tid_x(), y = tid_y();
-  if(x >= in.wd || y >= in.ht) return;
-  const float yl = ll_lum(fetch_h4(in, x, y));
+  const int x0 = 4 * tid_x(), y = tid_y();
+  if(x0 >= in.wd || y >= in.ht) return;
+  if(!vec || x0 + 4 > in.wd)
+  {
+    for(int k = 0; k < 4 && x0 + k < in.wd; k++)
+    {
+      const float yl = ll_lum(ld_h4(in, x0 + k, y));
+      for(int i = 0; i < a.num_gamma; i++) st_h1(out, x0 + k, y, ll_curve(yl, a.gam[i], a), i);
+      st_h1(out, x0 + k, y, yl, a.num_gamma);
+    }
+    return;
+  }
+  float yl[4];
+#pragma unroll
+  for(int k = 0; k < 4; k++) yl[k] = ll_lum(ld_h4(in, x0 + k, y));
+  __half *po = (__half *)out.p + (long)y * out.pitch + x0;
+  union { uint2 u; __half e[4]; } w;
   for(int i = 0; i < a.num_gamma; i++)
-    st_h1(out, x, y, ll_curve(yl, a.gam[i], a), i);
-  st_h1(out, x, y, yl, a.num_gamma);
+  {
+#pragma unroll
+    for(int k = 0; k < 4; k++) w.e[k] = f2h(ll_curve(yl[k], a.gam[i], a));
+    *(uint2 *)(po + i * out.pstride) = w.u;
+  }
+#pragma unroll
+  for(int k = 0; k < 4; k++) w.e[k] = f2h(yl[k]);
+  *(uint2 *)(po + a.num_gamma * out.pstride) = w.u;
 }
 
-// all planes of one texel per thread
-__global__ void __launch_bounds__(256) k_ll_reduce(dimg in, dimg out, int planes)
+// all planes of four adjacent output texels per thread
+__global__ void __launch_bounds__(256) k_ll_reduce(dimg in, dimg out, int planes, int vec)
 {
-  const int x = tid_x(), y = tid_y();
+  const int x = 4 * tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  ll_reduce_px(in, out, planes, x, y);
+  ll_reduce_px4(in, out, planes, x, y, vec != 0);
 }
 
 #define AS_R 4   // fine rows per thread: 64 x 16 fine texels per work group
@@ -38,7 +60,8 @@ __global__ void __launch_bounds__(256) k_ll_assemble(dimg coarse, dimg l0, dimg 
 {
   extern __shared__ float tile[];             // up to (num_gamma + 1) slots of AS_TN(AS_R)
   __shared__ int lims[2];
-  const int X0 = blockIdx.x * BX, Y0 = blockIdx.y * (BY * AS_R);
+  const int2 t = xcd_block();
+  const int X0 = t.x * BX, Y0 = t.y * (BY * AS_R);
   const int x = X0 + threadIdx.x;
   float v[AS_R], res[AS_R];
 #pragma unroll
@@ -88,8 +111,10 @@ int launch_llap_curve(const node_call &c)
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, a.num_gamma + 1, "llap/curve output")) return e;
   if(c.conn[1].wd < c.conn[0].wd || c.conn[1].ht < c.conn[0].ht)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "llap/curve: output smaller than input");
-  hipLaunchKernelGGL(k_ll_curve, grid2d(c.conn[0].wd, c.conn[0].ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+  const dimg out = to_dimg(c.conn[1]);
+  const int vec = out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
+  hipLaunchKernelGGL(k_ll_curve, grid2d((c.conn[0].wd + 3) / 4, c.conn[0].ht, 1), dim3(BX, BY), 0, c.stream,
+                     to_dimg(c.conn[0]), out, a, vec);
   return launch_post("llap/curve");
 }
 
@@ -97,8 +122,12 @@ int launch_llap_reduce(const node_call &c)
 {
   if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, c.dp, "llap/reduce inhi")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, c.dp, "llap/reduce outlo")) return e;
-  hipLaunchKernelGGL(k_ll_reduce, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), (int)c.dp);
+  const dimg in = to_dimg(c.conn[0]), out = to_dimg(c.conn[1]);
+  // 16-byte input rows and 8-byte output groups at every row and plane start
+  const int vec = in.pitch % 8 == 0 && in.pstride % 8 == 0 && (uintptr_t)in.p % 16 == 0 &&
+                  out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
+  hipLaunchKernelGGL(k_ll_reduce, grid2d((c.wd + 3) / 4, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                     in, out, (int)c.dp, vec);
   return launch_post("llap/reduce");
 }
 

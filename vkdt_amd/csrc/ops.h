@@ -738,6 +738,56 @@ DEV void ll_reduce_px(const dimg &in, const dimg &out, int planes, int x, int y)
   }
 }
 
+// four outputs (x0..x0+3, y) of ll_reduce_px.  When the footprint
+// 2x0-1 .. 2x0+7 lies inside the row and rows and planes are 16-byte
+// aligned (`vec`), the texels 2x0 .. 2x0+7 of each input row arrive in one
+// 16-byte load and the four results leave in one 8-byte store; the sums are
+// ll_reduce_px's, term for term.
+DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int y, bool vec)
+{
+  if(!vec || 2 * x0 + 8 > in.wd)
+  {
+    for(int k = 0; k < 4; k++) if(x0 + k < out.wd) ll_reduce_px(in, out, planes, x0 + k, y);
+    return;
+  }
+  const bool vout = x0 + 4 <= out.wd;
+  const int xl = mirror(2 * x0 - 1, in.wd);
+  const long rw[3] = { (long)mirror(2 * y - 1, in.ht) * in.pitch, (long)mirror(2 * y, in.ht) * in.pitch,
+                       (long)mirror(2 * y + 1, in.ht) * in.pitch };
+  const __half *p = (const __half *)in.p;
+  __half *po = (__half *)out.p + (long)y * out.pitch + x0;
+  for(int z = 0; z < planes; z++)
+  {
+    const __half *q = p + z * in.pstride;
+    float h[3][4];
+#pragma unroll
+    for(int r = 0; r < 3; r++)
+    {
+      float c[9];
+      c[0] = h2f(q[rw[r] + xl]);
+      union { uint4 u; __half e[8]; } v;
+      v.u = *(const uint4 *)(q + rw[r] + 2 * x0);
+#pragma unroll
+      for(int i = 0; i < 8; i++) c[i + 1] = h2f(v.e[i]);
+#pragma unroll
+      for(int k = 0; k < 4; k++) h[r][k] = c[2 * k] + 2.0f * c[2 * k + 1] + c[2 * k + 2];
+    }
+    float o[4];
+#pragma unroll
+    for(int k = 0; k < 4; k++) o[k] = (h[0][k] + 2.0f * h[1][k] + h[2][k]) * (1.0f / 16.0f);
+    __half *qo = po + z * out.pstride;
+    if(vout)
+    {
+      union { uint2 u; __half e[4]; } w;
+#pragma unroll
+      for(int k = 0; k < 4; k++) w.e[k] = f2h(o[k]);
+      *(uint2 *)qo = w.u;
+    }
+    else
+      for(int k = 0; k < 4; k++) if(x0 + k < out.wd) qo[k] = f2h(o[k]);
+  }
+}
+
 // assemble.comp:169-203 for a fine tile of 64 x 4R texels; thread (tx,ty)
 // owns the texels (X0+tx, Y0+ty+4k), k < R.  Every fine texel expands
 // (sample_soft at o/2+.5 = (1/2 1/2 1 1/2 1/2)/3 even, (1 1/2 1/2 1)/3 odd,
