@@ -123,6 +123,72 @@ int vkdt_hip_event_elapsed_ms(float *ms, void *ev_begin, void *ev_end);
 // Human readable text of the last error on this thread.
 const char *vkdt_hip_last_error(void);
 
+// ---- batched develop ------------------------------------------------------
+// The raw-development graph i-raw -> denoise -> demosaic -> colour ->
+// filmcurv -> llap of many stills or frames, of any sizes and CFAs, in one
+// launch per stage.  No reference counterpart: in the reference each image
+// is its own graph run (graph.c:719 dt_graph_run per export), which is what
+// vkdt_hip_node_launch replaces one dispatch at a time.  The batch runs the
+// same arithmetic as that per-node graph (stage outputs rounded to f16
+// where the graph stores f16), so every image's result equals its plain
+// graph run.
+//
+// One image of a batch: the values the per-node graph's nodes receive.
+// dt_graph_develop_image() (vkdt_amd/pipe/api.h) fills every field but the
+// two device pointers from a planned graph (its push constants, params and
+// node extents).
+typedef struct vkdt_develop_image_t
+{
+  const void *cfa;            // device: the ui16 mosaic i-raw uploads
+  uint32_t cfa_wd, cfa_ht;    // its extent
+  uint32_t cfa_pitch;         // texels per row
+  void    *out;               // device: rgba f32, wd x ht, tightly packed (llap/colour output)
+  uint32_t wd, ht;            // developed extent (denoise output: the crop box)
+  uint32_t filters;           // 9: X-Trans; other non-zero: Bayer RGGB
+  float    wb[4], black[4], white[4];   // denoise push constants (black/white in UNORM)
+  int32_t  crop[4];
+  float    noise_a, noise_b;
+  float    dn_params[8];      // denoise/params blob: strength, luma, detail, (pad), edges[4]
+  int32_t  dm_fixup;          // demosaic/params word 0
+  float    co_params[242];    // colour's committed uniform block
+  float    fc_params[4];      // filmcurv/params: light, contrast, bias, colour
+  float    ll_params[4];      // llap/params: sigma, shadows, hilights, clarity
+  int32_t  ll_numgamma;       // curve planes G-1 (llap/curve push constant)
+  int32_t  ll_levels;         // pyramid levels nl (reduce nodes + 1)
+}
+vkdt_develop_image_t;
+
+typedef struct vkdt_batch_t vkdt_batch_t;
+
+// Device scratch a batch of these images needs (intermediates only: the
+// caller owns the mosaics and the outputs).
+int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t *bytes);
+
+// Plan a batch: validate the images, lay out their intermediates in
+// `scratch` (at least vkdt_hip_batch_scratch_bytes, 256-byte aligned) and
+// build the per-stage tile lists.  `tune` selects the launch shape per
+// stage: NULL takes the built-in choice, else an array of
+// VKDT_BATCH_STAGES entries as written by vkdt_hip_batch_autotune.
+#define VKDT_BATCH_STAGES 9
+int vkdt_hip_batch_create(vkdt_batch_t **batch, const vkdt_develop_image_t *img, int n,
+                          void *scratch, size_t scratch_bytes, const int32_t *tune);
+
+// Enqueue the whole batch on `stream`.  The first run with `graph` != 0
+// captures the launch sequence into a hipGraph; later runs replay it.
+int vkdt_hip_batch_run(vkdt_batch_t *batch, void *stream, int graph);
+
+// Time each stage's launch shapes on this batch and keep the fastest
+// (written to `tune`, VKDT_BATCH_STAGES entries, when not NULL).
+int vkdt_hip_batch_autotune(vkdt_batch_t *batch, void *stream, int32_t *tune);
+
+// Per-stage GPU time of the last vkdt_hip_batch_run (ms, VKDT_BATCH_STAGES
+// entries, when timing was on) and the stage names.
+int vkdt_hip_batch_set_timing(vkdt_batch_t *batch, int on);
+int vkdt_hip_batch_stage_ms(vkdt_batch_t *batch, float *ms);
+const char *vkdt_hip_batch_stage_name(int stage);
+
+void vkdt_hip_batch_destroy(vkdt_batch_t *batch);
+
 #ifdef __cplusplus
 }
 #endif

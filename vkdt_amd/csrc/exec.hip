@@ -18,7 +18,7 @@ int vkdt_set_error(int code, const char *fmt, ...)
   return code;
 }
 
-static int hip_fail(hipError_t e, const char *what)
+int hip_fail(hipError_t e, const char *what)
 {
   if(e == hipSuccess) return VKDT_HIP_OK;
   return vkdt_set_error(e == hipErrorOutOfMemory ? VKDT_HIP_OOM : VKDT_HIP_DEVICE,

@@ -1,0 +1,751 @@
+// k_batch.hip -- the raw-development graph of many images per launch.
+//
+// A batch holds stills or frames of any sizes and CFAs.  Every stage is one
+// launch over the tiles of all images that have it: a flat tile list with a
+// per-image job table (binary searched by tile index), run either as one
+// tile per work group or by persistent work groups that pull tiles from
+// per-XCD queues (XCD k starts on the k-th eighth of the list, then steals
+// from the next eighths).  Stages:
+//
+//   group 0  half      denoise/half                   CFA ui16 -> Q rgba f16
+//   group 1  wavelet   denoise/downcov, down x3       Q -> Q (four launches)
+//   group 2  dnasm     denoise/assemble               5x Q -> Q
+//   group 3  doub      denoise/doub                   CFA + 2x Q -> mosaic f16
+//   group 4  gauss     demosaic/gauss                 mosaic -> block covariance
+//   group 5  develop   demosaic/splat + demosaic/fix + colour + filmcurv,
+//                      fused: the green of a tile and its halo lives in LDS,
+//                      fix/colour/filmcurv run in registers -> rgba f16
+//   group 6  pyr1      llap/curve + llap/reduce to level 1, fused: the
+//                      full-resolution curve planes exist only in LDS
+//   group 7  pyramid   llap/reduce levels 2.., llap/assemble levels nl-2..1
+//   group 8  final     llap/assemble level 0 + llap/colour, fused: level 0
+//                      planes are re-evaluated from the filmcurv output
+//
+// Wherever the per-node graph stores an intermediate as f16 and the next
+// node reads it back, the fused stages round the value to binary16 at the
+// same point (r16), so each image's output equals its per-node graph run.
+// Per 24 MP Bayer still this skips the splat/fix/colour intermediates and
+// the 11 full-resolution curve planes (about 96 B per CFA pixel).
+#include <vector>
+#include <algorithm>
+#include "launch.h"
+#include "ops.h"
+
+#define BT_LEVELS 12
+#define DEV_R 2     // develop: 64 x 8 texels per tile
+#define ASM_R 4     // assemble / final: 64 x 16 texels per tile
+
+struct bt_img
+{
+  dimg cfa, out;
+  dimg qh, qd[4], qa;      // denoise: half, downcov + down 1..3, assemble (Q rgba f16)
+  dimg mos, cov, film;     // doub output (f16), gauss output (rgba f16), filmcurv output (rgba f16)
+  dimg lp[BT_LEVELS];      // llap planes of level l >= 1, num_gamma + 1 planes
+  dimg la[BT_LEVELS];      // llap reconstruction of level l, 1 <= l <= nl-2
+  int  nl, xt, dm_r;
+  dn_args dn_half, dn_down[4], dn_asm, dn_doub;
+  co_args co;
+  fc_args fc;
+  ll_args ll;
+};
+
+struct bt_job { int img, tiles_x, start, level; };
+
+struct bt_launch
+{
+  const bt_img *img;
+  const bt_job *job;
+  int njob, total;
+  int *ctr;          // NUM_XCD tile counters (persistent launches)
+  int persist;
+};
+
+// ---- tile scheduling --------------------------------------------------------
+
+DEV int bt_find(const bt_job *job, int n, int t)
+{
+  int lo = 0, hi = n - 1;
+  while(lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if(job[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// next tile for this work group: its home queue (work group id mod XCDs)
+// first, then the following queues; `tried` counts exhausted queues
+DEV int bt_next(const bt_launch &L, int &tried)
+{
+  const int home = blockIdx.x % NUM_XCD;
+  for(; tried < NUM_XCD; tried++)
+  {
+    const int k = (home + tried) % NUM_XCD;
+    const int s = (int)((long long)L.total * k / NUM_XCD), e = (int)((long long)L.total * (k + 1) / NUM_XCD);
+    if(s >= e) continue;
+    const int t = s + atomicAdd(L.ctr + k, 1);
+    if(t < e) return t;
+  }
+  return L.total;
+}
+
+// run fn(job, local tile) over the launch's tiles.  fn may stage LDS with
+// __syncthreads (all threads call it with the same tile).
+template<typename F>
+DEV void bt_tiles(const bt_launch &L, F fn)
+{
+  if(!L.persist)
+  { // one tile per work group, XCD k on the k-th eighth of the list
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n / NUM_XCD, r = n - q * NUM_XCD, k = b % NUM_XCD;
+    const int t = k * q + min(k, r) + b / NUM_XCD;
+    if(t >= L.total) return;
+    const int j = bt_find(L.job, L.njob, t);
+    fn(L.job[j], t - L.job[j].start);
+    return;
+  }
+  __shared__ int s_t;
+  int tried = 0;
+  for(;;)
+  {
+    __syncthreads();   // the previous tile is done with s_t and its LDS
+    if(threadIdx.x == 0 && threadIdx.y == 0) s_t = bt_next(L, tried);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_t);
+    if(t >= L.total) break;
+    const int j = bt_find(L.job, L.njob, t);
+    fn(L.job[j], t - L.job[j].start);
+  }
+}
+
+__global__ void kb_reset(int *ctr, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) ctr[i] = 0;
+}
+
+// ---- denoise -----------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) kb_half(bt_launch L)
+{
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
+    if(x < im.qh.wd && y < im.qh.ht) st_h4(im.qh, x, y, dn_half_px(im.cfa, im.dn_half, x, y));
+  });
+}
+
+__global__ void __launch_bounds__(256) kb_downcov(bt_launch L)
+{
+  __shared__ dn_cov_smem s;
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const dimg nocov = {};
+    dn_downcov_tile(im.qh, im.qd[0], nocov, im.dn_down[0], (lt % j.tiles_x) * BX, (lt / j.tiles_x) * BY, s);
+  });
+}
+
+__global__ void __launch_bounds__(256) kb_down(bt_launch L)
+{
+  __shared__ float4 tile[TN];
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int l = j.level;
+    dn_down_tile(im.qd[l - 1], im.qd[l], im.dn_down[l], (lt % j.tiles_x) * BX, (lt / j.tiles_x) * BY, tile);
+  });
+}
+
+__global__ void __launch_bounds__(256) kb_dnasm(bt_launch L, asm_mats m)
+{
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
+    if(x < im.qa.wd && y < im.qa.ht)
+      st_h4(im.qa, x, y, dn_assemble_px(im.qh, im.qd[0], im.qd[1], im.qd[2], im.qd[3], im.dn_asm, m, x, y));
+  });
+}
+
+__global__ void __launch_bounds__(256) kb_doub(bt_launch L)
+{
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
+    if(x < im.mos.wd && y < im.mos.ht)
+      st_h1(im.mos, x, y, dn_doub_px(im.cfa, im.qa, im.qh, im.mos.wd, im.mos.ht, im.dn_doub, x, y));
+  });
+}
+
+// ---- demosaic + colour + filmcurv --------------------------------------------
+
+__global__ void __launch_bounds__(256) kb_gauss(bt_launch L)
+{
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
+    if(x >= im.cov.wd || y >= im.cov.ht) return;
+    st_h4(im.cov, x, y, im.xt ? dm_gauss_px<1>(im.mos, x, y) : dm_gauss_px<0>(im.mos, x, y));
+  });
+}
+
+// green of the tile plus an r-texel halo at mirrored coordinates (what the
+// per-node fix reads from the stored splat output), then fix, colour and
+// filmcurv per texel with the f16 rounding of each stored node output
+template<int XT>
+DEV void develop_tile(const bt_img &im, int X0, int Y0, float *gt)
+{
+  const int r = im.dm_r;
+  const int gw = BX + 2 * r, gh = BY * DEV_R + 2 * r;
+  const int W = im.film.wd, H = im.film.ht;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for(int i = tid; i < gw * gh; i += BX * BY)
+  {
+    const int ty = i / gw, tx = i - ty * gw;
+    gt[i] = r16(dm_splat_px<XT>(im.mos, im.cov, mirror(X0 - r + tx, W), mirror(Y0 - r + ty, H)));
+  }
+  __syncthreads();
+  const int x = X0 + threadIdx.x;
+  for(int k = 0; k < DEV_R; k++)
+  {
+    const int y = Y0 + threadIdx.y + BY * k;
+    if(x >= W || y >= H) continue;
+    const float4 f = dm_fix_px<XT>(im.mos, im.cov, r, x, y,
+                                   [&](int px, int py) { return gt[(py - Y0 + r) * gw + (px - X0 + r)]; });
+    const float4 c = r16(co_px(r16(f), im.co));
+    st_h4(im.film, x, y, fc_px(c, im.fc));
+  }
+}
+
+__global__ void __launch_bounds__(256) kb_develop(bt_launch L)
+{
+  __shared__ float gt[(BX + 6) * (BY * DEV_R + 6)];   // halo r <= 3
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * DEV_R);
+    if(im.xt) develop_tile<1>(im, X0, Y0, gt);
+    else      develop_tile<0>(im, X0, Y0, gt);
+  });
+}
+
+// ---- llap ------------------------------------------------------------------
+
+// level-1 tile (64 x 4) of curve + reduce: the footprint (2X0-1 .. 2X0+128,
+// 2Y0-1 .. 2Y0+8, mirrored) of every level-0 plane is evaluated once into
+// LDS as binary16 -- the values the curve node stores -- and reduced from
+// there in reduce.comp's order
+#define P1_W (2 * BX + 2)
+#define P1_H (2 * BY + 1)
+#define P1_N (P1_W * P1_H)
+
+__global__ void __launch_bounds__(256) kb_pyr1(bt_launch L)
+{
+  extern __shared__ __half pl[];   // (num_gamma + 1) x P1_N
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const dimg &f = im.film, &o = im.lp[1];
+    const int ng = im.ll.num_gamma;
+    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * BY;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for(int i = tid; i < P1_N; i += BX * BY)
+    {
+      const int ty = i / P1_W, tx = i - ty * P1_W;
+      const float yl = ll_lum(ld_h4(f, mirror(2 * X0 - 1 + tx, f.wd), mirror(2 * Y0 - 1 + ty, f.ht)));
+      for(int p = 0; p < ng; p++) pl[p * P1_N + i] = f2h(ll_curve(yl, im.ll.gam[p], im.ll));
+      pl[ng * P1_N + i] = f2h(yl);
+    }
+    __syncthreads();
+    const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
+    if(x >= o.wd || y >= o.ht) return;
+    const int c = 2 * threadIdx.y * P1_W + 2 * threadIdx.x;   // local (2x-1, 2y-1)
+    for(int p = 0; p <= ng; p++)
+    {
+      const __half *q = pl + p * P1_N + c;
+      const float a = h2f(q[0]) + 2.0f * h2f(q[1]) + h2f(q[2]);
+      const float b = h2f(q[P1_W]) + 2.0f * h2f(q[P1_W + 1]) + h2f(q[P1_W + 2]);
+      const float d = h2f(q[2 * P1_W]) + 2.0f * h2f(q[2 * P1_W + 1]) + h2f(q[2 * P1_W + 2]);
+      st_h1(o, x, y, (a + 2.0f * b + d) * (1.0f / 16.0f), p);
+    }
+  });
+}
+
+// reduce level l from l-1, four outputs per thread (256 x 4 per tile)
+__global__ void __launch_bounds__(256) kb_reduce(bt_launch L)
+{
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const dimg &o = im.lp[j.level];
+    const int x0 = ((lt % j.tiles_x) * BX + threadIdx.x) * 4, y = (lt / j.tiles_x) * BY + threadIdx.y;
+    if(x0 < o.wd && y < o.ht) ll_reduce_px4(im.lp[j.level - 1], o, im.ll.num_gamma + 1, x0, y, true);
+  });
+}
+
+// assemble level l (1 <= l <= nl-2) from level l+1
+__global__ void __launch_bounds__(256) kb_asm(bt_launch L)
+{
+  extern __shared__ float tile[];
+  __shared__ int lims[2];
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const int l = j.level, first = l == im.nl - 2;
+    const dimg &fine = im.lp[l];
+    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * ASM_R);
+    const int x = X0 + threadIdx.x, ng = im.ll.num_gamma;
+    float v[ASM_R], res[ASM_R];
+#pragma unroll
+    for(int k = 0; k < ASM_R; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      v[k] = (x < fine.wd && y < fine.ht) ? ld_h1(fine, x, y, ng) : 0.0f;
+    }
+    ll_assemble_tile<ASM_R>(first ? im.lp[l + 1] : im.la[l + 1], im.lp[l + 1], fine.wd, fine.ht, im.ll, first,
+                            X0, Y0, tile, lims, v,
+                            [&](int k, int p) { return ld_h1(fine, x, Y0 + threadIdx.y + BY * k, p); }, res);
+#pragma unroll
+    for(int k = 0; k < ASM_R; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < fine.wd && y < fine.ht) st_h1(im.la[l], x, y, res[k]);
+    }
+  });
+}
+
+// assemble level 0 + llap/colour: level 0 planes from the filmcurv output
+__global__ void __launch_bounds__(256) kb_final(bt_launch L)
+{
+  extern __shared__ float tile[];
+  __shared__ int lims[2];
+  bt_tiles(L, [&](const bt_job &j, int lt) {
+    const bt_img &im = L.img[j.img];
+    const dimg &f = im.film;
+    const int first = im.nl == 2;
+    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * ASM_R);
+    const int x = X0 + threadIdx.x;
+    float4 t[ASM_R];
+    float yl[ASM_R], v[ASM_R], res[ASM_R];
+#pragma unroll
+    for(int k = 0; k < ASM_R; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      t[k] = (x < f.wd && y < f.ht) ? ld_h4(f, x, y) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      yl[k] = ll_lum(t[k]);
+      v[k] = r16(yl[k]);
+    }
+    ll_assemble_tile<ASM_R>(first ? im.lp[1] : im.la[1], im.lp[1], f.wd, f.ht, im.ll, first, X0, Y0, tile, lims, v,
+                            [&](int k, int p) { return r16(ll_curve(yl[k], im.ll.gam[p], im.ll)); }, res);
+#pragma unroll
+    for(int k = 0; k < ASM_R; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < f.wd && y < f.ht) st_f4(im.out, x, y, ll_colour_px(r16(res[k]), t[k]));
+    }
+  });
+}
+
+// ---- host ---------------------------------------------------------------------
+
+namespace {
+
+enum kind_t { K_HALF, K_DOWNCOV, K_DOWN, K_DNASM, K_DOUB, K_GAUSS, K_DEVELOP, K_PYR1, K_REDUCE, K_ASM, K_FINAL };
+const int kind_group[] = { 0, 1, 1, 2, 3, 4, 5, 6, 7, 7, 8 };
+const char *group_name[VKDT_BATCH_STAGES] = { "half", "wavelet", "dnasm", "doub", "gauss", "develop", "pyr1",
+                                              "pyramid", "final" };
+
+struct launch_t
+{
+  int kind, level;
+  int job0, njob, total;
+  size_t lds;          // dynamic LDS bytes
+  int ctr0;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+int    pitch8(int w) { return (w + 7) & ~7; }
+
+// extents of one image's buffers
+struct dims_t
+{
+  int W, H, qw, qh, cw, ch, nl, ng, block;
+  int lw[BT_LEVELS], lh[BT_LEVELS];
+};
+
+int image_dims(const vkdt_develop_image_t &d, dims_t &m, int idx)
+{
+  if(!d.filters)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: not a mosaic (filters 0)", idx);
+  if(!(d.dn_params[0] > 0.0f))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: denoise strength <= 0 (noop path) not batched", idx);
+  if(d.wd < 6 || d.ht < 6 || d.wd > (1u << 16) || d.ht > (1u << 16))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: extent %ux%u outside [6, 65536]", idx, d.wd, d.ht);
+  if(!d.cfa_wd || !d.cfa_ht || d.cfa_pitch < d.cfa_wd)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: mosaic %ux%u pitch %u", idx, d.cfa_wd, d.cfa_ht, d.cfa_pitch);
+  if(d.ll_numgamma < 2 || d.ll_numgamma > LL_MAX_GAMMA)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: num_gamma %d outside [2,%d]", idx, d.ll_numgamma, LL_MAX_GAMMA);
+  if(d.ll_levels < 2 || d.ll_levels > BT_LEVELS)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: %d pyramid levels outside [2,%d]", idx, d.ll_levels, BT_LEVELS);
+  m.W = d.wd; m.H = d.ht;
+  m.block = d.filters == 9u ? 3 : 2;
+  m.qw = m.W / m.block; m.qh = m.H / m.block;   // denoise/main.c: roi / block
+  m.cw = m.W / m.block; m.ch = m.H / m.block;   // demosaic/main.c: wd / block
+  if(m.qw < 1 || m.qh < 1) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: empty block grid", idx);
+  m.nl = d.ll_levels;
+  m.ng = d.ll_numgamma;
+  m.lw[0] = m.W; m.lh[0] = m.H;
+  for(int l = 1; l < m.nl; l++) { m.lw[l] = (m.lw[l - 1] - 1) / 2 + 1; m.lh[l] = (m.lh[l - 1] - 1) / 2 + 1; }
+  return VKDT_HIP_OK;
+}
+
+size_t image_scratch(const dims_t &m)
+{
+  size_t s = 0;
+  const size_t q = align256((size_t)m.qw * m.qh * 8);
+  s += 6 * q;                                              // qh, qd[4], qa
+  s += align256((size_t)pitch8(m.W) * m.H * 2);            // mos
+  s += align256((size_t)m.cw * m.ch * 8);                  // cov
+  s += align256((size_t)m.W * m.H * 8);                    // film
+  for(int l = 1; l < m.nl; l++)
+    s += align256((size_t)(m.ng + 1) * pitch8(m.lw[l]) * m.lh[l] * 2);
+  for(int l = 1; l <= m.nl - 2; l++)
+    s += align256((size_t)pitch8(m.lw[l]) * m.lh[l] * 2);
+  return s;
+}
+
+dimg mk(void *p, int wd, int ht, int pitch, int chan, long pstride)
+{
+  dimg d;
+  d.p = p; d.wd = wd; d.ht = ht; d.pitch = pitch; d.chan = chan; d.pstride = pstride;
+  return d;
+}
+
+} // namespace
+
+struct vkdt_batch_t
+{
+  int n = 0, ncu = 256;
+  bt_img *d_img = nullptr;
+  bt_job *d_job = nullptr;
+  int *d_ctr = nullptr;
+  int nctr = 0;
+  std::vector<launch_t> seq;
+  int32_t tune[VKDT_BATCH_STAGES];
+  asm_mats mats;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int timing = 0;
+  hipEvent_t ev[VKDT_BATCH_STAGES + 1] = {};
+  float ms[VKDT_BATCH_STAGES] = {};
+};
+
+int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t *bytes)
+{
+  if(!bytes || n < 0 || (n && !img)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: bad arguments");
+  size_t s = 0;
+  for(int i = 0; i < n; i++)
+  {
+    dims_t m;
+    if(int e = image_dims(img[i], m, i)) return e;
+    s += image_scratch(m);
+  }
+  *bytes = s;
+  return VKDT_HIP_OK;
+}
+
+static void dn_common(dn_args &a, const vkdt_develop_image_t &d)
+{
+  a = dn_args{};
+  for(int k = 0; k < 4; k++)
+  {
+    a.wb[k] = d.wb[k];
+    a.black[k] = d.black[k];
+    a.white[k] = d.white[k];
+    a.inv_range[k] = 1.0f / (d.white[k] - d.black[k]);
+  }
+  a.noise_a = d.noise_a;
+  a.noise_b = d.noise_b;
+  a.filters = d.filters;
+  dn_params_init(a, d.dn_params);
+}
+
+static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m, char *&sp, int idx)
+{
+  b = bt_img{};
+  b.cfa = mk(const_cast<void *>(d.cfa), d.cfa_wd, d.cfa_ht, d.cfa_pitch, 1, 0);
+  b.out = mk(d.out, m.W, m.H, m.W, 4, 0);
+  const size_t q = align256((size_t)m.qw * m.qh * 8);
+  b.qh = mk(sp, m.qw, m.qh, m.qw, 4, 0); sp += q;
+  for(int l = 0; l < 4; l++) { b.qd[l] = mk(sp, m.qw, m.qh, m.qw, 4, 0); sp += q; }
+  b.qa = mk(sp, m.qw, m.qh, m.qw, 4, 0); sp += q;
+  b.mos = mk(sp, m.W, m.H, pitch8(m.W), 1, 0); sp += align256((size_t)pitch8(m.W) * m.H * 2);
+  b.cov = mk(sp, m.cw, m.ch, m.cw, 4, 0); sp += align256((size_t)m.cw * m.ch * 8);
+  b.film = mk(sp, m.W, m.H, m.W, 4, 0); sp += align256((size_t)m.W * m.H * 8);
+  for(int l = 1; l < m.nl; l++)
+  {
+    const long ps = (long)pitch8(m.lw[l]) * m.lh[l];
+    b.lp[l] = mk(sp, m.lw[l], m.lh[l], pitch8(m.lw[l]), 1, ps);
+    sp += align256((size_t)(m.ng + 1) * ps * 2);
+  }
+  for(int l = 1; l <= m.nl - 2; l++)
+  {
+    b.la[l] = mk(sp, m.lw[l], m.lh[l], pitch8(m.lw[l]), 1, 0);
+    sp += align256((size_t)pitch8(m.lw[l]) * m.lh[l] * 2);
+  }
+  b.nl = m.nl;
+  b.xt = d.filters == 9u;
+  b.dm_r = b.xt ? std::min(3, std::max(2, d.dm_fixup + 2)) : std::min(2, std::max(1, d.dm_fixup + 1));
+  // denoise push constants per node (denoise/main.c create_nodes): half and
+  // doub crop the mosaic, the wavelet levels and assemble run on the block grid
+  dn_common(b.dn_half, d);
+  for(int k = 0; k < 4; k++) b.dn_half.crop[k] = d.crop[k];
+  for(int l = 0; l < 4; l++)
+  {
+    dn_common(b.dn_down[l], d);
+    b.dn_down[l].level = l;
+    b.dn_down[l].block = m.block;
+    dn_level_scale(b.dn_down[l]);
+  }
+  dn_common(b.dn_asm, d);
+  dn_common(b.dn_doub, d);
+  for(int k = 0; k < 4; k++) b.dn_doub.crop[k] = d.crop[k];
+  if(const char *err = co_args_init(b.co, d.co_params))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: colour: %s", idx, err);
+  if(const char *err = fc_args_init(b.fc, d.fc_params))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: filmcurv: %s", idx, err);
+  ll_args_init(b.ll, d.ll_params, m.ng);
+  return VKDT_HIP_OK;
+}
+
+static int tiles_of(int kind, const dims_t &m, int level, int &tx)
+{
+  int w, h, tw = BX, th = BY;
+  switch(kind)
+  {
+    case K_HALF: case K_DOWNCOV: case K_DOWN: case K_DNASM: w = m.qw; h = m.qh; break;
+    case K_DOUB: w = m.W; h = m.H; break;
+    case K_GAUSS: w = m.cw; h = m.ch; break;
+    case K_DEVELOP: w = m.W; h = m.H; th = BY * DEV_R; break;
+    case K_PYR1: w = m.lw[1]; h = m.lh[1]; break;
+    case K_REDUCE: w = m.lw[level]; h = m.lh[level]; tw = 4 * BX; break;
+    case K_ASM: w = m.lw[level]; h = m.lh[level]; th = BY * ASM_R; break;
+    default: w = m.W; h = m.H; th = BY * ASM_R; break;   // K_FINAL
+  }
+  tx = (w + tw - 1) / tw;
+  return tx * ((h + th - 1) / th);
+}
+
+int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, int n, void *scratch,
+                          size_t scratch_bytes, const int32_t *tune)
+{
+  if(!out || n <= 0 || !img) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: no images");
+  *out = nullptr;
+  std::vector<dims_t> dm(n);
+  size_t need = 0;
+  int max_nl = 0, max_ng = 0;
+  for(int i = 0; i < n; i++)
+  {
+    if(int e = image_dims(img[i], dm[i], i)) return e;
+    if(!img[i].cfa || !img[i].out) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: no mosaic or output", i);
+    need += image_scratch(dm[i]);
+    max_nl = std::max(max_nl, dm[i].nl);
+    max_ng = std::max(max_ng, dm[i].ng);
+  }
+  if(!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: scratch %zu bytes at %p, need %zu (256-byte aligned)",
+                          scratch_bytes, scratch, need);
+  std::vector<bt_img> h(n);
+  char *sp = (char *)scratch;
+  for(int i = 0; i < n; i++)
+    if(int e = build_image(h[i], img[i], dm[i], sp, i)) return e;
+
+  vkdt_batch_t *b = new vkdt_batch_t;
+  b->n = n;
+  for(int s = 0; s < VKDT_BATCH_STAGES; s++) b->tune[s] = tune ? tune[s] : 0;
+  asm_mats_init(b->mats);
+  int dev = 0;
+  if(hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->ncu <= 0)
+    b->ncu = 256;
+
+  // launch sequence and job tables
+  std::vector<bt_job> jobs;
+  auto add = [&](int kind, int level, size_t lds) {
+    launch_t L = { kind, level, (int)jobs.size(), 0, 0, lds, b->nctr };
+    for(int i = 0; i < n; i++)
+    {
+      if(kind == K_REDUCE && level >= dm[i].nl) continue;
+      if(kind == K_ASM && level > dm[i].nl - 2) continue;
+      int tx;
+      const int t = tiles_of(kind, dm[i], level, tx);
+      if(t <= 0) continue;
+      jobs.push_back(bt_job{ i, tx, L.total, level });
+      L.total += t;
+      L.njob++;
+    }
+    if(L.njob) { b->seq.push_back(L); b->nctr += NUM_XCD; }
+  };
+  const size_t asm_lds = sizeof(float) * (max_ng + 1) * AS_TN(ASM_R);
+  add(K_HALF, 0, 0);
+  add(K_DOWNCOV, 0, 0);
+  for(int l = 1; l < 4; l++) add(K_DOWN, l, 0);
+  add(K_DNASM, 0, 0);
+  add(K_DOUB, 0, 0);
+  add(K_GAUSS, 0, 0);
+  add(K_DEVELOP, 0, 0);
+  add(K_PYR1, 1, sizeof(__half) * (max_ng + 1) * P1_N);
+  for(int l = 2; l < max_nl; l++) add(K_REDUCE, l, 0);
+  for(int l = max_nl - 2; l >= 1; l--) add(K_ASM, l, asm_lds);
+  add(K_FINAL, 0, asm_lds);
+
+  hipError_t e = hipMalloc((void **)&b->d_img, sizeof(bt_img) * n);
+  if(e == hipSuccess) e = hipMalloc((void **)&b->d_job, sizeof(bt_job) * std::max<size_t>(1, jobs.size()));
+  if(e == hipSuccess) e = hipMalloc((void **)&b->d_ctr, sizeof(int) * std::max(1, b->nctr));
+  if(e == hipSuccess) e = hipMemcpy(b->d_img, h.data(), sizeof(bt_img) * n, hipMemcpyHostToDevice);
+  if(e == hipSuccess && !jobs.empty())
+    e = hipMemcpy(b->d_job, jobs.data(), sizeof(bt_job) * jobs.size(), hipMemcpyHostToDevice);
+  for(int s = 0; e == hipSuccess && s <= VKDT_BATCH_STAGES; s++) e = hipEventCreate(&b->ev[s]);
+  if(e != hipSuccess)
+  {
+    const int rc = hip_fail(e, "batch create");
+    vkdt_hip_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return VKDT_HIP_OK;
+}
+
+static int enqueue(vkdt_batch_t *b, hipStream_t s, int timed)
+{
+  hipLaunchKernelGGL(kb_reset, dim3((b->nctr + 255) / 256), dim3(256), 0, s, b->d_ctr, b->nctr);
+  int group = -1;
+  for(const launch_t &L : b->seq)
+  {
+    const int g = kind_group[L.kind];
+    if(timed && g != group)
+    {
+      for(int k = group + 1; k <= g; k++)
+        if(int e = hip_fail(hipEventRecord(b->ev[k], s), "batch timing")) return e;
+      group = g;
+    }
+    const int persist = b->tune[g];
+    bt_launch a = { b->d_img, b->d_job + L.job0, L.njob, L.total, b->d_ctr + L.ctr0, persist };
+    const dim3 grid(persist ? b->ncu * persist : L.total), blk(BX, BY);
+    switch(L.kind)
+    {
+      case K_HALF:    hipLaunchKernelGGL(kb_half, grid, blk, 0, s, a); break;
+      case K_DOWNCOV: hipLaunchKernelGGL(kb_downcov, grid, blk, 0, s, a); break;
+      case K_DOWN:    hipLaunchKernelGGL(kb_down, grid, blk, 0, s, a); break;
+      case K_DNASM:   hipLaunchKernelGGL(kb_dnasm, grid, blk, 0, s, a, b->mats); break;
+      case K_DOUB:    hipLaunchKernelGGL(kb_doub, grid, blk, 0, s, a); break;
+      case K_GAUSS:   hipLaunchKernelGGL(kb_gauss, grid, blk, 0, s, a); break;
+      case K_DEVELOP: hipLaunchKernelGGL(kb_develop, grid, blk, 0, s, a); break;
+      case K_PYR1:    hipLaunchKernelGGL(kb_pyr1, grid, blk, L.lds, s, a); break;
+      case K_REDUCE:  hipLaunchKernelGGL(kb_reduce, grid, blk, 0, s, a); break;
+      case K_ASM:     hipLaunchKernelGGL(kb_asm, grid, blk, L.lds, s, a); break;
+      default:        hipLaunchKernelGGL(kb_final, grid, blk, L.lds, s, a); break;
+    }
+    if(int e = hip_fail(hipGetLastError(), group_name[g])) return e;
+  }
+  if(timed)
+    for(int k = group + 1; k <= VKDT_BATCH_STAGES; k++)
+      if(int e = hip_fail(hipEventRecord(b->ev[k], s), "batch timing")) return e;
+  return VKDT_HIP_OK;
+}
+
+static void drop_graph(vkdt_batch_t *b)
+{
+  if(b->exec) (void)hipGraphExecDestroy(b->exec);
+  if(b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int vkdt_hip_batch_run(vkdt_batch_t *b, void *stream, int graph)
+{
+  if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
+  hipStream_t s = (hipStream_t)stream;
+  if(!graph || !s || b->timing)
+  { // eager (timing records events between the stage groups)
+    if(int e = enqueue(b, s, b->timing)) return e;
+    if(b->timing)
+    {
+      if(int e = hip_fail(hipEventSynchronize(b->ev[VKDT_BATCH_STAGES]), "batch timing")) return e;
+      for(int k = 0; k < VKDT_BATCH_STAGES; k++)
+        if(int e = hip_fail(hipEventElapsedTime(&b->ms[k], b->ev[k], b->ev[k + 1]), "batch timing")) return e;
+    }
+    return VKDT_HIP_OK;
+  }
+  if(!b->exec)
+  {
+    if(int e = hip_fail(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "batch capture")) return e;
+    const int rc = enqueue(b, s, 0);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if(rc) { if(g) (void)hipGraphDestroy(g); return rc; }
+    if(int e = hip_fail(ee, "batch end capture")) return e;
+    b->graph = g;
+    if(int e = hip_fail(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "batch graph instantiate"))
+    {
+      drop_graph(b);
+      return e;
+    }
+  }
+  return hip_fail(hipGraphLaunch(b->exec, s), "batch graph launch");
+}
+
+int vkdt_hip_batch_set_timing(vkdt_batch_t *b, int on)
+{
+  if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
+  b->timing = on != 0;
+  return VKDT_HIP_OK;
+}
+
+int vkdt_hip_batch_stage_ms(vkdt_batch_t *b, float *ms)
+{
+  if(!b || !ms) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
+  for(int k = 0; k < VKDT_BATCH_STAGES; k++) ms[k] = b->ms[k];
+  return VKDT_HIP_OK;
+}
+
+const char *vkdt_hip_batch_stage_name(int stage)
+{
+  return stage >= 0 && stage < VKDT_BATCH_STAGES ? group_name[stage] : "";
+}
+
+int vkdt_hip_batch_autotune(vkdt_batch_t *b, void *stream, int32_t *tune)
+{
+  if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
+  // launch shape per stage group: 0 = one tile per work group, k = k
+  // persistent work groups per CU pulling from the tile queues
+  static const int cand[] = { 0, 1, 2, 4, 8 };
+  const int nc = sizeof(cand) / sizeof(cand[0]);
+  float best[VKDT_BATCH_STAGES];
+  int32_t pick[VKDT_BATCH_STAGES];
+  for(int k = 0; k < VKDT_BATCH_STAGES; k++) { best[k] = 1e30f; pick[k] = 0; }
+  const int was = b->timing;
+  b->timing = 1;
+  for(int c = 0; c < nc; c++)
+  {
+    for(int k = 0; k < VKDT_BATCH_STAGES; k++) b->tune[k] = cand[c];
+    for(int rep = 0; rep < 3; rep++)
+    {
+      if(int e = vkdt_hip_batch_run(b, stream, 0)) { b->timing = was; return e; }
+      if(rep == 0) continue;   // warm
+      for(int k = 0; k < VKDT_BATCH_STAGES; k++)
+        if(b->ms[k] < best[k]) { best[k] = b->ms[k]; pick[k] = cand[c]; }
+    }
+  }
+  b->timing = was;
+  for(int k = 0; k < VKDT_BATCH_STAGES; k++) b->tune[k] = pick[k];
+  if(tune) for(int k = 0; k < VKDT_BATCH_STAGES; k++) tune[k] = pick[k];
+  drop_graph(b);
+  return VKDT_HIP_OK;
+}
+
+void vkdt_hip_batch_destroy(vkdt_batch_t *b)
+{
+  if(!b) return;
+  drop_graph(b);
+  for(int s = 0; s <= VKDT_BATCH_STAGES; s++) if(b->ev[s]) (void)hipEventDestroy(b->ev[s]);
+  if(b->d_img) (void)hipFree(b->d_img);
+  if(b->d_job) (void)hipFree(b->d_job);
+  if(b->d_ctr) (void)hipFree(b->d_ctr);
+  delete b;
+}

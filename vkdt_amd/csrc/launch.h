@@ -53,6 +53,9 @@ static inline dim3 grid2d(uint32_t wd, uint32_t ht, uint32_t dp)
 
 int launch_post(const char *what);
 
+// maps a HIP error to VKDT_HIP_OOM / VKDT_HIP_DEVICE with its text (OK on success)
+int hip_fail(hipError_t e, const char *what);
+
 // launchers, one per reference kernel
 int launch_denoise_noop(const node_call &);
 int launch_denoise_half(const node_call &);

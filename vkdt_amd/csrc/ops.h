@@ -795,6 +795,8 @@ DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int 
 // luminance selects.  The tile's coarse footprint is staged in LDS only for
 // the planes some texel of the tile selects (the block's [min hi-1, max hi]
 // range), plus the coarse reconstruction in the last slot.
+//   first        the coarsest level: expand the luminance plane of l1
+//                instead of a coarse reconstruction
 //   v[k]         the fine luminance plane at the thread's texel k
 //   plane(k, p)  the fine curve plane p there
 // Texels outside (owd, oht) get no result.
@@ -806,7 +808,7 @@ DEV int wave_min(int v) { for(int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(
 DEV int wave_max(int v) { for(int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o)); return v; }
 
 template<int R, typename PlaneAt>
-DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, const ll_args &a,
+DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, const ll_args &a, int first,
                           int X0, int Y0, float *tile, int *lims, const float v[R], PlaneAt plane, float res[R])
 {
   constexpr int CN = AS_TN(R);
@@ -844,7 +846,7 @@ DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, 
     const int gx = mirror(cx0 + tx, l1.wd), gy = mirror(cy0 + ty, l1.ht);
     float c;
     if(sl < np) c = ld_h1(l1, gx, gy, p0 + sl);
-    else if(a.first > 0) c = ld_h1(l1, gx, gy, ng);
+    else if(first > 0) c = ld_h1(l1, gx, gy, ng);
     else c = ld_h1(coarse, gx, gy, 0);
     tile[idx] = c;
   }

@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include "vkdt_hip.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -312,6 +313,13 @@ int  dt_graph_read_config_line(dt_graph_t *graph, char *line);
 int  dt_graph_run(dt_graph_t *graph, dt_graph_run_t run);
 // module passes and node schedule only (no device work): roi, create_nodes
 int  dt_graph_plan(dt_graph_t *graph, dt_graph_run_t run);
+// every module's commit_params (the uniform blocks) without the device
+void dt_graph_commit(dt_graph_t *graph);
+// after dt_graph_plan: the batched-develop description of this graph's image
+// (commits the params first)
+// every field but the device pointers; the chain must be exactly
+// i-raw > denoise > demosaic > colour > filmcurv > llap.  0 on success.
+int  dt_graph_develop_image(dt_graph_t *graph, vkdt_develop_image_t *img);
 int  dt_module_add(dt_graph_t *graph, dt_token_t name, dt_token_t inst);
 int  dt_module_get(const dt_graph_t *graph, dt_token_t name, dt_token_t inst);
 int  dt_module_connect(dt_graph_t *graph, int m0, int c0, int m1, int c1);

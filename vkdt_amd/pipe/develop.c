@@ -1,0 +1,123 @@
+// develop.c -- one image of the batched develop path, read off a planned
+// graph.  The batch (vkdt_hip_batch_*) runs the raw-development chain
+// i-raw -> denoise -> demosaic -> colour -> filmcurv -> llap with the values
+// this graph's nodes would receive, so that its result equals the graph's.
+#include "api.h"
+
+static const dt_node_t *find_node(const dt_graph_t *g, const char *name, const char *kernel, int *count)
+{
+  const dt_node_t *r = 0;
+  int c = 0;
+  for(int n = 0; n < g->num_nodes; n++)
+    if(g->node[n].name == dt_token(name) && g->node[n].kernel == dt_token(kernel))
+    {
+      if(!r) r = g->node + n;
+      c++;
+    }
+  if(count) *count = c;
+  return r;
+}
+
+static const void *node_params(const dt_node_t *n, int *size)
+{
+  const dt_module_t *m = n->module;
+  if(m->committed_param_size) { *size = m->committed_param_size; return m->committed_param; }
+  *size = m->param_size;
+  return m->param;
+}
+
+// the module instance whose output feeds module m's connector `input`
+static const dt_module_t *upstream(const dt_graph_t *g, const dt_module_t *m)
+{
+  const int c = dt_module_get_connector(m, dt_token("input"));
+  if(c < 0 || dt_cid_unset(m->connector[c].connected)) return 0;
+  return g->module + m->connector[c].connected.i;
+}
+
+static int fail(dt_graph_t *g, const char *msg)
+{
+  snprintf(g->err, sizeof(g->err), "develop image: %s", msg);
+  return 1;
+}
+
+int dt_graph_develop_image(dt_graph_t *g, vkdt_develop_image_t *d)
+{
+  memset(d, 0, sizeof(*d));
+  dt_graph_commit(g);
+  const dt_node_t *doub = find_node(g, "denoise", "doub", 0);
+  const dt_node_t *half = find_node(g, "denoise", "half", 0);
+  const dt_node_t *gauss = find_node(g, "demosaic", "gauss", 0);
+  const dt_node_t *fix = find_node(g, "demosaic", "fix", 0);
+  const dt_node_t *col = find_node(g, "colour", "main", 0);
+  const dt_node_t *fc = find_node(g, "filmcurv", "main", 0);
+  const dt_node_t *curve = find_node(g, "llap", "curve", 0);
+  const dt_node_t *llc = find_node(g, "llap", "colour", 0);
+  int nreduce = 0;
+  find_node(g, "llap", "reduce", &nreduce);
+  if(!doub || !half) return fail(g, "denoise is not the mosaic wavelet path (strength > 0 on a raw)");
+  if(!gauss || !fix) return fail(g, "demosaic is not the gaussian splat path");
+  if(!col || !fc || !curve || !llc) return fail(g, "colour, filmcurv or llap missing");
+  // the chain must be exactly i-raw -> denoise -> demosaic -> colour -> filmcurv -> llap
+  const dt_module_t *chain[6] = { llc->module, fc->module, col->module, fix->module, doub->module, 0 };
+  for(int k = 0; k < 5; k++)
+  {
+    const dt_module_t *up = upstream(g, chain[k]);
+    if(!up || (k < 4 && up != chain[k + 1]))
+      return fail(g, "the modules are not the chain i-raw > denoise > demosaic > colour > filmcurv > llap");
+    if(k == 4)
+    {
+      if(up->name != dt_token("i-raw")) return fail(g, "denoise does not read i-raw");
+      chain[5] = up;
+    }
+  }
+  for(int n = 0; n < g->num_nodes; n++)
+  {
+    const dt_node_t *nd = g->node + n;
+    if(nd->culled) continue;
+    const dt_token_t mn = nd->name;
+    if(mn != dt_token("denoise") && mn != dt_token("demosaic") && mn != dt_token("colour") &&
+       mn != dt_token("filmcurv") && mn != dt_token("llap") && nd->module != chain[5] &&
+       nd->module->connector[0].type != dt_token("sink"))
+      return fail(g, "the graph runs nodes outside the raw-development chain");
+  }
+
+  const uint32_t *pc = doub->push_constant;
+  const dt_module_t *raw = chain[5];
+  d->cfa_wd = raw->connector[0].roi.wd;
+  d->cfa_ht = raw->connector[0].roi.ht;
+  d->cfa_pitch = d->cfa_wd;
+  d->wd = doub->wd;
+  d->ht = doub->ht;
+  for(int k = 0; k < 4; k++)
+  {
+    memcpy(d->wb + k, pc + k, 4);
+    memcpy(d->black + k, pc + 4 + k, 4);
+    memcpy(d->white + k, pc + 8 + k, 4);
+    d->crop[k] = (int32_t)pc[12 + k];
+  }
+  d->filters = pc[16];
+  memcpy(&d->noise_a, pc + 17, 4);
+  memcpy(&d->noise_b, pc + 18, 4);
+  if(pc[19]) return fail(g, "dng gain maps are not supported");
+
+  int sz = 0;
+  const void *p = node_params(doub, &sz);
+  if(sz < 32) return fail(g, "denoise params");
+  memcpy(d->dn_params, p, 32);
+  p = node_params(fix, &sz);
+  d->dm_fixup = sz >= 4 ? ((const int32_t *)p)[0] : 0;
+  p = node_params(col, &sz);
+  if(sz < (int)sizeof(d->co_params)) return fail(g, "colour params");
+  memcpy(d->co_params, p, sizeof(d->co_params));
+  p = node_params(fc, &sz);
+  if(sz < 16) return fail(g, "filmcurv params");
+  memcpy(d->fc_params, p, 16);
+  p = node_params(curve, &sz);
+  if(sz < 16) return fail(g, "llap params");
+  memcpy(d->ll_params, p, 16);
+  d->ll_numgamma = (int32_t)curve->push_constant[0];
+  d->ll_levels = nreduce + 1;
+  if(llc->wd != d->wd || llc->ht != d->ht || fix->wd != d->wd || fix->ht != d->ht)
+    return fail(g, "demosaic or llap extent differs from the denoise output");
+  return 0;
+}
