@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the raw-development path on one MI355X.
 
-Default workload: the plain per-image path on a 24 MP Bayer still
-(6000 x 4000 RGGB, 14-bit in uint16, black 512, white 16383, seeded scene of
-smooth gradients + 200 rectangles over 12 stops, 2% clipped, Poisson-Gaussian
-noise a=2e-4 b=1e-6) through the graph i-raw -> denoise -> demosaic -> colour
--> filmcurv -> llap -> o-pfm, run by the host runtime (module DSOs, one HIP
-launch per node).  The DNG is decoded and uploaded once before timing; a
-step is one pass of every node over the resident raw (no host transfer).
+--mode batch (default): 64 stills of mixed sizes and CFAs -- 16x 4000x3000,
+24x 6000x4000, 16x 8256x5504, 8x 9504x6336, a quarter of them X-Trans by a
+seeded shuffle, 14-bit in uint16, per-image seeded scenes (smooth gradients +
+200 rectangles over 12 stops, 2% clipped) with Poisson-Gaussian noise
+a ~ U[1e-4, 1e-3], b = 1e-6 -- developed through i-raw -> denoise ->
+demosaic -> colour -> filmcurv -> llap as one batched hipGraph (one launch
+per stage over all 64, mosaics resident in HBM).  A step is one replay of
+that graph.  The same 64 images through the plain per-image path (one
+per-node graph per image) are reported beside it.
 
-Prints one JSON line: value is Mpix/s
-(CFA pixels after i-raw block rounding per second), roofline is for the
-node kernel with the largest share of the step, measured with HIP events
-on the graph's stream inside the timed region, cpu_baseline times the
-oracle on the host on a bounded sample (the 2048 x 1536 case).
+--mode plain: the per-image drop-in path on one 24 MP Bayer still
+(6000 x 4000) or X-Trans still (6240 x 4160), run by the host runtime
+(module DSOs, one HIP launch per node).  The DNG is decoded and uploaded
+once before timing; a step is one pass of every node over the resident raw.
+
+Prints one JSON line: value is CFA Mpix/s (after i-raw block rounding);
+roofline is for the kernel or stage with the largest share of a step,
+timed with HIP events on the stream it runs on inside the measured run;
+cpu_baseline times the oracle on the host on a bounded sample (the
+2048 x 1536 Bayer case).
 """
 import argparse
 import json
@@ -94,30 +101,16 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--breakdown", default="", help="write per-kernel timings as JSON to this path")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a plain device copy of 1 GiB (5 times), the reference kernel "
-                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
-    args = ap.parse_args()
-    if args.calibrate_copy:
-        calibrate_copy()
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(images are independent units and would shard across GPUs with no communication)",
-              file=sys.stderr)
-        sys.exit(2)
+def plain_graph(dng, kind, noise_a, noise_b, out_dir, extra=()):
+    from vkdt_amd import pipe
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=noise_a, noise_b=noise_b, extra=list(extra)))
+    g.export("o-pfm", os.path.join(out_dir, "out"))
+    return g
 
-    import numpy as np
+
+def run_plain(args):
+    """the per-image drop-in path on one still, per-node timings"""
     from vkdt_amd import hip, pipe, synth
-    hip.init(0)
     if args.kind == "bayer":
         W, H, workload = 6000, 4000, "plain per-image path, 24 MP Bayer RGGB still 6000x4000"
     else:
@@ -128,8 +121,7 @@ def main():
     dng = os.path.join(tmp, "still.dng")
     synth.write_dng(dng, cfa, args.kind)
     extra = ["param:llap:01:numgamma:8", "param:llap:01:levels:6"] if args.kind == "xtrans" else []
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb, extra=extra))
-    g.export("o-pfm", os.path.join(tmp, "out"))
+    g = plain_graph(dng, args.kind, na, nb, tmp, extra)
     g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)        # plan, allocate, upload, first pass
     ip = g.img_param("i-raw", "main")
     P = ip["full"][0] * ip["full"][1]
@@ -177,8 +169,9 @@ def main():
         with open(args.breakdown, "w") as f:
             json.dump({k: dict(v, ms_per_step=v["ms"] / args.steps,
                                gbs=v["bytes"] / (v["ms"] * 1e-3) / 1e9) for k, v in per.items()}, f, indent=1)
+    g.close()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
-    line = {
+    return {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -188,8 +181,199 @@ def main():
                    "graph": "i-raw>denoise(strength .5)>demosaic>colour>filmcurv>llap(clarity .2)>o-pfm"},
         "roofline": roof, "graph_roofline": graph_roof, "cpu_baseline": cpu,
     }
+
+
+# 64 stills: 16x 12 MP, 24x 24 MP, 16x 45 MP, 8x 61 MP
+BATCH_SIZES = [(4000, 3000, 16), (6000, 4000, 24), (8256, 5504, 16), (9504, 6336, 8)]
+
+
+def batch_workload(seed=2024):
+    """(W, H, kind, a) per image: 25% X-Trans by a seeded shuffle, gaussian
+    noise term a ~ U[1e-4, 1e-3], poissonian b = 1e-6"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = [(W, H) for W, H, n in BATCH_SIZES for _ in range(n)]
+    xt = np.zeros(len(shapes), bool)
+    xt[rng.permutation(len(shapes))[:len(shapes) // 4]] = True
+    a = rng.uniform(1e-4, 1e-3, len(shapes))
+    return [(W, H, "xtrans" if x else "bayer", float(ai)) for (W, H), x, ai in zip(shapes, xt, a)]
+
+
+def stage_bytes(d):
+    """compulsory HBM bytes of each batch stage group for one image: each
+    stored input read once, each stored output written once (the fused
+    stages' internal values never reach HBM)"""
+    W, H, ng, nl = d.wd, d.ht, d.ll_numgamma, d.ll_levels
+    b = 3 if d.filters == 9 else 2
+    Q = (W // b) * (H // b)
+    lw, lh = [W], [H]
+    for _ in range(1, nl):
+        lw.append((lw[-1] - 1) // 2 + 1)
+        lh.append((lh[-1] - 1) // 2 + 1)
+    L = [lw[l] * lh[l] for l in range(nl)]
+    pyr = sum((ng + 1) * 2 * (L[l - 1] + L[l]) for l in range(2, nl))            # reduce 2..
+    pyr += sum(4 * 2 * L[l] + 3 * 2 * L[l + 1] for l in range(1, nl - 1))        # assemble nl-2..1
+    return [2 * W * H + 8 * Q,                  # half: mosaic -> Q rgba
+            4 * 16 * Q,                         # downcov + 3 down
+            6 * 8 * Q,                          # assemble: 5 levels -> Q
+            4 * W * H + 16 * Q,                 # doub: mosaic + 2 Q -> mosaic f16
+            2 * W * H + 8 * Q,                  # gauss: mosaic f16 -> block covariance
+            10 * W * H + 8 * Q,                 # develop: mosaic + cov -> rgba f16
+            8 * W * H + (ng + 1) * 2 * L[1],    # pyr1: rgba f16 -> level 1 planes
+            pyr,                                # pyramid levels
+            24 * W * H + 3 * 2 * L[1]]          # final: rgba f16 + level 1 -> rgba f32
+
+
+def run_batch(args):
+    """64 mixed stills in one batched hipGraph vs the plain per-image path"""
+    import numpy as np
+    import torch
+    from vkdt_amd import batch, hip, pipe, synth, synth_gpu
+    hip.init(0)
+    torch.cuda.init()
+    work = batch_workload()
+    tmp = tempfile.mkdtemp(prefix="vkdt_batch_")
+    combos = sorted(set((W, H, k) for W, H, k, _ in work))
+    dngs = {}
+    for ci, (W, H, k) in enumerate(combos):     # one DNG per shape and CFA: metadata + the plain run's input
+        cfa, _ = synth_gpu.mosaic(W, H, 900 + ci, k)
+        dngs[(W, H, k)] = os.path.join(tmp, f"s{ci}.dng")
+        synth.write_dng(dngs[(W, H, k)], cfa.cpu().numpy().view(np.uint16), k)
+        del cfa
+    print(f"[bench] wrote {len(combos)} DNGs", file=sys.stderr, flush=True)
+
+    # plain per-image path: each shape/CFA timed (median of 3 resident runs)
+    plain_ms = {}
+    for (W, H, k) in combos:
+        na, nb = synth.noise_params(5e-4, 1e-6)
+        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp)
+        g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)
+        g.run(pipe.RUN_RECORD)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            g.run(pipe.RUN_RECORD)
+            hip.sync(g.stream())
+            ts.append((time.perf_counter() - t0) * 1e3)
+        plain_ms[(W, H, k)] = sorted(ts)[1]
+        g.close()
+    print(f"[bench] plain {dict((f'{W}x{H}{k[0]}', round(v, 2)) for (W, H, k), v in plain_ms.items())}",
+          file=sys.stderr, flush=True)
+
+    # the batch: descriptors from each image's planned graph, mosaics on the GPU
+    descs, keep = [], []
+    for i, (W, H, k, a) in enumerate(work):
+        na, nb = synth.noise_params(a, 1e-6)
+        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp)
+        g.plan()
+        d = batch.describe(g)
+        g.close()
+        # the mosaic as i-raw stages it: the block-rounded extent of the DNG
+        cfa, _ = synth_gpu.mosaic(d.cfa_wd, d.cfa_ht, i, k, a=a, b=1e-6)
+        out = torch.empty((d.ht, d.wd, 4), dtype=torch.float32, device="cuda")
+        d.cfa, d.out = cfa.data_ptr(), out.data_ptr()
+        descs.append(d)
+        keep += [cfa, out]
+        if (i + 1) % 16 == 0:
+            print(f"[bench] planned and synthesised {i + 1} images", file=sys.stderr, flush=True)
+    P = sum(d.wd * d.ht for d in descs)
+    nbytes = batch.scratch_bytes(descs)
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    print(f"[bench] {len(descs)} images, {P / 1e6:.0f} Mpix, scratch {nbytes / 2**30:.1f} GiB",
+          file=sys.stderr, flush=True)
+    stream = hip.Stream()
+    b = batch.Batch(descs, scratch.data_ptr(), nbytes)
+    tune = b.autotune(stream.ptr)
+    b.set_timing(True)
+    b.run(stream.ptr)
+    stage_ms = b.stage_ms()
+    b.set_timing(False)
+    for _ in range(args.warmup):
+        b.run(stream.ptr, graph=True)
+    stream.sync()
+    e0, e1 = hip.Event(), hip.Event()
+    t0 = time.perf_counter()
+    e0.record(stream.ptr)
+    for _ in range(args.steps):
+        b.run(stream.ptr, graph=True)
+    e1.record(stream.ptr)
+    stream.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1) / args.steps
+    ms_step = wall * 1e3 / args.steps
+    value = P / (ms_step * 1e-3) / 1e6
+    b.close()
+
+    names = batch.stage_names()
+    sb = [sum(x) for x in zip(*[stage_bytes(d) for d in descs])]
+    dom = max(range(len(names)), key=lambda s: stage_ms[s])
+    achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    plain_total = sum(plain_ms[(W, H, k)] for W, H, k, _ in work)
+    plain_value = P / (plain_total * 1e-3) / 1e6
+    roof = {"bound": "hbm", "kernel": "batch/" + names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(sb[dom]), "avg_launch_ms": round(stage_ms[dom], 4)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    return {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scenes generated on the GPU, Poisson-Gaussian noise)",
+        "config": {"workload": "batched: 64 mixed stills in one hipGraph (16x 4000x3000, 24x 6000x4000, "
+                               "16x 8256x5504, 8x 9504x6336; 16 X-Trans by seeded shuffle; noise a~U[1e-4,1e-3])",
+                   "images": len(descs), "cfa_pixels": P, "storage": "f16 intermediates, rgba f32 out",
+                   "graph": "i-raw>denoise(strength .5)>demosaic>colour>filmcurv>llap(clarity .2)",
+                   "tune": dict(zip(names, tune))},
+        "gpu_ms_per_step": round(gpu_ms, 3),
+        "plain": {"value": round(plain_value, 2), "unit": "Mpix/s", "ms_64": round(plain_total, 2),
+                  "method": "per-node graph, resident raw; each of the 8 shape/CFA pairs timed (median of 3) "
+                            "and summed over the 64 images"},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "stages_ms": dict(zip(names, [round(m, 3) for m in stage_ms])),
+        "roofline": roof,
+        "graph_roofline": {"compulsory_bytes": int(sum(sb)), "bytes_per_px": round(sum(sb) / P, 1),
+                           "gpu_ms": round(sum(stage_ms), 3),
+                           "achieved": round(sum(sb) / (sum(stage_ms) * 1e-3) / 1e9, 1),
+                           "frac": round(sum(sb) / (sum(stage_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 20 plain, 5 batch)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default: 3 plain, 2 batch)")
+    ap.add_argument("--mode", default="batch", choices=["batch", "plain"],
+                    help="batch: 64 mixed stills in one batched hipGraph vs the plain path; "
+                         "plain: the per-image path on one still")
+    ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", default="", help="plain mode: write per-kernel timings as JSON to this path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB (5 times), the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(images are independent units and would shard across GPUs with no communication)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = 5 if args.mode == "batch" else 20
+    if args.warmup is None:
+        args.warmup = 2 if args.mode == "batch" else 3
+    if args.mode == "plain":
+        from vkdt_amd import hip
+        hip.init(0)
+        line = run_plain(args)
+    else:
+        line = run_batch(args)
     print(json.dumps(line), flush=True)
-    g.close()
 
 
 if __name__ == "__main__":

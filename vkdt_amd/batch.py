@@ -91,6 +91,9 @@ class Batch:
         hip.check(self.L.vkdt_hip_batch_run(self.ptr, stream, 1 if graph else 0), "batch run")
 
     def autotune(self, stream=None):
+        """time the launch shapes of each stage group and keep the fastest;
+        returns them as ints: persistent work groups per CU (0: one tile per
+        work group) | rows per thread of the per-pixel stages << 8"""
         t = (C.c_int32 * STAGES)()
         hip.check(self.L.vkdt_hip_batch_autotune(self.ptr, stream, t), "batch autotune")
         return list(t)

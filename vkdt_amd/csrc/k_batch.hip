@@ -32,7 +32,9 @@
 #include "ops.h"
 
 #define BT_LEVELS 12
-#define DEV_R 2     // develop: 64 x 8 texels per tile
+#define COV_R 2     // downcov: 64 x 8 texels per tile
+#define DOWN_R 4    // down: 64 x 16 texels per tile
+#define DEV_R 4     // develop: 64 x 16 texels per tile
 #define ASM_R 4     // assemble / final: 64 x 16 texels per tile
 
 struct bt_img
@@ -57,10 +59,28 @@ struct bt_launch
   const bt_job *job;
   int njob, total;
   int *ctr;          // NUM_XCD tile counters (persistent launches)
-  int persist;
+  int persist;       // 0: one tile per work group, k: k persistent work groups per CU
+  int rows;          // per-pixel stages: rows per thread (tile 64 x 4 rows)
 };
 
 // ---- tile scheduling --------------------------------------------------------
+
+// values every lane of the work group holds alike (job and image records):
+// kept in scalar registers
+DEV int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+DEV long rfl(long v)
+{
+  return (long)(((uint64_t)(uint32_t)rfl((int)((uint64_t)v >> 32)) << 32) | (uint32_t)rfl((int)(uint64_t)v));
+}
+DEV dimg rfl(const dimg &d)
+{
+  dimg r;
+  r.p = (void *)rfl((long)d.p);
+  r.wd = rfl(d.wd); r.ht = rfl(d.ht); r.pitch = rfl(d.pitch); r.chan = rfl(d.chan);
+  r.pstride = rfl(d.pstride);
+  return r;
+}
+DEV bt_job rfl(const bt_job &j) { return bt_job{ rfl(j.img), rfl(j.tiles_x), rfl(j.start), rfl(j.level) }; }
 
 DEV int bt_find(const bt_job *job, int n, int t)
 {
@@ -90,10 +110,13 @@ DEV int bt_next(const bt_launch &L, int &tried)
   return L.total;
 }
 
-// run fn(job, local tile) over the launch's tiles.  fn may stage LDS with
-// __syncthreads (all threads call it with the same tile).
-template<typename F>
-DEV void bt_tiles(const bt_launch &L, F fn)
+// Run the launch's tiles.  setup(job) reads an image's arguments (LDS
+// copies, buffer views in registers) and runs once per work group and
+// image; fn(job, local tile) does one tile.  Both may use __syncthreads
+// (every thread calls them alike).  Persistent work groups keep the image
+// of their last tile and redo setup only when the queue moves them on.
+template<typename S, typename F>
+DEV void bt_tiles(const bt_launch &L, S setup, F fn)
 {
   if(!L.persist)
   { // one tile per work group, XCD k on the k-th eighth of the list
@@ -101,12 +124,15 @@ DEV void bt_tiles(const bt_launch &L, F fn)
     const int q = n / NUM_XCD, r = n - q * NUM_XCD, k = b % NUM_XCD;
     const int t = k * q + min(k, r) + b / NUM_XCD;
     if(t >= L.total) return;
-    const int j = bt_find(L.job, L.njob, t);
-    fn(L.job[j], t - L.job[j].start);
+    const bt_job jb = rfl(L.job[bt_find(L.job, L.njob, t)]);
+    setup(jb);
+    __syncthreads();
+    fn(jb, t - jb.start);
     return;
   }
   __shared__ int s_t;
-  int tried = 0;
+  int tried = 0, js = 0, je = 0;   // tile range [js, je) of the current image
+  bt_job jb = { 0, 1, 0, 0 };
   for(;;)
   {
     __syncthreads();   // the previous tile is done with s_t and its LDS
@@ -114,8 +140,16 @@ DEV void bt_tiles(const bt_launch &L, F fn)
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_t);
     if(t >= L.total) break;
-    const int j = bt_find(L.job, L.njob, t);
-    fn(L.job[j], t - L.job[j].start);
+    if(t < js || t >= je)
+    {
+      const int j = bt_find(L.job, L.njob, t);
+      jb = rfl(L.job[j]);
+      js = jb.start;
+      je = rfl(j + 1 < L.njob ? L.job[j + 1].start : L.total);
+      setup(jb);
+      __syncthreads();
+    }
+    fn(jb, t - js);
   }
 }
 
@@ -125,54 +159,110 @@ __global__ void kb_reset(int *ctr, int n)
   if(i < n) ctr[i] = 0;
 }
 
+// Per-image arguments live in global memory, which the stages' stores may
+// alias as far as the compiler knows: read after a store they would be
+// re-fetched with vector loads on every use.  setup() therefore copies a
+// stage's argument block into LDS and its buffer views into registers.
+template<typename T>
+DEV void lds_copy(T &dst, const T &src)
+{
+  static_assert(sizeof(T) % 4 == 0, "word copy");
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for(int i = tid; i < (int)(sizeof(T) / 4); i += BX * BY) ((int *)&dst)[i] = ((const int *)&src)[i];
+}
+
+// per-pixel stages: tiles of 64 x 4R texels, R = L.rows (1..8)
+#define PIX_TILE(L, jb, lt, X0, Y0) \
+  const int X0 = ((lt) % (jb).tiles_x) * BX, Y0 = ((lt) / (jb).tiles_x) * (BY * (L).rows)
+
 // ---- denoise -----------------------------------------------------------------
 
 __global__ void __launch_bounds__(256) kb_half(bt_launch L)
 {
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
-    if(x < im.qh.wd && y < im.qh.ht) st_h4(im.qh, x, y, dn_half_px(im.cfa, im.dn_half, x, y));
+  __shared__ dn_args a;
+  dimg cfa, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.dn_half);
+    cfa = rfl(im.cfa); o = rfl(im.qh);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, dn_half_px(cfa, a, x, y));
+    }
   });
 }
 
 __global__ void __launch_bounds__(256) kb_downcov(bt_launch L)
 {
-  __shared__ dn_cov_smem s;
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
+  __shared__ dn_cov_smem<COV_R> s;
+  __shared__ dn_args a;
+  dimg in, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.dn_down[0]);
+    in = rfl(im.qh); o = rfl(im.qd[0]);
+  }, [&](const bt_job &jb, int lt) {
     const dimg nocov = {};
-    dn_downcov_tile(im.qh, im.qd[0], nocov, im.dn_down[0], (lt % j.tiles_x) * BX, (lt / j.tiles_x) * BY, s);
+    dn_downcov_tile<COV_R>(in, o, nocov, a, (lt % jb.tiles_x) * BX, (lt / jb.tiles_x) * (BY * COV_R), s);
   });
 }
 
 __global__ void __launch_bounds__(256) kb_down(bt_launch L)
 {
-  __shared__ float4 tile[TN];
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int l = j.level;
-    dn_down_tile(im.qd[l - 1], im.qd[l], im.dn_down[l], (lt % j.tiles_x) * BX, (lt / j.tiles_x) * BY, tile);
+  __shared__ float4 tile[DN_TN(DOWN_R)];
+  __shared__ dn_args a;
+  dimg in, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.dn_down[jb.level]);
+    in = rfl(im.qd[jb.level - 1]); o = rfl(im.qd[jb.level]);
+  }, [&](const bt_job &jb, int lt) {
+    dn_down_tile<DOWN_R>(in, o, a, (lt % jb.tiles_x) * BX, (lt / jb.tiles_x) * (BY * DOWN_R), tile);
   });
 }
 
 __global__ void __launch_bounds__(256) kb_dnasm(bt_launch L, asm_mats m)
 {
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
-    if(x < im.qa.wd && y < im.qa.ht)
-      st_h4(im.qa, x, y, dn_assemble_px(im.qh, im.qd[0], im.qd[1], im.qd[2], im.qd[3], im.dn_asm, m, x, y));
+  __shared__ dn_args a;
+  dimg s0, s1, s2, s3, s4, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.dn_asm);
+    s0 = rfl(im.qh); s1 = rfl(im.qd[0]); s2 = rfl(im.qd[1]); s3 = rfl(im.qd[2]); s4 = rfl(im.qd[3]); o = rfl(im.qa);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, dn_assemble_px(s0, s1, s2, s3, s4, a, m, x, y));
+    }
   });
 }
 
 __global__ void __launch_bounds__(256) kb_doub(bt_launch L)
 {
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
-    if(x < im.mos.wd && y < im.mos.ht)
-      st_h1(im.mos, x, y, dn_doub_px(im.cfa, im.qa, im.qh, im.mos.wd, im.mos.ht, im.dn_doub, x, y));
+  __shared__ dn_args a;
+  dimg cfa, crs0, crs1, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.dn_doub);
+    cfa = rfl(im.cfa); crs0 = rfl(im.qa); crs1 = rfl(im.qh); o = rfl(im.mos);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h1(o, x, y, dn_doub_px(cfa, crs0, crs1, o.wd, o.ht, a, x, y));
+    }
   });
 }
 
@@ -180,50 +270,101 @@ __global__ void __launch_bounds__(256) kb_doub(bt_launch L)
 
 __global__ void __launch_bounds__(256) kb_gauss(bt_launch L)
 {
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int x = (lt % j.tiles_x) * BX + threadIdx.x, y = (lt / j.tiles_x) * BY + threadIdx.y;
-    if(x >= im.cov.wd || y >= im.cov.ht) return;
-    st_h4(im.cov, x, y, im.xt ? dm_gauss_px<1>(im.mos, x, y) : dm_gauss_px<0>(im.mos, x, y));
+  dimg in, o;
+  int xt = 0;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    in = rfl(im.mos); o = rfl(im.cov); xt = rfl(im.xt);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, xt ? dm_gauss_px<1>(in, x, y) : dm_gauss_px<0>(in, x, y));
+    }
   });
 }
 
 // green of the tile plus an r-texel halo at mirrored coordinates (what the
 // per-node fix reads from the stored splat output), then fix, colour and
-// filmcurv per texel with the f16 rounding of each stored node output
+// filmcurv per texel with the f16 rounding of each stored node output.
+// Tiles whose footprint -- the fix radius r plus the splat radius rs around
+// the tile -- lies inside the image first stage that mosaic window in LDS
+// (no mirroring or border wrap can occur there); border tiles read the
+// mosaic through the bounds-checked global path.  Both see the same values.
+#define DEV_M 5   // r + rs <= 3 + 2
 template<int XT>
-DEV void develop_tile(const bt_img &im, int X0, int Y0, float *gt)
+DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const co_args &co, const fc_args &fc,
+                      int r, int X0, int Y0, float *gt, float *mt)
 {
-  const int r = im.dm_r;
+  constexpr int rs = XT ? 2 : 1;
   const int gw = BX + 2 * r, gh = BY * DEV_R + 2 * r;
-  const int W = im.film.wd, H = im.film.ht;
+  const int W = film.wd, H = film.ht;
+  const int m = r + rs, mw = BX + 2 * m, mh = BY * DEV_R + 2 * m;
   const int tid = threadIdx.y * BX + threadIdx.x;
+  const int x = X0 + threadIdx.x;
+  auto green = [&](int px, int py) { return gt[(py - Y0 + r) * gw + (px - X0 + r)]; };
+  if(X0 - m >= 0 && Y0 - m >= 0 && X0 + BX + m <= W && Y0 + BY * DEV_R + m <= H)
+  {
+    for(int i = tid; i < mw * mh; i += BX * BY)
+    {
+      const int ty = i / mw, tx = i - ty * mw;
+      mt[i] = ld_h1(mos, X0 - m + tx, Y0 - m + ty);
+    }
+    __syncthreads();
+    auto mosl = [&](int px, int py) { return mt[(py - Y0 + m) * mw + (px - X0 + m)]; };
+    for(int i = tid; i < gw * gh; i += BX * BY)
+    {
+      const int ty = i / gw, tx = i - ty * gw;
+      gt[i] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
+    }
+    __syncthreads();
+#pragma unroll 1
+    for(int k = 0; k < DEV_R; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      const float4 f = dm_fix_at<XT>(mosl, cov, r, x, y, green);
+      const float4 c = r16(co_px(r16(f), co));
+      st_h4(film, x, y, fc_px(c, fc));
+    }
+    return;
+  }
   for(int i = tid; i < gw * gh; i += BX * BY)
   {
     const int ty = i / gw, tx = i - ty * gw;
-    gt[i] = r16(dm_splat_px<XT>(im.mos, im.cov, mirror(X0 - r + tx, W), mirror(Y0 - r + ty, H)));
+    gt[i] = r16(dm_splat_px<XT>(mos, cov, mirror(X0 - r + tx, W), mirror(Y0 - r + ty, H)));
   }
   __syncthreads();
-  const int x = X0 + threadIdx.x;
+#pragma unroll 1
   for(int k = 0; k < DEV_R; k++)
   {
     const int y = Y0 + threadIdx.y + BY * k;
     if(x >= W || y >= H) continue;
-    const float4 f = dm_fix_px<XT>(im.mos, im.cov, r, x, y,
-                                   [&](int px, int py) { return gt[(py - Y0 + r) * gw + (px - X0 + r)]; });
-    const float4 c = r16(co_px(r16(f), im.co));
-    st_h4(im.film, x, y, fc_px(c, im.fc));
+    const float4 f = dm_fix_px<XT>(mos, cov, r, x, y, green);
+    const float4 c = r16(co_px(r16(f), co));
+    st_h4(film, x, y, fc_px(c, fc));
   }
 }
 
 __global__ void __launch_bounds__(256) kb_develop(bt_launch L)
 {
-  __shared__ float gt[(BX + 6) * (BY * DEV_R + 6)];   // halo r <= 3
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * DEV_R);
-    if(im.xt) develop_tile<1>(im, X0, Y0, gt);
-    else      develop_tile<0>(im, X0, Y0, gt);
+  __shared__ float gt[(BX + 6) * (BY * DEV_R + 6)];                 // green, halo r <= 3
+  __shared__ float mt[(BX + 2 * DEV_M) * (BY * DEV_R + 2 * DEV_M)];  // mosaic, halo r + rs
+  __shared__ co_args co;
+  __shared__ fc_args fc;
+  dimg mos, cov, film;
+  int xt = 0, r = 1;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(co, im.co);
+    lds_copy(fc, im.fc);
+    mos = rfl(im.mos); cov = rfl(im.cov); film = rfl(im.film); xt = rfl(im.xt); r = rfl(im.dm_r);
+  }, [&](const bt_job &jb, int lt) {
+    const int X0 = (lt % jb.tiles_x) * BX, Y0 = (lt / jb.tiles_x) * (BY * DEV_R);
+    if(xt) develop_tile<1>(mos, cov, film, co, fc, r, X0, Y0, gt, mt);
+    else   develop_tile<0>(mos, cov, film, co, fc, r, X0, Y0, gt, mt);
   });
 }
 
@@ -240,17 +381,21 @@ __global__ void __launch_bounds__(256) kb_develop(bt_launch L)
 __global__ void __launch_bounds__(256) kb_pyr1(bt_launch L)
 {
   extern __shared__ __half pl[];   // (num_gamma + 1) x P1_N
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const dimg &f = im.film, &o = im.lp[1];
-    const int ng = im.ll.num_gamma;
-    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * BY;
+  __shared__ ll_args a;
+  dimg f, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.ll);
+    f = rfl(im.film); o = rfl(im.lp[1]);
+  }, [&](const bt_job &jb, int lt) {
+    const int ng = a.num_gamma;
+    const int X0 = (lt % jb.tiles_x) * BX, Y0 = (lt / jb.tiles_x) * BY;
     const int tid = threadIdx.y * BX + threadIdx.x;
     for(int i = tid; i < P1_N; i += BX * BY)
     {
       const int ty = i / P1_W, tx = i - ty * P1_W;
       const float yl = ll_lum(ld_h4(f, mirror(2 * X0 - 1 + tx, f.wd), mirror(2 * Y0 - 1 + ty, f.ht)));
-      for(int p = 0; p < ng; p++) pl[p * P1_N + i] = f2h(ll_curve(yl, im.ll.gam[p], im.ll));
+      for(int p = 0; p < ng; p++) pl[p * P1_N + i] = f2h(ll_curve(yl, a.gam[p], a));
       pl[ng * P1_N + i] = f2h(yl);
     }
     __syncthreads();
@@ -260,10 +405,10 @@ __global__ void __launch_bounds__(256) kb_pyr1(bt_launch L)
     for(int p = 0; p <= ng; p++)
     {
       const __half *q = pl + p * P1_N + c;
-      const float a = h2f(q[0]) + 2.0f * h2f(q[1]) + h2f(q[2]);
-      const float b = h2f(q[P1_W]) + 2.0f * h2f(q[P1_W + 1]) + h2f(q[P1_W + 2]);
-      const float d = h2f(q[2 * P1_W]) + 2.0f * h2f(q[2 * P1_W + 1]) + h2f(q[2 * P1_W + 2]);
-      st_h1(o, x, y, (a + 2.0f * b + d) * (1.0f / 16.0f), p);
+      const float s0 = h2f(q[0]) + 2.0f * h2f(q[1]) + h2f(q[2]);
+      const float s1 = h2f(q[P1_W]) + 2.0f * h2f(q[P1_W + 1]) + h2f(q[P1_W + 2]);
+      const float s2 = h2f(q[2 * P1_W]) + 2.0f * h2f(q[2 * P1_W + 1]) + h2f(q[2 * P1_W + 2]);
+      st_h1(o, x, y, (s0 + 2.0f * s1 + s2) * (1.0f / 16.0f), p);
     }
   });
 }
@@ -271,11 +416,14 @@ __global__ void __launch_bounds__(256) kb_pyr1(bt_launch L)
 // reduce level l from l-1, four outputs per thread (256 x 4 per tile)
 __global__ void __launch_bounds__(256) kb_reduce(bt_launch L)
 {
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const dimg &o = im.lp[j.level];
-    const int x0 = ((lt % j.tiles_x) * BX + threadIdx.x) * 4, y = (lt / j.tiles_x) * BY + threadIdx.y;
-    if(x0 < o.wd && y < o.ht) ll_reduce_px4(im.lp[j.level - 1], o, im.ll.num_gamma + 1, x0, y, true);
+  dimg in, o;
+  int planes = 1;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    in = rfl(im.lp[jb.level - 1]); o = rfl(im.lp[jb.level]); planes = rfl(im.ll.num_gamma + 1);
+  }, [&](const bt_job &jb, int lt) {
+    const int x0 = ((lt % jb.tiles_x) * BX + threadIdx.x) * 4, y = (lt / jb.tiles_x) * BY + threadIdx.y;
+    if(x0 < o.wd && y < o.ht) ll_reduce_px4(in, o, planes, x0, y, true);
   });
 }
 
@@ -284,12 +432,18 @@ __global__ void __launch_bounds__(256) kb_asm(bt_launch L)
 {
   extern __shared__ float tile[];
   __shared__ int lims[2];
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const int l = j.level, first = l == im.nl - 2;
-    const dimg &fine = im.lp[l];
-    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * ASM_R);
-    const int x = X0 + threadIdx.x, ng = im.ll.num_gamma;
+  __shared__ ll_args a;
+  dimg fine, l1, coarse, o;
+  int first = 0;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    const int l = jb.level;
+    lds_copy(a, im.ll);
+    first = rfl((int)(l == im.nl - 2));
+    fine = rfl(im.lp[l]); l1 = rfl(im.lp[l + 1]); coarse = rfl(first ? im.lp[l + 1] : im.la[l + 1]); o = rfl(im.la[l]);
+  }, [&](const bt_job &jb, int lt) {
+    const int X0 = (lt % jb.tiles_x) * BX, Y0 = (lt / jb.tiles_x) * (BY * ASM_R);
+    const int x = X0 + threadIdx.x, ng = a.num_gamma;
     float v[ASM_R], res[ASM_R];
 #pragma unroll
     for(int k = 0; k < ASM_R; k++)
@@ -297,14 +451,13 @@ __global__ void __launch_bounds__(256) kb_asm(bt_launch L)
       const int y = Y0 + threadIdx.y + BY * k;
       v[k] = (x < fine.wd && y < fine.ht) ? ld_h1(fine, x, y, ng) : 0.0f;
     }
-    ll_assemble_tile<ASM_R>(first ? im.lp[l + 1] : im.la[l + 1], im.lp[l + 1], fine.wd, fine.ht, im.ll, first,
-                            X0, Y0, tile, lims, v,
+    ll_assemble_tile<ASM_R>(coarse, l1, fine.wd, fine.ht, a, first, X0, Y0, tile, lims, v,
                             [&](int k, int p) { return ld_h1(fine, x, Y0 + threadIdx.y + BY * k, p); }, res);
 #pragma unroll
     for(int k = 0; k < ASM_R; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      if(x < fine.wd && y < fine.ht) st_h1(im.la[l], x, y, res[k]);
+      if(x < fine.wd && y < fine.ht) st_h1(o, x, y, res[k]);
     }
   });
 }
@@ -314,11 +467,16 @@ __global__ void __launch_bounds__(256) kb_final(bt_launch L)
 {
   extern __shared__ float tile[];
   __shared__ int lims[2];
-  bt_tiles(L, [&](const bt_job &j, int lt) {
-    const bt_img &im = L.img[j.img];
-    const dimg &f = im.film;
-    const int first = im.nl == 2;
-    const int X0 = (lt % j.tiles_x) * BX, Y0 = (lt / j.tiles_x) * (BY * ASM_R);
+  __shared__ ll_args a;
+  dimg f, l1, coarse, o;
+  int first = 0;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.ll);
+    first = rfl((int)(im.nl == 2));
+    f = rfl(im.film); l1 = rfl(im.lp[1]); coarse = rfl(first ? im.lp[1] : im.la[1]); o = rfl(im.out);
+  }, [&](const bt_job &jb, int lt) {
+    const int X0 = (lt % jb.tiles_x) * BX, Y0 = (lt / jb.tiles_x) * (BY * ASM_R);
     const int x = X0 + threadIdx.x;
     float4 t[ASM_R];
     float yl[ASM_R], v[ASM_R], res[ASM_R];
@@ -330,13 +488,13 @@ __global__ void __launch_bounds__(256) kb_final(bt_launch L)
       yl[k] = ll_lum(t[k]);
       v[k] = r16(yl[k]);
     }
-    ll_assemble_tile<ASM_R>(first ? im.lp[1] : im.la[1], im.lp[1], f.wd, f.ht, im.ll, first, X0, Y0, tile, lims, v,
-                            [&](int k, int p) { return r16(ll_curve(yl[k], im.ll.gam[p], im.ll)); }, res);
+    ll_assemble_tile<ASM_R>(coarse, l1, f.wd, f.ht, a, first, X0, Y0, tile, lims, v,
+                            [&](int k, int p) { return r16(ll_curve(yl[k], a.gam[p], a)); }, res);
 #pragma unroll
     for(int k = 0; k < ASM_R; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      if(x < f.wd && y < f.ht) st_f4(im.out, x, y, ll_colour_px(r16(res[k]), t[k]));
+      if(x < f.wd && y < f.ht) st_f4(o, x, y, ll_colour_px(r16(res[k]), t[k]));
     }
   });
 }
@@ -420,11 +578,15 @@ dimg mk(void *p, int wd, int ht, int pitch, int chan, long pstride)
 
 struct vkdt_batch_t
 {
-  int n = 0, ncu = 256;
-  bt_img *d_img = nullptr;
+  int n = 0, ncu = 256, max_nl = 0, max_ng = 0;
+  std::vector<dims_t> dm;
+  std::vector<bt_img> img;         // host copies of the device tables
+  std::vector<bt_job> job;
+  bt_img *d_img = nullptr;         // device tables (in the caller's scratch)
   bt_job *d_job = nullptr;
   int *d_ctr = nullptr;
   int nctr = 0;
+  int dirty = 1;                   // tables to upload before the next run
   std::vector<launch_t> seq;
   int32_t tune[VKDT_BATCH_STAGES];
   asm_mats mats;
@@ -434,6 +596,15 @@ struct vkdt_batch_t
   hipEvent_t ev[VKDT_BATCH_STAGES + 1] = {};
   float ms[VKDT_BATCH_STAGES] = {};
 };
+
+// the device tables at the end of the scratch: image records, job lists
+// (at most one job per image and launch) and the tile counters
+#define BT_MAX_LAUNCH (9 + 2 * BT_LEVELS)
+static size_t table_bytes(int n)
+{
+  return align256(sizeof(bt_img) * n) + align256(sizeof(bt_job) * (size_t)BT_MAX_LAUNCH * n)
+       + align256(sizeof(int) * NUM_XCD * BT_MAX_LAUNCH);
+}
 
 int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t *bytes)
 {
@@ -445,7 +616,7 @@ int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t 
     if(int e = image_dims(img[i], m, i)) return e;
     s += image_scratch(m);
   }
-  *bytes = s;
+  *bytes = s + table_bytes(n);
   return VKDT_HIP_OK;
 }
 
@@ -513,14 +684,23 @@ static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m
   return VKDT_HIP_OK;
 }
 
-static int tiles_of(int kind, const dims_t &m, int level, int &tx)
+static int is_pixel_stage(int kind) { return kind == K_HALF || kind == K_DNASM || kind == K_DOUB || kind == K_GAUSS; }
+
+// launch shape of a stage group: tune = persist | rows << 8 (rows: per-pixel
+// stages' rows per thread, 1..8, default 4)
+static int tune_persist(int32_t t) { return t & 0xff; }
+static int tune_rows(int32_t t) { const int r = (t >> 8) & 0xff; return r < 1 ? 4 : (r > 8 ? 8 : r); }
+
+static int tiles_of(int kind, const dims_t &m, int level, int rows, int &tx)
 {
   int w, h, tw = BX, th = BY;
   switch(kind)
   {
-    case K_HALF: case K_DOWNCOV: case K_DOWN: case K_DNASM: w = m.qw; h = m.qh; break;
-    case K_DOUB: w = m.W; h = m.H; break;
-    case K_GAUSS: w = m.cw; h = m.ch; break;
+    case K_HALF: case K_DNASM: w = m.qw; h = m.qh; th = BY * rows; break;
+    case K_DOWNCOV: w = m.qw; h = m.qh; th = BY * COV_R; break;
+    case K_DOWN: w = m.qw; h = m.qh; th = BY * DOWN_R; break;
+    case K_DOUB: w = m.W; h = m.H; th = BY * rows; break;
+    case K_GAUSS: w = m.cw; h = m.ch; th = BY * rows; break;
     case K_DEVELOP: w = m.W; h = m.H; th = BY * DEV_R; break;
     case K_PYR1: w = m.lw[1]; h = m.lh[1]; break;
     case K_REDUCE: w = m.lw[level]; h = m.lh[level]; tw = 4 * BX; break;
@@ -529,6 +709,44 @@ static int tiles_of(int kind, const dims_t &m, int level, int &tx)
   }
   tx = (w + tw - 1) / tw;
   return tx * ((h + th - 1) / th);
+}
+
+// launch sequence and job tables for the current tune
+static void build_launches(vkdt_batch_t *b)
+{
+  b->seq.clear();
+  b->job.clear();
+  b->nctr = 0;
+  auto add = [&](int kind, int level, size_t lds) {
+    const int rows = is_pixel_stage(kind) ? tune_rows(b->tune[kind_group[kind]]) : 1;
+    launch_t L = { kind, level, (int)b->job.size(), 0, 0, lds, b->nctr };
+    for(int i = 0; i < b->n; i++)
+    {
+      const dims_t &m = b->dm[i];
+      if(kind == K_REDUCE && level >= m.nl) continue;
+      if(kind == K_ASM && level > m.nl - 2) continue;
+      int tx;
+      const int t = tiles_of(kind, m, level, rows, tx);
+      if(t <= 0) continue;
+      b->job.push_back(bt_job{ i, tx, L.total, level });
+      L.total += t;
+      L.njob++;
+    }
+    if(L.njob) { b->seq.push_back(L); b->nctr += NUM_XCD; }
+  };
+  const size_t asm_lds = sizeof(float) * (b->max_ng + 1) * AS_TN(ASM_R);
+  add(K_HALF, 0, 0);
+  add(K_DOWNCOV, 0, 0);
+  for(int l = 1; l < 4; l++) add(K_DOWN, l, 0);
+  add(K_DNASM, 0, 0);
+  add(K_DOUB, 0, 0);
+  add(K_GAUSS, 0, 0);
+  add(K_DEVELOP, 0, 0);
+  add(K_PYR1, 1, sizeof(__half) * (b->max_ng + 1) * P1_N);
+  for(int l = 2; l < b->max_nl; l++) add(K_REDUCE, l, 0);
+  for(int l = b->max_nl - 2; l >= 1; l--) add(K_ASM, l, asm_lds);
+  add(K_FINAL, 0, asm_lds);
+  b->dirty = 1;
 }
 
 int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, int n, void *scratch,
@@ -547,6 +765,8 @@ int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, i
     max_nl = std::max(max_nl, dm[i].nl);
     max_ng = std::max(max_ng, dm[i].ng);
   }
+  const size_t tables = need;
+  need += table_bytes(n);
   if(!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: scratch %zu bytes at %p, need %zu (256-byte aligned)",
                           scratch_bytes, scratch, need);
@@ -557,57 +777,36 @@ int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, i
 
   vkdt_batch_t *b = new vkdt_batch_t;
   b->n = n;
-  for(int s = 0; s < VKDT_BATCH_STAGES; s++) b->tune[s] = tune ? tune[s] : 0;
+  b->dm = dm;
+  b->img = h;
+  b->max_nl = max_nl;
+  b->max_ng = max_ng;
+  for(int s = 0; s < VKDT_BATCH_STAGES; s++) b->tune[s] = tune ? tune[s] : (4 << 8);
   asm_mats_init(b->mats);
   int dev = 0;
   if(hipGetDevice(&dev) != hipSuccess) dev = 0;
   if(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->ncu <= 0)
     b->ncu = 256;
-
-  // launch sequence and job tables
-  std::vector<bt_job> jobs;
-  auto add = [&](int kind, int level, size_t lds) {
-    launch_t L = { kind, level, (int)jobs.size(), 0, 0, lds, b->nctr };
-    for(int i = 0; i < n; i++)
-    {
-      if(kind == K_REDUCE && level >= dm[i].nl) continue;
-      if(kind == K_ASM && level > dm[i].nl - 2) continue;
-      int tx;
-      const int t = tiles_of(kind, dm[i], level, tx);
-      if(t <= 0) continue;
-      jobs.push_back(bt_job{ i, tx, L.total, level });
-      L.total += t;
-      L.njob++;
-    }
-    if(L.njob) { b->seq.push_back(L); b->nctr += NUM_XCD; }
-  };
-  const size_t asm_lds = sizeof(float) * (max_ng + 1) * AS_TN(ASM_R);
-  add(K_HALF, 0, 0);
-  add(K_DOWNCOV, 0, 0);
-  for(int l = 1; l < 4; l++) add(K_DOWN, l, 0);
-  add(K_DNASM, 0, 0);
-  add(K_DOUB, 0, 0);
-  add(K_GAUSS, 0, 0);
-  add(K_DEVELOP, 0, 0);
-  add(K_PYR1, 1, sizeof(__half) * (max_ng + 1) * P1_N);
-  for(int l = 2; l < max_nl; l++) add(K_REDUCE, l, 0);
-  for(int l = max_nl - 2; l >= 1; l--) add(K_ASM, l, asm_lds);
-  add(K_FINAL, 0, asm_lds);
-
-  hipError_t e = hipMalloc((void **)&b->d_img, sizeof(bt_img) * n);
-  if(e == hipSuccess) e = hipMalloc((void **)&b->d_job, sizeof(bt_job) * std::max<size_t>(1, jobs.size()));
-  if(e == hipSuccess) e = hipMalloc((void **)&b->d_ctr, sizeof(int) * std::max(1, b->nctr));
-  if(e == hipSuccess) e = hipMemcpy(b->d_img, h.data(), sizeof(bt_img) * n, hipMemcpyHostToDevice);
-  if(e == hipSuccess && !jobs.empty())
-    e = hipMemcpy(b->d_job, jobs.data(), sizeof(bt_job) * jobs.size(), hipMemcpyHostToDevice);
-  for(int s = 0; e == hipSuccess && s <= VKDT_BATCH_STAGES; s++) e = hipEventCreate(&b->ev[s]);
-  if(e != hipSuccess)
-  {
-    const int rc = hip_fail(e, "batch create");
-    vkdt_hip_batch_destroy(b);
-    return rc;
-  }
+  char *tp = (char *)scratch + tables;
+  b->d_img = (bt_img *)tp;
+  tp += align256(sizeof(bt_img) * n);
+  b->d_job = (bt_job *)tp;
+  tp += align256(sizeof(bt_job) * (size_t)BT_MAX_LAUNCH * n);
+  b->d_ctr = (int *)tp;
+  build_launches(b);
   *out = b;
+  return VKDT_HIP_OK;
+}
+
+// the tables go up in stream order before the first launch that reads them
+static int upload_tables(vkdt_batch_t *b, hipStream_t s)
+{
+  if(!b->dirty) return VKDT_HIP_OK;
+  if(int e = hip_fail(hipMemcpyAsync(b->d_img, b->img.data(), sizeof(bt_img) * b->n, hipMemcpyHostToDevice, s),
+                      "batch tables")) return e;
+  if(int e = hip_fail(hipMemcpyAsync(b->d_job, b->job.data(), sizeof(bt_job) * b->job.size(), hipMemcpyHostToDevice, s),
+                      "batch tables")) return e;
+  b->dirty = 0;
   return VKDT_HIP_OK;
 }
 
@@ -624,8 +823,9 @@ static int enqueue(vkdt_batch_t *b, hipStream_t s, int timed)
         if(int e = hip_fail(hipEventRecord(b->ev[k], s), "batch timing")) return e;
       group = g;
     }
-    const int persist = b->tune[g];
-    bt_launch a = { b->d_img, b->d_job + L.job0, L.njob, L.total, b->d_ctr + L.ctr0, persist };
+    const int persist = tune_persist(b->tune[g]);
+    const int rows = is_pixel_stage(L.kind) ? tune_rows(b->tune[g]) : 1;
+    bt_launch a = { b->d_img, b->d_job + L.job0, L.njob, L.total, b->d_ctr + L.ctr0, persist, rows };
     const dim3 grid(persist ? b->ncu * persist : L.total), blk(BX, BY);
     switch(L.kind)
     {
@@ -661,6 +861,7 @@ int vkdt_hip_batch_run(vkdt_batch_t *b, void *stream, int graph)
 {
   if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
   hipStream_t s = (hipStream_t)stream;
+  if(int e = upload_tables(b, s)) return e;
   if(!graph || !s || b->timing)
   { // eager (timing records events between the stage groups)
     if(int e = enqueue(b, s, b->timing)) return e;
@@ -693,6 +894,9 @@ int vkdt_hip_batch_run(vkdt_batch_t *b, void *stream, int graph)
 int vkdt_hip_batch_set_timing(vkdt_batch_t *b, int on)
 {
   if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
+  for(int k = 0; on && k <= VKDT_BATCH_STAGES; k++)
+    if(!b->ev[k])
+      if(int e = hip_fail(hipEventCreate(&b->ev[k]), "batch timing")) return e;
   b->timing = on != 0;
   return VKDT_HIP_OK;
 }
@@ -712,31 +916,43 @@ const char *vkdt_hip_batch_stage_name(int stage)
 int vkdt_hip_batch_autotune(vkdt_batch_t *b, void *stream, int32_t *tune)
 {
   if(!b) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch: null");
-  // launch shape per stage group: 0 = one tile per work group, k = k
-  // persistent work groups per CU pulling from the tile queues
-  static const int cand[] = { 0, 1, 2, 4, 8 };
-  const int nc = sizeof(cand) / sizeof(cand[0]);
+  // launch shape per stage group: persistence (0 = one tile per work
+  // group, k = k persistent work groups per CU pulling from the tile
+  // queues) and, for the per-pixel stages, rows per thread
+  static const int pers[] = { 0, 1, 2, 4, 8 };
+  static const int rows[] = { 1, 2, 4, 8 };
+  const int nc = 12;
   float best[VKDT_BATCH_STAGES];
   int32_t pick[VKDT_BATCH_STAGES];
-  for(int k = 0; k < VKDT_BATCH_STAGES; k++) { best[k] = 1e30f; pick[k] = 0; }
+  for(int k = 0; k < VKDT_BATCH_STAGES; k++) { best[k] = 1e30f; pick[k] = b->tune[k]; }
   const int was = b->timing;
-  b->timing = 1;
-  for(int c = 0; c < nc; c++)
+  if(int e = vkdt_hip_batch_set_timing(b, 1)) return e;
+  drop_graph(b);
+  int rc = VKDT_HIP_OK;
+  for(int c = 0; c < nc && !rc; c++)
   {
-    for(int k = 0; k < VKDT_BATCH_STAGES; k++) b->tune[k] = cand[c];
-    for(int rep = 0; rep < 3; rep++)
+    int32_t cand[VKDT_BATCH_STAGES];
+    for(int k = 0; k < VKDT_BATCH_STAGES; k++)
     {
-      if(int e = vkdt_hip_batch_run(b, stream, 0)) { b->timing = was; return e; }
-      if(rep == 0) continue;   // warm
+      const int pix = k == kind_group[K_HALF] || k == kind_group[K_DNASM] || k == kind_group[K_DOUB] ||
+                      k == kind_group[K_GAUSS];
+      cand[k] = pix ? (pers[(c % 3) * 2] | rows[c / 3] << 8) : (pers[c % 5] | 4 << 8);
+      b->tune[k] = cand[k];
+    }
+    build_launches(b);
+    for(int rep = 0; rep < 3 && !rc; rep++)
+    {
+      rc = vkdt_hip_batch_run(b, stream, 0);
+      if(rc || rep == 0) continue;   // first run warms
       for(int k = 0; k < VKDT_BATCH_STAGES; k++)
-        if(b->ms[k] < best[k]) { best[k] = b->ms[k]; pick[k] = cand[c]; }
+        if(b->ms[k] < best[k]) { best[k] = b->ms[k]; pick[k] = cand[k]; }
     }
   }
   b->timing = was;
   for(int k = 0; k < VKDT_BATCH_STAGES; k++) b->tune[k] = pick[k];
+  build_launches(b);
   if(tune) for(int k = 0; k < VKDT_BATCH_STAGES; k++) tune[k] = pick[k];
-  drop_graph(b);
-  return VKDT_HIP_OK;
+  return rc;
 }
 
 void vkdt_hip_batch_destroy(vkdt_batch_t *b)
@@ -744,8 +960,5 @@ void vkdt_hip_batch_destroy(vkdt_batch_t *b)
   if(!b) return;
   drop_graph(b);
   for(int s = 0; s <= VKDT_BATCH_STAGES; s++) if(b->ev[s]) (void)hipEventDestroy(b->ev[s]);
-  if(b->d_img) (void)hipFree(b->d_img);
-  if(b->d_job) (void)hipFree(b->d_job);
-  if(b->d_ctr) (void)hipFree(b->d_ctr);
   delete b;
 }

@@ -53,18 +53,21 @@ __global__ void __launch_bounds__(256) k_dn_half(dimg in, dimg out, dn_args a)
   st_h4(out, x, y, dn_half_px(in, a, x, y));
 }
 
+#define COV_R 2    // downcov: 64 x 8 texels per work group
+#define DOWN_R 4   // down: 64 x 16 texels per work group
+
 __global__ void __launch_bounds__(256) k_dn_downcov(dimg in, dimg out, dimg cov, dn_args a)
 {
-  __shared__ dn_cov_smem s;
+  __shared__ dn_cov_smem<COV_R> s;
   const int2 t = xcd_block();
-  dn_downcov_tile(in, out, cov, a, t.x * BX, t.y * BY, s);
+  dn_downcov_tile<COV_R>(in, out, cov, a, t.x * BX, t.y * BY * COV_R, s);
 }
 
 __global__ void __launch_bounds__(256) k_dn_down(dimg in, dimg out, dn_args a)
 {
-  __shared__ float4 tile[TN];
+  __shared__ float4 tile[DN_TN(DOWN_R)];
   const int2 t = xcd_block();
-  dn_down_tile(in, out, a, t.x * BX, t.y * BY, tile);
+  dn_down_tile<DOWN_R>(in, out, a, t.x * BX, t.y * BY * DOWN_R, tile);
 }
 
 __global__ void __launch_bounds__(256) k_dn_assemble(dimg s0, dimg s1, dimg s2, dimg s3, dimg s4, dimg out,
@@ -144,7 +147,7 @@ int launch_denoise_downcov(const node_call &c)
       return vkdt_set_error(VKDT_HIP_INCOMPLETE, "denoise/downcov: cov buffer smaller than output + crop");
     cov = to_dimg(c.conn[2]);
   }
-  hipLaunchKernelGGL(k_dn_downcov, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+  hipLaunchKernelGGL(k_dn_downcov, grid2d(c.wd, (c.ht + COV_R - 1) / COV_R, 1), dim3(BX, BY), 0, c.stream,
                      to_dimg(c.conn[0]), to_dimg(c.conn[1]), cov, a);
   return launch_post("denoise/downcov");
 }
@@ -156,7 +159,7 @@ int launch_denoise_down(const node_call &c)
   if(a.level < 0) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "denoise/down: negative level (response branch) not supported");
   if(int e = check_conn(c, 0, VKDT_FMT_F16, 4, 1, "denoise/down input")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "denoise/down output")) return e;
-  hipLaunchKernelGGL(k_dn_down, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+  hipLaunchKernelGGL(k_dn_down, grid2d(c.wd, (c.ht + DOWN_R - 1) / DOWN_R, 1), dim3(BX, BY), 0, c.stream,
                      to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
   return launch_post("denoise/down");
 }

@@ -5,7 +5,8 @@
 //
 // Per-tile operations are work-group level (they stage LDS and call
 // __syncthreads): every thread of the group must call them with the same
-// tile origin.  A tile is BX x BY = 64 x 4 output texels.
+// tile origin.  A work group is BX x BY = 64 x 4 threads; a tile of R rows
+// per thread covers 64 x 4R texels.
 #pragma once
 #include "common.h"
 
@@ -115,16 +116,18 @@ DEV float4 dn_half_px(const dimg &in, const dn_args &a, int x, int y)
   return rgba;
 }
 
-// LDS tile of an rgba f16 image with a 2 texel mirrored halo
+// LDS tile of an rgba f16 image: a work-group tile of 64 x 4R texels
+// (thread (tx,ty) owns rows ty + 4k, k < R) plus a 2 texel mirrored halo
 #define TH 2
 #define TW (BX + 2 * TH)
-#define TT (BY + 2 * TH)
-#define TN (TW * TT)
+#define DN_TT(R) (BY * (R) + 2 * TH)
+#define DN_TN(R) (TW * DN_TT(R))
 
+template<int R>
 DEV void load_tile4(float4 *tile, const dimg &in, int ox, int oy)
 {
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for(int i = tid; i < TN; i += BX * BY)
+  for(int i = tid; i < DN_TN(R); i += BX * BY)
   {
     const int ty = i / TW, tx = i - ty * TW;
     tile[i] = ld_h4(in, mirror(ox + tx, in.wd), mirror(oy + ty, in.ht));
@@ -136,15 +139,17 @@ DEV void load_tile4(float4 *tile, const dimg &in, int ox, int oy)
 // with the operation the statistics use.  (i-2)/l equals (i-2)*(1/l)
 // exactly for the numerators 0, +-1, +-2 (a sign change or a power of two
 // times a correctly rounded reciprocal), so the loops multiply.
-struct dn_cov_smem { float4 tile[TN]; float lum[TN], rlum[TN], rlum2[TN], lum25[TN]; };
+template<int R>
+struct dn_cov_smem { float4 tile[DN_TN(R)]; float lum[DN_TN(R)], rlum[DN_TN(R)], rlum2[DN_TN(R)], lum25[DN_TN(R)]; };
 
+template<int R>
 DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const dn_args &a, int X0, int Y0,
-                         dn_cov_smem &s)
+                         dn_cov_smem<R> &s)
 {
-  load_tile4(s.tile, in, X0 + a.crop[0] - TH, Y0 + a.crop[1] - TH);
+  load_tile4<R>(s.tile, in, X0 + a.crop[0] - TH, Y0 + a.crop[1] - TH);
   __syncthreads();
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for(int i = tid; i < TN; i += BX * BY)
+  for(int i = tid; i < DN_TN(R); i += BX * BY)
   {
     const float4 v = s.tile[i];
     const float l = lum2020(v.x, v.y, v.z);
@@ -154,74 +159,79 @@ DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const
     s.lum25[i] = l / 25.0f;
   }
   __syncthreads();
-  const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
-  if(x >= out.wd || y >= out.ht) return;
-  const int c0 = threadIdx.y * TW + threadIdx.x;   // tile index of offset (-2,-2)
-  float mw0 = 0, mw1 = 0, mb0 = 0, mb1 = 0, smw = 0, smb = 0;
+  const int x = X0 + threadIdx.x;
+#pragma unroll 1
+  for(int k = 0; k < R; k++)
+  {
+    const int y = Y0 + threadIdx.y + BY * k;
+    if(x >= out.wd || y >= out.ht) continue;
+    const int c0 = (threadIdx.y + BY * k) * TW + threadIdx.x;   // tile index of offset (-2,-2)
+    float mw0 = 0, mw1 = 0, mb0 = 0, mb1 = 0, smw = 0, smb = 0;
 #pragma unroll
-  for(int j = 0; j < 5; j++)
+    for(int j = 0; j < 5; j++)
 #pragma unroll
-    for(int i = 0; i < 5; i++)
-    {
-      const float px = s.lum[c0 + j * TW + i], rp = s.rlum[c0 + j * TW + i];
-      mw0 += (float)(i - 2) * px; mw1 += (float)(j - 2) * px;
-      smw += px;
-      mb0 += (float)(i - 2) * rp; mb1 += (float)(j - 2) * rp;
-      smb += rp;
-    }
-  mw0 /= smw; mw1 /= smw;
-  mb0 /= smb; mb1 /= smb;
-  float Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0, mean_b = 0;
+      for(int i = 0; i < 5; i++)
+      {
+        const float px = s.lum[c0 + j * TW + i], rp = s.rlum[c0 + j * TW + i];
+        mw0 += (float)(i - 2) * px; mw1 += (float)(j - 2) * px;
+        smw += px;
+        mb0 += (float)(i - 2) * rp; mb1 += (float)(j - 2) * rp;
+        smb += rp;
+      }
+    mw0 /= smw; mw1 /= smw;
+    mb0 /= smb; mb1 /= smb;
+    float Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0, mean_b = 0;
 #pragma unroll
-  for(int j = 0; j < 5; j++)
+    for(int j = 0; j < 5; j++)
 #pragma unroll
-    for(int i = 0; i < 5; i++)
-    {
-      const int t = c0 + j * TW + i;
-      const float px = s.lum[t];
-      mean_b += s.lum25[t];
-      float p2 = px * px;
-      float dx = (float)(i - 2) - mw0, dy = (float)(j - 2) - mw1;
-      Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
-      sw += p2;
-      dx = (float)(i - 2) - mb0; dy = (float)(j - 2) - mb1;
-      p2 = s.rlum2[t];
-      Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
-      sb += p2;
-    }
+      for(int i = 0; i < 5; i++)
+      {
+        const int t = c0 + j * TW + i;
+        const float px = s.lum[t];
+        mean_b += s.lum25[t];
+        float p2 = px * px;
+        float dx = (float)(i - 2) - mw0, dy = (float)(j - 2) - mw1;
+        Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
+        sw += p2;
+        dx = (float)(i - 2) - mb0; dy = (float)(j - 2) - mb1;
+        p2 = s.rlum2[t];
+        Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
+        sb += p2;
+      }
 #pragma unroll
-  for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
-  float S[4];
-  const bool usew = det2(Sw) < det2(Sb);
+    for(int q = 0; q < 4; q++) { Sw[q] /= sw; Sb[q] /= sb; }
+    float S[4];
+    const bool usew = det2(Sw) < det2(Sb);
 #pragma unroll
-  for(int k = 0; k < 4; k++) S[k] = usew ? Sw[k] : Sb[k];
-  float ev[2], e0[2], e1[2];
-  evd2x2(ev, e0, e1, S);
-  ev[1] *= 0.05f;
-  ev[0] = clampf(ev[0], 0.01f, 25.0f);
-  ev[1] = clampf(ev[1], 0.01f, 25.0f);
-  if(cov.p) st_h4(cov, x + a.crop[0], y + a.crop[1], make_float4(ev[0], ev[1], e0[0], e0[1]));
-  const float ri0 = -0.5f * frcp(ev[0]), ri1 = -0.5f * frcp(ev[1]);
-  float r0 = 0, r1 = 0, r2 = 0, wt = 0;
-  const float hot = 2.0f * mean_b;
+    for(int q = 0; q < 4; q++) S[q] = usew ? Sw[q] : Sb[q];
+    float ev[2], e0[2], e1[2];
+    evd2x2(ev, e0, e1, S);
+    ev[1] *= 0.05f;
+    ev[0] = clampf(ev[0], 0.01f, 25.0f);
+    ev[1] = clampf(ev[1], 0.01f, 25.0f);
+    if(cov.p) st_h4(cov, x + a.crop[0], y + a.crop[1], make_float4(ev[0], ev[1], e0[0], e0[1]));
+    const float ri0 = -0.5f * frcp(ev[0]), ri1 = -0.5f * frcp(ev[1]);
+    float r0 = 0, r1 = 0, r2 = 0, wt = 0;
+    const float hot = 2.0f * mean_b;
 #pragma unroll
-  for(int j = 0; j < 5; j++)
+    for(int j = 0; j < 5; j++)
 #pragma unroll
-    for(int i = 0; i < 5; i++)
-    {
-      const float4 v = s.tile[c0 + j * TW + i];
-      if(v.x > hot) continue;   // hot pixel
-      const float x0 = (float)(i - 2) * e0[0] + (float)(j - 2) * e0[1];
-      const float x1 = (float)(i - 2) * e1[0] + (float)(j - 2) * e1[1];
-      const float w = fmaxf(1e-9f, fexp(x0 * x0 * ri0 + x1 * x1 * ri1));
-      r0 += w * v.x; r1 += w * v.y; r2 += w * v.z;
-      wt += w;
-    }
-  const float d = frcp(fmaxf(wt, 1e-8f));
-  float edge = clampf(75.0f * fmaxf(0.0f, ev[1] - 0.09f), 0.0f, 1.0f);
-  edge = smoothstepf(0.4f, 0.75f, edge);
-  edge = clampf(0.02f + edge, 0.0f, 1.0f);
-  st_h4(out, sorted1(x, out.wd), sorted1(y, out.ht), make_float4(r0 * d, r1 * d, r2 * d, edge));
+      for(int i = 0; i < 5; i++)
+      {
+        const float4 v = s.tile[c0 + j * TW + i];
+        if(v.x > hot) continue;   // hot pixel
+        const float x0 = (float)(i - 2) * e0[0] + (float)(j - 2) * e0[1];
+        const float x1 = (float)(i - 2) * e1[0] + (float)(j - 2) * e1[1];
+        const float w = fmaxf(1e-9f, fexp(x0 * x0 * ri0 + x1 * x1 * ri1));
+        r0 += w * v.x; r1 += w * v.y; r2 += w * v.z;
+        wt += w;
+      }
+    const float d = frcp(fmaxf(wt, 1e-8f));
+    float edge = clampf(75.0f * fmaxf(0.0f, ev[1] - 0.09f), 0.0f, 1.0f);
+    edge = smoothstepf(0.4f, 0.75f, edge);
+    edge = clampf(0.02f + edge, 0.0f, 1.0f);
+    st_h4(out, sorted1(x, out.wd), sorted1(y, out.ht), make_float4(r0 * d, r1 * d, r2 * d, edge));
+  }
 }
 
 DEV float gamma08(float f) { return f < 0.0f ? f : fpow(f, 0.8f); }
@@ -239,42 +249,48 @@ DEV float4 tile_bilin(const float4 *tile, int b, float fa, float fc)
 }
 
 // down.comp:59-108 (level >= 0) for the tile at (X0,Y0)
+template<int R>
 DEV void dn_down_tile(const dimg &in, const dimg &out, const dn_args &a, int X0, int Y0, float4 *tile)
 {
-  load_tile4(tile, in, X0 + a.crop[0] - TH, Y0 + a.crop[1] - TH);
+  load_tile4<R>(tile, in, X0 + a.crop[0] - TH, Y0 + a.crop[1] - TH);
   __syncthreads();
-  const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
-  if(x >= out.wd || y >= out.ht) return;
-  const int c = (threadIdx.y + TH) * TW + threadIdx.x + TH;   // tile index of the centre
-  const float t = 0.2f;
-  const float4 cc = tile[c];
-  const float rgb[3] = { cc.x, cc.y, cc.z };
-  float sum[3] = { t * rgb[0], t * rgb[1], t * rgb[2] };
-  float wgt[3] = { t, t, t };
-  float sig[3];
-  noise_sigma(sig, a, a.black[1], a.inv_range[1], rgb[0]);
-  float wc[3], g0[3];
-#pragma unroll
-  for(int k = 0; k < 3; k++) { wc[k] = frcp(a.level_scale * sig[k]); g0[k] = gamma08(rgb[k]); }
-  // four flower taps at (+1.2,+.4) (-1.2,-.4) (+.4,-1.2) (-.4,+1.2)
-  const int   ox[4] = { 1, -2, 0, -1 }, oy[4] = { 0, -1, -2, 1 };
-  const float fa[4] = { 0.2f, 0.8f, 0.4f, 0.6f }, fc[4] = { 0.4f, 0.6f, 0.8f, 0.2f };
-#pragma unroll
-  for(int o = 0; o < 4; o++)
+  const int x = X0 + threadIdx.x;
+#pragma unroll 1
+  for(int k = 0; k < R; k++)
   {
-    const float4 cl = tile_bilin(tile, c + oy[o] * TW + ox[o], fa[o], fc[o]);
-    const float col[3] = { cl.x, cl.y, cl.z };
+    const int y = Y0 + threadIdx.y + BY * k;
+    if(x >= out.wd || y >= out.ht) continue;
+    const int c = (threadIdx.y + BY * k + TH) * TW + threadIdx.x + TH;   // tile index of the centre
+    const float t = 0.2f;
+    const float4 cc = tile[c];
+    const float rgb[3] = { cc.x, cc.y, cc.z };
+    float sum[3] = { t * rgb[0], t * rgb[1], t * rgb[2] };
+    float wgt[3] = { t, t, t };
+    float sig[3];
+    noise_sigma(sig, a, a.black[1], a.inv_range[1], rgb[0]);
+    float wc[3], g0[3];
 #pragma unroll
-    for(int k = 0; k < 3; k++)
+    for(int q = 0; q < 3; q++) { wc[q] = frcp(a.level_scale * sig[q]); g0[q] = gamma08(rgb[q]); }
+    // four flower taps at (+1.2,+.4) (-1.2,-.4) (+.4,-1.2) (-.4,+1.2)
+    const int   ox[4] = { 1, -2, 0, -1 }, oy[4] = { 0, -1, -2, 1 };
+    const float fa[4] = { 0.2f, 0.8f, 0.4f, 0.6f }, fc[4] = { 0.4f, 0.6f, 0.8f, 0.2f };
+#pragma unroll
+    for(int o = 0; o < 4; o++)
     {
-      const float e = clampf(1.0f - 0.5f * (wc[k] * fabsf(gamma08(col[k]) - g0[k])), 0.0f, 1.0f);
-      const float w = e * ((1.0f - t) * 0.25f);
-      sum[k] += w * col[k];
-      wgt[k] += w;
+      const float4 cl = tile_bilin(tile, c + oy[o] * TW + ox[o], fa[o], fc[o]);
+      const float col[3] = { cl.x, cl.y, cl.z };
+#pragma unroll
+      for(int q = 0; q < 3; q++)
+      {
+        const float e = clampf(1.0f - 0.5f * (wc[q] * fabsf(gamma08(col[q]) - g0[q])), 0.0f, 1.0f);
+        const float w = e * ((1.0f - t) * 0.25f);
+        sum[q] += w * col[q];
+        wgt[q] += w;
+      }
     }
+    st_h4(out, sorted1(x, out.wd), sorted1(y, out.ht),
+          make_float4(sum[0] * frcp(wgt[0]), sum[1] * frcp(wgt[1]), sum[2] * frcp(wgt[2]), 1.0f));
   }
-  st_h4(out, sorted1(x, out.wd), sorted1(y, out.ht),
-        make_float4(sum[0] * frcp(wgt[0]), sum[1] * frcp(wgt[1]), sum[2] * frcp(wgt[2]), 1.0f));
 }
 
 // assemble.comp:43-165
@@ -413,7 +429,7 @@ DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
 {
   constexpr int n0 = XT ? 0 : -1, n1 = 3, bs = XT ? 3 : 2;
   constexpr int NS = XT ? 5 : 8;
-  // pi/px equals pi*(1/px) exactly for the offsets -1..2 (see dn_cov_smem)
+  // pi/px equals pi*(1/px) exactly for the offsets -1..2 (as in dn_downcov_tile)
   float px[NS], rp[NS];
   int pi[NS], pj[NS];
   int n = 0;
@@ -487,8 +503,10 @@ DEV float gauss_w(float4 cov, float r0, float r1, int i, int j, float lo)
 
 // splat.comp:16-140: green at (x,y).  Only the green splats reach the
 // stored channel, so only green sites are weighted (same sum and order).
-template<int XT>
-DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
+//   mos(px, py): the mosaic at (px, py), 0 outside (texelFetch);
+//   W, H its extent
+template<int XT, typename MosAt>
+DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
 {
   const float4 cov = XT ? fetch_h4(ga, x / 3, y / 3) : fetch_h4(ga, (x + 1) / 2, (y + 1) / 2);
   const float r0 = -0.5f * frcp(clampf(cov.x, 0.01f, 25.0f)), r1 = -0.5f * frcp(clampf(cov.y, 0.01f, 25.0f));
@@ -504,9 +522,9 @@ DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
       int px = x + i, py = y + j;
       if(px < 0) px += 6;
       if(py < 0) py += 6;
-      if(px >= in.wd) px -= 6;
-      if(py >= in.ht) py -= 6;
-      const float col = fetch_h1(in, px, py);
+      if(px >= W) px -= 6;
+      if(py >= H) py -= 6;
+      const float col = mos(px, py);
       const float wt = (i == 0 && j == 0) ? 666.0f : gauss_w(cov, r0, r1, i, j, 1e-4f);
       g += col * wt;
       w += wt;
@@ -514,11 +532,18 @@ DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
   return g * frcp(fmaxf(1e-8f, w));
 }
 
+template<int XT>
+DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
+{
+  return dm_splat_at<XT>([&](int px, int py) { return fetch_h1(in, px, py); }, in.wd, in.ht, ga, x, y);
+}
+
 // fix.comp:21-135: rgb at (x,y) from the mosaic, the green plane (through
 // `green(px,py)`, mirrored coordinates) and the block covariance.  Green
 // sites copy the centre green, red and blue sites are ratio-weighted.
-template<int XT, typename GreenAt>
-DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
+//   mos(px, py): the mosaic at mirrored (px, py)
+template<int XT, typename MosAt, typename GreenAt>
+DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt green)
 {
   float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
   const float gc = green(x, y);
@@ -541,7 +566,7 @@ DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenA
       else k = (((px + py) & 1) == 1) ? 1 : ((px & 1) == 0 ? 0 : 2);
       if(k == 1) { any_green = true; continue; }
       const float gh = green(px, py);
-      const float col = centre_h1(in, px, py);
+      const float col = mos(px, py);
       const float wt = gauss_w(cov, r0, r1, i, j, 1e-3f);
       const float v = col * gcn * frcp(1e-4f + gh) * wt;
       if(k == 0) { sr += v; wr += wt; }
@@ -549,6 +574,12 @@ DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenA
     }
   const float g = any_green ? gc : 0.0f;
   return make_float4(sr * frcp(fmaxf(1e-8f, wr)), g, sb * frcp(fmaxf(1e-8f, wb)), 1.0f);
+}
+
+template<int XT, typename GreenAt>
+DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
+{
+  return dm_fix_at<XT>([&](int px, int py) { return centre_h1(in, px, py); }, cv, r, x, y, green);
 }
 
 // ==========================================================================
