@@ -119,6 +119,11 @@ int vkdt_hip_event_create(void **event);
 int vkdt_hip_event_destroy(void *event);
 int vkdt_hip_event_record(void *event, void *stream);
 int vkdt_hip_event_elapsed_ms(float *ms, void *ev_begin, void *ev_end);
+// Non-blocking completion test: 1 when the work recorded before `event`
+// has finished, 0 while it runs, < 0 on error.  No reference counterpart
+// (the reference waits on its timeline semaphore, graph.c:842-874); the
+// serving loop polls it to retire batches while it admits new requests.
+int vkdt_hip_event_query(void *event);
 
 // Human readable text of the last error on this thread.
 const char *vkdt_hip_last_error(void);
@@ -160,21 +165,25 @@ vkdt_develop_image_t;
 
 typedef struct vkdt_batch_t vkdt_batch_t;
 
-// Device scratch a batch of these images needs (intermediates only: the
-// caller owns the mosaics and the outputs).
+// Device scratch a batch of these images needs: their intermediates and the
+// batch's own tables (the caller owns the mosaics and the outputs).
 int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t *bytes);
 
 // Plan a batch: validate the images, lay out their intermediates in
 // `scratch` (at least vkdt_hip_batch_scratch_bytes, 256-byte aligned) and
 // build the per-stage tile lists.  `tune` selects the launch shape per
-// stage: NULL takes the built-in choice, else an array of
-// VKDT_BATCH_STAGES entries as written by vkdt_hip_batch_autotune.
+// stage group: NULL takes the built-in choice, else VKDT_BATCH_STAGES
+// entries as written by vkdt_hip_batch_autotune, each
+//   persistent work groups per CU (0: one tile per work group)
+//   | rows per thread of the per-pixel stages (1..8) << 8.
 #define VKDT_BATCH_STAGES 9
 int vkdt_hip_batch_create(vkdt_batch_t **batch, const vkdt_develop_image_t *img, int n,
                           void *scratch, size_t scratch_bytes, const int32_t *tune);
 
-// Enqueue the whole batch on `stream`.  The first run with `graph` != 0
-// captures the launch sequence into a hipGraph; later runs replay it.
+// Enqueue the whole batch on `stream`.  The first run (and the first after
+// a retune) also uploads the batch's tables into its scratch, in stream
+// order.  The first run with `graph` != 0 captures the launch sequence into
+// a hipGraph; later runs replay it.
 int vkdt_hip_batch_run(vkdt_batch_t *batch, void *stream, int graph);
 
 // Time each stage's launch shapes on this batch and keep the fastest
@@ -187,6 +196,8 @@ int vkdt_hip_batch_set_timing(vkdt_batch_t *batch, int on);
 int vkdt_hip_batch_stage_ms(vkdt_batch_t *batch, float *ms);
 const char *vkdt_hip_batch_stage_name(int stage);
 
+// Release the host side of a batch (after its work has completed; the
+// scratch stays the caller's).
 void vkdt_hip_batch_destroy(vkdt_batch_t *batch);
 
 #ifdef __cplusplus

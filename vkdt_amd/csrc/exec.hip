@@ -162,6 +162,13 @@ int vkdt_hip_event_create(void **event)
 int vkdt_hip_event_destroy(void *event) { return hip_fail(hipEventDestroy((hipEvent_t)event), "hipEventDestroy"); }
 int vkdt_hip_event_record(void *event, void *stream) { return hip_fail(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord"); }
 int vkdt_hip_event_elapsed_ms(float *ms, void *b, void *e) { return hip_fail(hipEventElapsedTime(ms, (hipEvent_t)b, (hipEvent_t)e), "hipEventElapsedTime"); }
+int vkdt_hip_event_query(void *event)
+{
+  const hipError_t e = hipEventQuery((hipEvent_t)event);
+  if(e == hipSuccess) return 1;
+  if(e == hipErrorNotReady) return 0;
+  return hip_fail(e, "hipEventQuery");
+}
 
 const char *vkdt_hip_last_error(void) { return g_err; }
 

@@ -64,6 +64,7 @@ def lib():
         L.vkdt_hip_event_destroy.argtypes = [vp]
         L.vkdt_hip_event_record.argtypes = [vp, vp]
         L.vkdt_hip_event_elapsed_ms.argtypes = [C.POINTER(C.c_float), vp, vp]
+        L.vkdt_hip_event_query.argtypes = [vp]
         L.vkdt_hip_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -153,6 +154,13 @@ class Event:
 
     def record(self, stream=None):
         check(lib().vkdt_hip_event_record(self.ptr, stream), "record")
+
+    def done(self):
+        """True once the work recorded before this event has finished"""
+        rc = lib().vkdt_hip_event_query(self.ptr)
+        if rc < 0:
+            check(rc, "event query")
+        return rc == 1
 
     def elapsed_ms(self, end):
         ms = C.c_float()
