@@ -340,6 +340,126 @@ def run_batch(args):
     }
 
 
+# serving: 8 raw-video streams (4096x2160 RGGB, 12-bit: black 256, white 4095,
+# 30 fps, scenes drifting 2 px per frame) and 24-61 MP 14-bit stills
+STREAMS, FRAME_W, FRAME_H, POOL_FRAMES = 8, 4096, 2160, 8
+STILLS = [(6000, 4000), (7008, 4672), (8256, 5504), (9504, 6336)]
+
+
+def serve_workload(seconds, rate=600.0, p_still=0.2, seed=31):
+    """Poisson arrivals at `rate`; each is a still with probability p_still,
+    else the next frame of a uniformly chosen stream"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out, nxt = 0.0, [], [0] * STREAMS
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        if rng.random() < p_still:
+            out.append((("still", int(rng.integers(len(STILLS)))), t))
+        else:
+            s = int(rng.integers(STREAMS))
+            out.append((("frame", s, nxt[s] % POOL_FRAMES), t))
+            nxt[s] += 1
+
+
+def run_serve(args, tune):
+    """the serving loop on the mixed request stream, beside a FIFO of the
+    plain per-image path over the same arrivals (measured per-image times)"""
+    import numpy as np
+    import torch
+    from vkdt_amd import batch, hip, pipe, serve, synth, synth_gpu
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()      # the batch run's tensors go back to the driver
+    tmp = tempfile.mkdtemp(prefix="vkdt_serve_")
+    rng = np.random.default_rng(17)
+    profiles, host, plain_ms = {}, {}, {}
+
+    def dng_for(W, H, black, white, seed):
+        cfa, _ = synth_gpu.mosaic(W, H, seed, "bayer", black=black, white=white)
+        path = os.path.join(tmp, f"p{W}x{H}.dng")
+        synth.write_dng(path, cfa.cpu().numpy().view(np.uint16), "bayer", black=black, white=white)
+        return path
+
+    def plan(path, a, black, white):
+        na, nb = synth.noise_params(a, 1e-6, black, white)
+        g = plain_graph(path, "bayer", na, nb, tmp)
+        g.plan()
+        d = batch.describe(g)
+        g.close()
+        return d, na, nb, a
+
+    def time_plain(path, na, nb):
+        g = plain_graph(path, "bayer", na, nb, tmp)
+        g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)
+        ts = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            g.run(pipe.RUN_RECORD)
+            hip.sync(g.stream())
+            ts.append((time.perf_counter() - t0) * 1e3)
+        g.close()
+        return sorted(ts)[1]
+
+    fdng = dng_for(FRAME_W, FRAME_H, 256, 4095, 700)
+    for s in range(STREAMS):
+        d, na, nb, a = plan(fdng, float(rng.uniform(1e-4, 1e-3)), 256, 4095)
+        profiles[("frame", s)] = serve.Profile(d, f"stream{s}")
+        sc = None
+        for f in range(POOL_FRAMES):
+            cfa, sc = synth_gpu.mosaic(d.cfa_wd, d.cfa_ht, 1000 + s, "bayer", a=a, black=256, white=4095,
+                                       origin=(2 * f, 0), canvas=(d.cfa_wd + 2 * POOL_FRAMES, d.cfa_ht), scale=sc)
+            host[("frame", s, f)] = cfa.cpu().pin_memory()
+        if s == 0:
+            plain_ms["frame"] = time_plain(fdng, na, nb)
+    for i, (W, H) in enumerate(STILLS):
+        path = dng_for(W, H, 512, 16383, 710 + i)
+        d, na, nb, _ = plan(path, 2e-4, 512, 16383)
+        profiles[("still", i)] = serve.Profile(d, f"still{W}x{H}")
+        cfa, _ = synth_gpu.mosaic(d.cfa_wd, d.cfa_ht, 2000 + i, "bayer")
+        host[("still", i)] = cfa.cpu().pin_memory()
+        plain_ms[("still", i)] = time_plain(path, na, nb)
+    torch.cuda.synchronize()
+    print(f"[bench] serving profiles ready; plain ms {plain_ms}", file=sys.stderr, flush=True)
+
+    work = serve_workload(args.serve_seconds)
+    reqs = []
+    for i, (key, t) in enumerate(work):
+        prof = profiles[key[:2]]
+        reqs.append(serve.Request(i, prof, host[key].data_ptr(), t, tag=key[0]))
+    srv = serve.Server(200 << 30, slots=3, max_batch=48, tune=tune)
+    try:
+        wall = srv.run(reqs)
+    finally:
+        srv.close()
+    steady = [r for r in reqs if r.arrival >= 0.5]
+    s = serve.summary(steady, wall - 0.5)
+    nst = sum(1 for r in reqs if r.tag == "still")
+    # the plain path as one FIFO server over the same arrivals
+    fin, lat = 0.0, []
+    for key, t in work:
+        ms = plain_ms["frame"] if key[0] == "frame" else plain_ms[key[:2]]
+        fin = max(fin, t) + ms * 1e-3
+        if t >= 0.5:
+            lat.append((fin - t) * 1e3)
+    px = sum(r.profile.pixels for r in reqs)
+    busy = sum(plain_ms["frame"] if k[0] == "frame" else plain_ms[k[:2]] for k, _ in work) * 1e-3
+    sizes = [b[0] for b in srv.batches]
+    return {
+        "workload": f"Poisson {len(work) / args.serve_seconds:.0f} req/s for {args.serve_seconds:g} s: "
+                    f"{STREAMS} streams of {FRAME_W}x{FRAME_H} RGGB 12-bit frames + 24-61 MP 14-bit stills",
+        "requests": len(reqs), "stills": nst, "frames": len(reqs) - nst,
+        "offered_mpix_per_s": round(px / args.serve_seconds / 1e6, 1),
+        "served": s, "batches": len(sizes), "mean_batch": round(float(np.mean(sizes)), 2),
+        "budget_gib": 200, "peak_gib": round(srv.arena.peak / 2**30, 2), "slots": 3,
+        "plain_fifo": {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                       "capacity_mpix_per_s": round(px / busy / 1e6, 1),
+                       "method": "FIFO over the same arrivals with the per-node graph's measured time per shape"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +470,8 @@ def main():
                          "plain: the per-image path on one still")
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="batch mode: also run the serving loop on this many seconds of arrivals (0: skip)")
     ap.add_argument("--breakdown", default="", help="plain mode: write per-kernel timings as JSON to this path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), the reference kernel "
@@ -373,6 +495,8 @@ def main():
         line = run_plain(args)
     else:
         line = run_batch(args)
+        if args.serve_seconds > 0:
+            line["serving"] = run_serve(args, [line["config"]["tune"][k] for k in line["config"]["tune"]])
     print(json.dumps(line), flush=True)
 
 
