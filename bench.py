@@ -244,7 +244,7 @@ def run_batch(args):
 
     # plain per-image path: each shape/CFA timed (median of 3 resident runs)
     plain_ms = {}
-    for (W, H, k) in combos:
+    for (W, H, k) in ([] if args.no_plain else combos):
         na, nb = synth.noise_params(5e-4, 1e-6)
         g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp)
         g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)
@@ -309,10 +309,11 @@ def run_batch(args):
     sb = [sum(x) for x in zip(*[stage_bytes(d) for d in descs])]
     dom = max(range(len(names)), key=lambda s: stage_ms[s])
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
-    plain_total = sum(plain_ms[(W, H, k)] for W, H, k, _ in work)
-    plain_value = P / (plain_total * 1e-3) / 1e6
+    plain_total = sum(plain_ms[(W, H, k)] for W, H, k, _ in work) if plain_ms else None
+    plain_value = P / (plain_total * 1e-3) / 1e6 if plain_ms else None
     roof = {"bound": "hbm", "kernel": "batch/" + names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("batch64", "batch/" + names[dom]),
             "bytes_per_launch": int(sb[dom]), "avg_launch_ms": round(stage_ms[dom], 4)}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     return {
@@ -326,11 +327,13 @@ def run_batch(args):
                    "graph": "i-raw>denoise(strength .5)>demosaic>colour>filmcurv>llap(clarity .2)",
                    "tune": dict(zip(names, tune))},
         "gpu_ms_per_step": round(gpu_ms, 3),
-        "plain": {"value": round(plain_value, 2), "unit": "Mpix/s", "ms_64": round(plain_total, 2),
-                  "method": "per-node graph, resident raw; each of the 8 shape/CFA pairs timed (median of 3) "
-                            "and summed over the 64 images"},
-        "speedup_vs_plain": round(value / plain_value, 3),
+        "plain": None if not plain_ms else {
+            "value": round(plain_value, 2), "unit": "Mpix/s", "ms_64": round(plain_total, 2),
+            "method": "per-node graph, resident raw; each shape/CFA pair timed (median of 3) "
+                      "and summed over the 64 images"},
+        "speedup_vs_plain": round(value / plain_value, 3) if plain_ms else None,
         "stages_ms": dict(zip(names, [round(m, 3) for m in stage_ms])),
+        "stages_compulsory_gb": dict(zip(names, [round(b / 1e9, 2) for b in sb])),
         "roofline": roof,
         "graph_roofline": {"compulsory_bytes": int(sum(sb)), "bytes_per_px": round(sum(sb) / P, 1),
                            "gpu_ms": round(sum(stage_ms), 3),
@@ -470,6 +473,7 @@ def main():
                          "plain: the per-image path on one still")
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="batch mode: skip timing the plain path beside it")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="batch mode: also run the serving loop on this many seconds of arrivals (0: skip)")
     ap.add_argument("--breakdown", default="", help="plain mode: write per-kernel timings as JSON to this path")
