@@ -325,7 +325,7 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
     for(int k = 0; k < DEV_R; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      const float4 f = dm_fix_at<XT>(mosl, cov, r, x, y, green);
+      const float4 f = dm_fix_r<XT>(mosl, cov, r, x, y, green);
       const float4 c = r16(co_px(r16(f), co));
       st_h4(film, x, y, fc_px(c, fc));
     }
@@ -348,6 +348,7 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
   }
 }
 
+template<int XT>
 __global__ void __launch_bounds__(256) kb_develop(bt_launch L)
 {
   __shared__ float gt[(BX + 6) * (BY * DEV_R + 6)];                 // green, halo r <= 3
@@ -355,16 +356,15 @@ __global__ void __launch_bounds__(256) kb_develop(bt_launch L)
   __shared__ co_args co;
   __shared__ fc_args fc;
   dimg mos, cov, film;
-  int xt = 0, r = 1;
+  int r = 1;
   bt_tiles(L, [&](const bt_job &jb) {
     const bt_img &im = L.img[jb.img];
     lds_copy(co, im.co);
     lds_copy(fc, im.fc);
-    mos = rfl(im.mos); cov = rfl(im.cov); film = rfl(im.film); xt = rfl(im.xt); r = rfl(im.dm_r);
+    mos = rfl(im.mos); cov = rfl(im.cov); film = rfl(im.film); r = rfl(im.dm_r);
   }, [&](const bt_job &jb, int lt) {
     const int X0 = (lt % jb.tiles_x) * BX, Y0 = (lt / jb.tiles_x) * (BY * DEV_R);
-    if(xt) develop_tile<1>(mos, cov, film, co, fc, r, X0, Y0, gt, mt);
-    else   develop_tile<0>(mos, cov, film, co, fc, r, X0, Y0, gt, mt);
+    develop_tile<XT>(mos, cov, film, co, fc, r, X0, Y0, gt, mt);
   });
 }
 
@@ -503,8 +503,9 @@ __global__ void __launch_bounds__(256) kb_final(bt_launch L)
 
 namespace {
 
-enum kind_t { K_HALF, K_DOWNCOV, K_DOWN, K_DNASM, K_DOUB, K_GAUSS, K_DEVELOP, K_PYR1, K_REDUCE, K_ASM, K_FINAL };
-const int kind_group[] = { 0, 1, 1, 2, 3, 4, 5, 6, 7, 7, 8 };
+enum kind_t { K_HALF, K_DOWNCOV, K_DOWN, K_DNASM, K_DOUB, K_GAUSS, K_DEVELOP, K_DEVELOP_XT, K_PYR1, K_REDUCE, K_ASM,
+              K_FINAL };
+const int kind_group[] = { 0, 1, 1, 2, 3, 4, 5, 5, 6, 7, 7, 8 };
 const char *group_name[VKDT_BATCH_STAGES] = { "half", "wavelet", "dnasm", "doub", "gauss", "develop", "pyr1",
                                               "pyramid", "final" };
 
@@ -599,7 +600,7 @@ struct vkdt_batch_t
 
 // the device tables at the end of the scratch: image records, job lists
 // (at most one job per image and launch) and the tile counters
-#define BT_MAX_LAUNCH (9 + 2 * BT_LEVELS)
+#define BT_MAX_LAUNCH (10 + 2 * BT_LEVELS)
 static size_t table_bytes(int n)
 {
   return align256(sizeof(bt_img) * n) + align256(sizeof(bt_job) * (size_t)BT_MAX_LAUNCH * n)
@@ -701,7 +702,7 @@ static int tiles_of(int kind, const dims_t &m, int level, int rows, int &tx)
     case K_DOWN: w = m.qw; h = m.qh; th = BY * DOWN_R; break;
     case K_DOUB: w = m.W; h = m.H; th = BY * rows; break;
     case K_GAUSS: w = m.cw; h = m.ch; th = BY * rows; break;
-    case K_DEVELOP: w = m.W; h = m.H; th = BY * DEV_R; break;
+    case K_DEVELOP: case K_DEVELOP_XT: w = m.W; h = m.H; th = BY * DEV_R; break;
     case K_PYR1: w = m.lw[1]; h = m.lh[1]; break;
     case K_REDUCE: w = m.lw[level]; h = m.lh[level]; tw = 4 * BX; break;
     case K_ASM: w = m.lw[level]; h = m.lh[level]; th = BY * ASM_R; break;
@@ -725,6 +726,8 @@ static void build_launches(vkdt_batch_t *b)
       const dims_t &m = b->dm[i];
       if(kind == K_REDUCE && level >= m.nl) continue;
       if(kind == K_ASM && level > m.nl - 2) continue;
+      if(kind == K_DEVELOP && m.block == 3) continue;      // Bayer and X-Trans develop
+      if(kind == K_DEVELOP_XT && m.block != 3) continue;   // run as two launches
       int tx;
       const int t = tiles_of(kind, m, level, rows, tx);
       if(t <= 0) continue;
@@ -742,6 +745,7 @@ static void build_launches(vkdt_batch_t *b)
   add(K_DOUB, 0, 0);
   add(K_GAUSS, 0, 0);
   add(K_DEVELOP, 0, 0);
+  add(K_DEVELOP_XT, 0, 0);
   add(K_PYR1, 1, sizeof(__half) * (b->max_ng + 1) * P1_N);
   for(int l = 2; l < b->max_nl; l++) add(K_REDUCE, l, 0);
   for(int l = b->max_nl - 2; l >= 1; l--) add(K_ASM, l, asm_lds);
@@ -835,7 +839,8 @@ static int enqueue(vkdt_batch_t *b, hipStream_t s, int timed)
       case K_DNASM:   hipLaunchKernelGGL(kb_dnasm, grid, blk, 0, s, a, b->mats); break;
       case K_DOUB:    hipLaunchKernelGGL(kb_doub, grid, blk, 0, s, a); break;
       case K_GAUSS:   hipLaunchKernelGGL(kb_gauss, grid, blk, 0, s, a); break;
-      case K_DEVELOP: hipLaunchKernelGGL(kb_develop, grid, blk, 0, s, a); break;
+      case K_DEVELOP: hipLaunchKernelGGL(kb_develop<0>, grid, blk, 0, s, a); break;
+      case K_DEVELOP_XT: hipLaunchKernelGGL(kb_develop<1>, grid, blk, 0, s, a); break;
       case K_PYR1:    hipLaunchKernelGGL(kb_pyr1, grid, blk, L.lds, s, a); break;
       case K_REDUCE:  hipLaunchKernelGGL(kb_reduce, grid, blk, 0, s, a); break;
       case K_ASM:     hipLaunchKernelGGL(kb_asm, grid, blk, L.lds, s, a); break;

@@ -501,6 +501,32 @@ DEV float gauss_w(float4 cov, float r0, float r1, int i, int j, float lo)
   return clampf(fexp(ofx * ofx * r0 + ofy * ofy * r1), lo, 1.0f);
 }
 
+// gauss_w of every offset |i|,|j| <= R for one covariance.  The arguments
+// of (i,j) and (-i,-j) are equal bit for bit (the offsets enter through
+// products whose sign flips exactly and are then squared), so each
+// exponential is taken once, for the half plane j > 0 or (j = 0, i > 0),
+// and mirrored; every tap then reads its weight whatever the CFA colour.
+template<int R>
+struct sym_weights
+{
+  float e[2 * R + 1][2 * R + 1];
+  __device__ __forceinline__ sym_weights(float4 cov, float r0, float r1, float lo)
+  {
+#pragma unroll
+    for(int j = 0; j <= R; j++)
+#pragma unroll
+      for(int i = -R; i <= R; i++)
+      {
+        if(j == 0 && i <= 0) continue;
+        const float w = gauss_w(cov, r0, r1, i, j, lo);
+        e[j + R][i + R] = w;
+        e[R - j][R - i] = w;
+      }
+    e[R][R] = gauss_w(cov, r0, r1, 0, 0, lo);
+  }
+  __device__ __forceinline__ float operator()(int i, int j) const { return e[j + R][i + R]; }
+};
+
 // splat.comp:16-140: green at (x,y).  Only the green splats reach the
 // stored channel, so only green sites are weighted (same sum and order).
 //   mos(px, py): the mosaic at (px, py), 0 outside (texelFetch);
@@ -511,6 +537,7 @@ DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
   const float4 cov = XT ? fetch_h4(ga, x / 3, y / 3) : fetch_h4(ga, (x + 1) / 2, (y + 1) / 2);
   const float r0 = -0.5f * frcp(clampf(cov.x, 0.01f, 25.0f)), r1 = -0.5f * frcp(clampf(cov.y, 0.01f, 25.0f));
   constexpr int r = XT ? 2 : 1;
+  const sym_weights<r> tapw(cov, r0, r1, 1e-4f);
   float g = 0.0f, w = 0.0f;
 #pragma unroll
   for(int j = -r; j <= r; j++)
@@ -525,7 +552,7 @@ DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
       if(px >= W) px -= 6;
       if(py >= H) py -= 6;
       const float col = mos(px, py);
-      const float wt = (i == 0 && j == 0) ? 666.0f : gauss_w(cov, r0, r1, i, j, 1e-4f);
+      const float wt = (i == 0 && j == 0) ? 666.0f : tapw(i, j);
       g += col * wt;
       w += wt;
     }
@@ -541,9 +568,47 @@ DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
 // fix.comp:21-135: rgb at (x,y) from the mosaic, the green plane (through
 // `green(px,py)`, mirrored coordinates) and the block covariance.  Green
 // sites copy the centre green, red and blue sites are ratio-weighted.
-//   mos(px, py): the mosaic at mirrored (px, py)
+//   mos(px, py): the mosaic at mirrored (px, py); R the fix radius
+template<int XT, int R, typename MosAt, typename GreenAt>
+DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int x, int y, GreenAt green)
+{
+  float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
+  const float gc = green(x, y);
+  const float4 cov = XT ? fetch_h4(cv, (x + 1) / 3, (y + 1) / 3) : fetch_h4(cv, (x + 1) / 2, (y + 1) / 2);
+  const float hi = XT ? 10.0f : 49.0f, sc = XT ? 3.0f : 2.0f;
+  const float r0 = -0.5f * frcp(sc * clampf(cov.x, 1.0f, hi)), r1 = -0.5f * frcp(sc * clampf(cov.y, 1.0f, hi));
+  const float gcn = 1e-4f + gc;
+  const sym_weights<R> tapw(cov, r0, r1, 1e-3f);
+  bool any_green = false;
+#pragma unroll
+  for(int j = -R; j <= R; j++)
+#pragma unroll
+    for(int i = -R; i <= R; i++)
+    {
+      const int px = x + i, py = y + j;   // may be negative: truncating division as in GLSL
+      int k;
+      if(XT)
+      {
+        const int blue_top = ((px / 3 + py / 3) & 1) > 0;
+        const int mx = px - (px / 3) * 3, my = py - (py / 3) * 3;
+        k = (((mx + my) & 1) == 0) ? 1 : ((blue_top ^ (my == 1)) ? 2 : 0);
+      }
+      else k = (((px + py) & 1) == 1) ? 1 : ((px & 1) == 0 ? 0 : 2);
+      if(k == 1) { any_green = true; continue; }
+      const float gh = green(px, py);
+      const float col = mos(px, py);
+      const float wt = tapw(i, j);
+      const float v = col * gcn * frcp(1e-4f + gh) * wt;
+      if(k == 0) { sr += v; wr += wt; }
+      else       { sb += v; wb += wt; }
+    }
+  const float g = any_green ? gc : 0.0f;
+  return make_float4(sr * frcp(fmaxf(1e-8f, wr)), g, sb * frcp(fmaxf(1e-8f, wb)), 1.0f);
+}
+
+// the same for any radius r, one weight per tap
 template<int XT, typename MosAt, typename GreenAt>
-DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt green)
+DEV float4 dm_fix_loop(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt green)
 {
   float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
   const float gc = green(x, y);
@@ -576,10 +641,20 @@ DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt gre
   return make_float4(sr * frcp(fmaxf(1e-8f, wr)), g, sb * frcp(fmaxf(1e-8f, wb)), 1.0f);
 }
 
+// the fix radius r is uniform per image (1..2 Bayer, 2..3 X-Trans); the
+// default radius (demosaic colour 0) runs unrolled with symmetric weights
+template<int XT, typename MosAt, typename GreenAt>
+DEV float4 dm_fix_r(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt green)
+{
+  constexpr int R0 = XT ? 2 : 1;
+  if(r == R0) return dm_fix_at<XT, R0>(mos, cv, x, y, green);
+  return dm_fix_loop<XT>(mos, cv, r, x, y, green);
+}
+
 template<int XT, typename GreenAt>
 DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
 {
-  return dm_fix_at<XT>([&](int px, int py) { return centre_h1(in, px, py); }, cv, r, x, y, green);
+  return dm_fix_r<XT>([&](int px, int py) { return centre_h1(in, px, py); }, cv, r, x, y, green);
 }
 
 // ==========================================================================
