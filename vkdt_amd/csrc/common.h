@@ -235,3 +235,14 @@ DEV int2 xcd_block()
 }
 DEV int tid_x() { return xcd_block().x * BX + threadIdx.x; }
 DEV int tid_y() { return xcd_block().y * BY + threadIdx.y; }
+
+// Bayer-uniform waves: the 64x4 texels of a tile at (X0,Y0) handed out so
+// that each wave (threadIdx.y) holds one CFA colour, 32 texels of each of
+// two rows of equal parity.  Per-tap colour tests are then uniform across
+// the wave and the taps a colour skips are branched over.  A permutation
+// of the tile's texels.
+DEV int2 bayer_wave_xy(int X0, int Y0)
+{
+  const int l = threadIdx.x, w = threadIdx.y;
+  return make_int2(X0 + 2 * (l & 31) + (w & 1), Y0 + (w >> 1) + 2 * (l >> 5));
+}

@@ -315,19 +315,33 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
     }
     __syncthreads();
     auto mosl = [&](int px, int py) { return mt[(py - Y0 + m) * mw + (px - X0 + m)]; };
-    for(int i = tid; i < gw * gh; i += BX * BY)
-    {
-      const int ty = i / gw, tx = i - ty * gw;
-      gt[i] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
+    if(XT)
+      for(int i = tid; i < gw * gh; i += BX * BY)
+      {
+        const int ty = i / gw, tx = i - ty * gw;
+        gt[i] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
+      }
+    else
+    { // Bayer: the sites with tx + ty even first, then the odd ones, so that
+      // consecutive lanes share one splat pattern (X0, Y0 are even and gw is
+      // even, so the parity of tx + ty is that of the texel's x + y)
+      const int hw = gw / 2, ht2 = gh * hw;
+      for(int e = tid; e < gw * gh; e += BX * BY)
+      {
+        const int c = e >= ht2, f = e - c * ht2;
+        const int ty = f / hw, tx = 2 * (f - ty * hw) + ((ty + c) & 1);
+        gt[ty * gw + tx] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
+      }
     }
     __syncthreads();
 #pragma unroll 1
     for(int k = 0; k < DEV_R; k++)
-    {
-      const int y = Y0 + threadIdx.y + BY * k;
-      const float4 f = dm_fix_r<XT>(mosl, cov, r, x, y, green);
+    { // Bayer: each wave one CFA colour (bayer_wave_xy) in each 64x4 band
+      const int2 q = XT ? make_int2(x, Y0 + threadIdx.y + BY * k) : bayer_wave_xy(X0, Y0 + BY * k);
+      const int xq = q.x, y = q.y;
+      const float4 f = dm_fix_r<XT>(mosl, cov, r, xq, y, green);
       const float4 c = r16(co_px(r16(f), co));
-      st_h4(film, x, y, fc_px(c, fc));
+      st_h4(film, xq, y, fc_px(c, fc));
     }
     return;
   }

@@ -49,10 +49,20 @@ __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
   st_h4(out, x, y, dm_gauss_px<XT>(orig, x, y));
 }
 
+// texel of this thread: Bayer tiles are handed out colour-uniform per wave
+template<int XT>
+DEV int2 cfa_xy()
+{
+  const int2 t = xcd_block();
+  if(XT) return make_int2(t.x * BX + threadIdx.x, t.y * BY + threadIdx.y);
+  return bayer_wave_xy(t.x * BX, t.y * BY);
+}
+
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 {
-  const int x = tid_x(), y = tid_y();
+  const int2 p = cfa_xy<XT>();
+  const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
   st_h1(out, x, y, dm_splat_px<XT>(in, ga, x, y));
 }
@@ -60,7 +70,8 @@ __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 template<int XT>
 __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg out, int r)
 {
-  const int x = tid_x(), y = tid_y();
+  const int2 p = cfa_xy<XT>();
+  const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
   st_h4(out, x, y, dm_fix_px<XT>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
 }
