@@ -147,10 +147,22 @@ static inline double st(const or_buf_t *b, double x)
   return (b->format == OR_F16 && g_storage_f16) ? or_f16_round(x) : x;
 }
 
+// float -> UNORM16 conversion of an R16_UNORM store: round(clamp(v,0,1) * 65535)
+// to nearest, ties to even (parity unpinned: the Vulkan spec leaves the tie
+// rule to the implementation)
+uint16_t or_unorm16(double v)
+{
+  const float f = (float)v;
+  if(!(f > 0.0f)) return 0;   // NaN and negatives
+  if(f >= 1.0f) return 65535;
+  return (uint16_t)nearbyintf(f * 65535.0f);
+}
+
 static inline void store(const or_buf_t *b, int x, int y, int pl, v4 v)
 {
   if(x < 0 || y < 0 || x >= b->wd || y >= b->ht) return;
   const size_t idx = ((size_t)pl * b->ht + y) * b->wd + x;
+  if(b->format == OR_UI16) { ((uint16_t *)b->data)[idx] = or_unorm16(v.x); return; }
   double *d = (double *)b->data;
   if(b->chan == 1) { d[idx] = st(b, v.x); return; }
   d[4 * idx + 0] = st(b, v.x);
@@ -1029,6 +1041,240 @@ static int ll_colour(const or_buf_t *c)
 }
 
 /* ------------------------------------------------------------------------
+ * hilite (src/pipe/modules/hilite): highlight inpainting on the mosaic
+ * pc: wb[4] (float bits), filters; params: white, desat, soft
+ * --------------------------------------------------------------------- */
+
+// the 2x2 block at (2x,2y) as textureGather returns it: x (i0,j1), y (i1,j1),
+// z (i1,j0), w (i0,j0), mirrored addresses
+static inline v4 gather2(const or_buf_t *in, int x, int y)
+{
+  const int x0 = or_mirror(2 * x, in->wd), x1 = or_mirror(2 * x + 1, in->wd);
+  const int y0 = or_mirror(2 * y, in->ht), y1 = or_mirror(2 * y + 1, in->ht);
+  v4 c = { R(load_raw(in, x0, y1, 0, 0)), R(load_raw(in, x1, y1, 0, 0)),
+           R(load_raw(in, x1, y0, 0, 0)), R(load_raw(in, x0, y0, 0, 0)) };
+  return c;
+}
+
+// hilite/half.comp:20-71
+static int hl_half(const int32_t *pci, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const real white = R(par[0]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 rgba;
+    if(pci[4] == 9)
+    {
+      real cc[9];
+      for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) cc[3 * i + j] = fetch(in, 3 * x + i, 3 * y + j, 0).x;
+      if(cc[1] >= white) cc[1] = cc[7];
+      if(cc[7] >= white) cc[7] = cc[1];
+      if(cc[3] >= white) cc[3] = cc[5];
+      if(cc[5] >= white) cc[5] = cc[3];
+      const real maxg = FMAX(FMAX(FMAX(cc[0], cc[2]), cc[4]), FMAX(cc[6], cc[8]));
+      if(maxg >= white) cc[0] = cc[2] = cc[4] = cc[6] = cc[8] = R(1.0);
+      const real c0 = (cc[1] + cc[7]) * R(0.5), c1 = (cc[3] + cc[5]) * R(0.5);
+      if(((x + y) & 1) > 0) { rgba.x = c0; rgba.z = c1; }
+      else                  { rgba.z = c0; rgba.x = c1; }
+      rgba.y = (cc[0] + cc[2] + cc[4] + cc[6] + cc[8]) / R(5.0);
+      rgba.w = R(1.0);
+    }
+    else
+    {
+      v4 g = gather2(in, x, y);
+      if(g.x >= white) g.x = g.z;
+      if(g.z >= white) g.z = g.x;
+      rgba.x = g.w; rgba.y = (g.x + g.z) / R(2.0); rgba.z = g.y; rgba.w = R(1.0);
+    }
+    store(out, x, y, 0, rgba);
+  }
+  return 0;
+}
+
+// hilite/reduce.comp:19-60
+static int hl_reduce(const float *pcf, const float *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  real white = R(par[0]);
+  if(!(white > R(0.0))) white = R(1.0);
+  const real w[5] = { R(1.0) / R(16.0), R(4.0) / R(16.0), R(6.0) / R(16.0), R(4.0) / R(16.0), R(1.0) / R(16.0) };
+  const real sw[5] = { R(1.0), R(2.0), R(0.0), R(-2.0), R(-1.0) };
+  real wb[3] = { R(pcf[0]), R(pcf[1]), R(pcf[2]) };
+  if(!(wb[0] * wb[0] + wb[1] * wb[1] + wb[2] * wb[2] > R(1e-3))) wb[0] = wb[1] = wb[2] = R(1.0);
+  const real ds = R(par[1]) * R(par[1]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real ex = 0, ey = 0, col[3] = { 0, 0, 0 }, wgt = 0;
+    for(int jj = -2; jj <= 2; jj++) for(int ii = -2; ii <= 2; ii++)
+    {
+      const v4 t = centre(in, 2 * x + ii, 2 * y + jj, 0);
+      real rgb[3] = { t.x, t.y, t.z };
+      const real l = lum2020(rgb[0], rgb[1], rgb[2]);
+      ex += w[jj + 2] * sw[ii + 2] * l;
+      ey += w[ii + 2] * sw[jj + 2] * l;
+      const real u = w[ii + 2] * w[jj + 2];
+      const real rw = rgb[0] * wb[0], gw = rgb[1] * wb[1], bw = rgb[2] * wb[2];
+      const real cmax = FMAX(rw, FMAX(gw, bw)), cmin = FMIN(rw, FMIN(gw, bw));
+      const real sat = (cmax - cmin) / FMAX(R(1e-3), cmax);
+      if(rgb[0] < white && rgb[1] < white && rgb[2] < white)
+      {
+        const real sm = smoothstepr(R(0.2), R(1.0), FMAX(rgb[0], FMAX(rgb[1], rgb[2])) / white);
+        real tt = smoothstepr(R(0.15), R(0.9), sat);
+        tt = clampr(R(5.0) * ds * ds * sm * tt, R(0.0), R(1.0));
+        for(int k = 0; k < 3; k++)
+        {
+          rgb[k] = mixr(rgb[k], (cmax + cmin) / wb[k] * R(0.5), tt);
+          col[k] += rgb[k] * u;
+        }
+        wgt += u;
+      }
+    }
+    v4 o;
+    if(wgt == R(0.0)) { o.x = o.y = o.z = R(1.0); }
+    else { o.x = col[0] / wgt; o.y = col[1] / wgt; o.z = col[2] / wgt; }
+    o.w = SQRT(ex * ex + ey * ey);
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// hilite/assemble.comp:23-118: expand the coarse level (the 5-tap binomial
+// on the taps of the fine texel's parity) and blend it into the fine level
+static void hl_expand(const or_buf_t *img, int ox, int oy, real rgb[3])
+{
+  const real w[5] = { R(1.0) / R(16.0), R(4.0) / R(16.0), R(6.0) / R(16.0), R(4.0) / R(16.0), R(1.0) / R(16.0) };
+  real cx = 0, cy = 0, cz = 0, wgt = 0;
+  const int ix = ox / 2, iy = oy / 2;
+  const int d = (ox & 1) + 2 * (oy & 1);
+  const int i0 = (d & 1) ? 0 : -1, j0 = (d & 2) ? 0 : -1;
+  for(int ii = i0; ii <= 1; ii++) for(int jj = j0; jj <= 1; jj++)
+  {
+    const v4 t = centre(img, ix + ii, iy + jj, 0);
+    const real wx = (d & 1) ? w[2 * ii + 1] : w[2 * ii + 2];
+    const real wy = (d & 2) ? w[2 * jj + 1] : w[2 * jj + 2];
+    cx += t.x * wy * wx; cy += t.y * wy * wx; cz += t.z * wy * wx;
+    wgt += wy * wx;
+  }
+  if(wgt == R(0.0)) { rgb[0] = 0; rgb[1] = 1; rgb[2] = 1; return; }
+  rgb[0] = cx / wgt; rgb[1] = cy / wgt; rgb[2] = cz / wgt;
+}
+
+static inline void hl_scale_weights(const real up[3], real *wr, real *wg, real *wb)
+{
+  *wr = EXP(up[0] - FMAX(up[1], up[2]));
+  *wg = EXP(up[1] - FMAX(up[0], up[2]));
+  *wb = EXP(up[2] - FMAX(up[0], up[1]));
+}
+
+static int hl_assemble(const float *par, const or_buf_t *c)
+{
+  const or_buf_t *fine = c + 0, *coarse = c + 1, *out = c + 2;
+  const real white = R(par[0]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    real up[3];
+    hl_expand(coarse, x, y, up);
+    v4 f = fetch(fine, x, y, 0);
+    const real sr = f.x / FMAX(R(0.001), up[0]);
+    const real sg = f.y / FMAX(R(0.001), up[1]);
+    const real sb = f.z / FMAX(R(0.001), up[2]);
+    real wr, wg, wb;
+    hl_scale_weights(up, &wr, &wg, &wb);
+    const real scale = (sr * wr + sg * wg + sb * wb) / (wr + wg + wb);
+    real rec[3];
+    for(int k = 0; k < 3; k++) rec[k] = clampr(up[k] * scale, R(-65535.0), R(65535.0));
+    real t = R(par[2]);
+    if(f.x >= white || f.y >= white || f.z >= white) t = R(1.0);
+    if(isnan(f.w)) f.w = R(0.0);
+    t = clampr(mixr(t, R(1.0), SQRT(FMAX(R(0), f.w))), R(0.0), R(1.0));
+    v4 o = { mixr(f.x, rec[0], t), mixr(f.y, rec[1], t), mixr(f.z, rec[2], t), R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// hilite/doub.comp:20-121: back to the mosaic, clipped sites filled from
+// the reconstruction scaled to the block's unclipped values (ui16 output)
+static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int wd, int ht)
+{
+  const or_buf_t *in = c + 0, *coarse = c + 1, *out = c + 2;
+  const real white = R(par[0]);
+  const real softw = R(0.97) * white;
+  // every invocation of the ceil(wd/8) x ceil(ht/8) groups of 8x8 runs
+  // unless its block index lies outside the output's extent (doub.comp:24-25)
+  const int nx = (wd + 7) / 8 * 8, ny = (ht + 7) / 8 * 8;
+  for(int y = 0; y < ny; y++) for(int x = 0; x < nx; x++)
+  {
+    if(x >= out->wd || y >= out->ht) continue;
+    const v4 u = fetch(coarse, x, y, 0);
+    real up[3] = { u.x, u.y, u.z };
+    if(pci[4] == 9)
+    {
+      if(((x + y) & 1) == 0) { const real t = up[0]; up[0] = up[2]; up[2] = t; }
+      real cc[9];
+      for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) cc[3 * i + j] = fetch(in, 3 * x + i, 3 * y + j, 0).x;
+      const real minr = FMIN(cc[1], cc[7]), minb = FMIN(cc[3], cc[5]);
+      const real ming = FMIN(FMIN(FMIN(cc[0], cc[2]), cc[4]), FMIN(cc[6], cc[8]));
+      const real sr = minr / FMAX(R(0.001), up[0]);
+      const real sg = ming / FMAX(R(0.001), up[1]);
+      const real sb = minb / FMAX(R(0.001), up[2]);
+      real wr, wg, wb;
+      hl_scale_weights(up, &wr, &wg, &wb);
+      const real scale = (sr * wr + sg * wg + sb * wb) / (wr + wg + wb);
+      const real maxr = FMAX(cc[1], cc[7]), maxb = FMAX(cc[3], cc[5]);
+      const real maxg = FMAX(FMAX(FMAX(cc[0], cc[2]), cc[4]), FMAX(cc[6], cc[8]));
+      const real maxrgb = FMAX(maxr, FMAX(maxg, maxb));
+      if(maxrgb > softw)
+      {
+        real t = smoothstepr(softw, white, maxrgb);
+        cc[1] = mixr(cc[1], up[0] * scale, t);
+        cc[7] = mixr(cc[7], up[0] * scale, t);
+        cc[3] = mixr(cc[3], up[2] * scale, t);
+        cc[5] = mixr(cc[5], up[2] * scale, t);
+        t = smoothstepr(softw, white, ming);
+        cc[0] = mixr(cc[0], up[1] * scale, t);
+        cc[2] = mixr(cc[2], up[1] * scale, t);
+        cc[4] = mixr(cc[4], up[1] * scale, t);
+        cc[6] = mixr(cc[6], up[1] * scale, t);
+        cc[8] = mixr(cc[8], up[1] * scale, t);
+      }
+      for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++)
+      {
+        v4 o = { cc[3 * i + j], cc[3 * i + j], cc[3 * i + j], R(1.0) };
+        store(out, 3 * x + i, 3 * y + j, 0, o);
+      }
+    }
+    else
+    {
+      v4 g = gather2(in, x, y);
+      const real ming = FMIN(g.x, g.z);
+      const real sr = g.w / FMAX(R(0.001), up[0]);
+      const real sg = ming / FMAX(R(0.001), up[1]);
+      const real sb = g.y / FMAX(R(0.001), up[2]);
+      real wr, wg, wb;
+      hl_scale_weights(up, &wr, &wg, &wb);
+      const real scale = (sr * wr + sg * wg + sb * wb) / (wr + wg + wb);
+      const real maxrgb = FMAX(FMAX(g.x, g.y), FMAX(g.z, g.w));
+      if(maxrgb > softw)
+      {
+        const real t = smoothstepr(softw, white, maxrgb);
+        g.x = mixr(g.x, scale * up[1], t);   // c = mix(c, scale * upsm.gbgr, t)
+        g.y = mixr(g.y, scale * up[2], t);
+        g.z = mixr(g.z, scale * up[1], t);
+        g.w = mixr(g.w, scale * up[0], t);
+      }
+      v4 o00 = { g.w, 0, 0, 1 }, o10 = { g.z, 0, 0, 1 }, o01 = { g.x, 0, 0, 1 }, o11 = { g.y, 0, 0, 1 };
+      store(out, 2 * x, 2 * y, 0, o00);
+      store(out, 2 * x + 1, 2 * y, 0, o10);
+      store(out, 2 * x, 2 * y + 1, 0, o01);
+      store(out, 2 * x + 1, 2 * y + 1, 0, o11);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
  * node dispatch
  * --------------------------------------------------------------------- */
 
@@ -1057,6 +1303,10 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("llap", "reduce"))      return nconn >= 2 ? ll_reduce(conn, dp) : -1;
   if(IS("llap", "assemble"))    return nconn >= 4 ? ll_assemble(pci, conn) : -1;
   if(IS("llap", "colour"))      return nconn >= 3 ? ll_colour(conn) : -1;
+  if(IS("hilite", "half"))      return nconn >= 2 ? hl_half(pci, par, conn) : -1;
+  if(IS("hilite", "reduce"))    return nconn >= 2 ? hl_reduce(pcf, par, conn) : -1;
+  if(IS("hilite", "assemble"))  return nconn >= 3 ? hl_assemble(par, conn) : -1;
+  if(IS("hilite", "doub"))      return nconn >= 3 ? hl_doub(pci, par, conn, wd, ht) : -1;
 #undef IS
   return -1;
 }
@@ -1130,6 +1380,53 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
   free(half); free(asm_);
   for(int l = 0; l < 4; l++) free(s[l]);
   if(st_denoise) memcpy(st_denoise, dn, sizeof(double) * W * H);
+
+  // hilite: half -> reduce x L -> assemble x L (coarse to fine) -> doub,
+  // L reduce levels until the next extent reaches 1 or L = 14
+  // (hilite/main.c:43-85); its output is f16 like its input
+  if(p->hl_on)
+  {
+    int ew[16], eh[16], L = 0;
+    ew[0] = Qw; eh[0] = Qh;
+    for(int l = 1; l < 15; l++)
+    {
+      ew[l] = (ew[l - 1] - 1) / 2 + 1;
+      eh[l] = (eh[l - 1] - 1) / 2 + 1;
+      L = l;
+      if((ew[l] - 1) / 2 + 1 <= 1 || (eh[l] - 1) / 2 + 1 <= 1 || l + 1 == 15) break;
+    }
+    int32_t pc[5];
+    memcpy(pc, p->wb, 4 * sizeof(float));
+    pc[4] = p->filters;
+    float hp[3] = { p->hl_white, p->hl_desat, p->hl_soft };
+    double *lev[16], *asmo[16];
+    for(int l = 0; l <= L; l++) lev[l] = alloc_img(ew[l], eh[l], 4, 1);
+    or_buf_t ch[2] = { { dn, W, H, 1, OR_F16, 1 }, { lev[0], Qw, Qh, 4, OR_F16, 1 } };
+    err |= or_node("hilite", "half", pc, sizeof(pc), hp, sizeof(hp), ch, 2, Qw, Qh, 1);
+    for(int l = 1; l <= L; l++)
+    {
+      or_buf_t cr[2] = { { lev[l - 1], ew[l - 1], eh[l - 1], 4, OR_F16, 1 }, { lev[l], ew[l], eh[l], 4, OR_F16, 1 } };
+      err |= or_node("hilite", "reduce", pc, sizeof(pc), hp, sizeof(hp), cr, 2, ew[l], eh[l], 1);
+    }
+    double *coarse = lev[L];
+    int cw = ew[L], chh = eh[L];
+    for(int l = L; l >= 1; l--)
+    {
+      asmo[l - 1] = alloc_img(ew[l - 1], eh[l - 1], 4, 1);
+      or_buf_t ca[3] = { { lev[l - 1], ew[l - 1], eh[l - 1], 4, OR_F16, 1 }, { coarse, cw, chh, 4, OR_F16, 1 },
+                         { asmo[l - 1], ew[l - 1], eh[l - 1], 4, OR_F16, 1 } };
+      err |= or_node("hilite", "assemble", pc, sizeof(pc), hp, sizeof(hp), ca, 3, ew[l - 1], eh[l - 1], 1);
+      if(l < L) free(coarse);
+      coarse = asmo[l - 1]; cw = ew[l - 1]; chh = eh[l - 1];
+    }
+    double *hl = alloc_img(W, H, 1, 1);
+    or_buf_t cd[3] = { { dn, W, H, 1, OR_F16, 1 }, { coarse, Qw, Qh, 4, OR_F16, 1 }, { hl, W, H, 1, OR_F16, 1 } };
+    err |= or_node("hilite", "doub", pc, sizeof(pc), hp, sizeof(hp), cd, 3, Qw, Qh, 1);
+    free(coarse);
+    for(int l = 0; l <= L; l++) free(lev[l]);
+    free(dn);
+    dn = hl;
+  }
 
   // demosaic: gauss -> splat -> fix ("down" is wired but never sampled)
   double *cov = alloc_img(Qw, Qh, 4, 1), *green = alloc_img(W, H, 1, 1), *dm = alloc_img(W, H, 4, 1);

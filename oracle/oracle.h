@@ -51,6 +51,8 @@ int or_node(
 double or_f16_round(double x);
 // mirrored-repeat texel index as the sampler computes it (exposed for tests)
 int or_mirror(int i, int n);
+// float -> R16_UNORM store conversion
+uint16_t or_unorm16(double v);
 // bilinear sample of a single-channel double image at texture-space (u, v)
 double or_bilinear(const double *img, int wd, int ht, double u, double v);
 // textureGather component 0 of a ui16 image: (i0j1, i1j1, i1j0, i0j0)
@@ -83,6 +85,9 @@ typedef struct or_develop_t
   float   ll_sigma, ll_shadows, ll_hilights, ll_clarity;
   int32_t ll_num_gamma;       // curve samples (10 in the reference)
   int32_t ll_max_levels;      // pyramid depth cap (12 in the reference)
+  // hilite between denoise and demosaic (bin/default-darkroom.i-raw:17-18)
+  int32_t hl_on;              // 0: denoise feeds demosaic directly
+  float   hl_white, hl_desat, hl_soft;
 }
 or_develop_t;
 

@@ -83,6 +83,10 @@ class DevelopParams:
     clarity: float = 0.2
     num_gamma: int = 10
     max_levels: int = 12
+    hilite: bool = False
+    hl_white: float = 0.985
+    hl_desat: float = 0.3
+    hl_soft: float = 0.6
     cam: np.ndarray = field(default_factory=lambda: cam_to_rec2020(synth.xyz_to_cam()))
 
     @property
@@ -187,14 +191,19 @@ def build(W, H, p):
                     + [("f", 0.0)] * 4)
     add("denoise", "doub", pc, dpar, ["raw", "asm", "half", "dn"], W, H)
 
+    mos = "dn"
+    if p.hilite:
+        mos = "hl"
+        add_hilite(W, H, p, buf, add, "dn", "hl")
+
     dmpc = mixed_blob([("f", v) for v in wb] + [("i", p.filters)])
     dmpar = mixed_blob([("i", p.dm_colour), ("i", 0)])
     buf("cov", Qw, Qh, 4, F16)
-    add("demosaic", "gauss", dmpc, dmpar, ["dn", "dn", "cov"], Qw, Qh)
+    add("demosaic", "gauss", dmpc, dmpar, [mos, mos, "cov"], Qw, Qh)
     buf("green", W, H, 1, F16)
-    add("demosaic", "splat", dmpc, dmpar, ["dn", "cov", "green"], W, H)
+    add("demosaic", "splat", dmpc, dmpar, [mos, "cov", "green"], W, H)
     buf("dm", W, H, 4, F16)
-    add("demosaic", "fix", dmpc, dmpar, ["dn", "green", "cov", "dm"], W, H)
+    add("demosaic", "fix", dmpc, dmpar, [mos, "green", "cov", "dm"], W, H)
 
     buf("co", W, H, 4, F16)
     add("colour", "main", mixed_blob([("i", 0)] * 3), colour_params(p), ["dm", "co"], W, H)
@@ -221,6 +230,40 @@ def build(W, H, p):
     return bufs, nodes
 
 
+def hilite_levels(Qw, Qh):
+    """extents of the hilite pyramid: level 0 the block grid, then one per
+    reduce node (hilite/main.c:43-85)"""
+    ew, eh = [Qw], [Qh]
+    for l in range(1, 15):
+        ew.append((ew[-1] - 1) // 2 + 1)
+        eh.append((eh[-1] - 1) // 2 + 1)
+        if (ew[-1] - 1) // 2 + 1 <= 1 or (eh[-1] - 1) // 2 + 1 <= 1 or l + 1 == 15:
+            break
+    return ew, eh
+
+
+def add_hilite(W, H, p, buf, add, src, dst, fmt_out=F16):
+    """hilite's nodes from mosaic buffer `src` to `dst` (hilite/main.c:6-86)"""
+    block = 3 if p.kind == "xtrans" else 2
+    Qw, Qh = W // block, H // block
+    pc = mixed_blob([("f", v) for v in p.wb] + [("i", p.filters)])
+    par = f32blob([p.hl_white, p.hl_desat, p.hl_soft])
+    ew, eh = hilite_levels(Qw, Qh)
+    L = len(ew) - 1
+    for l in range(L + 1):
+        buf(f"hl_l{l}", ew[l], eh[l], 4, F16)
+    add("hilite", "half", pc, par, [src, "hl_l0"], Qw, Qh)
+    for l in range(1, L + 1):
+        add("hilite", "reduce", pc, par, [f"hl_l{l-1}", f"hl_l{l}"], ew[l], eh[l])
+    coarse = f"hl_l{L}"
+    for l in range(L, 0, -1):
+        buf(f"hl_a{l-1}", ew[l - 1], eh[l - 1], 4, F16)
+        add("hilite", "assemble", pc, par, [f"hl_l{l-1}", coarse, f"hl_a{l-1}"], ew[l - 1], eh[l - 1])
+        coarse = f"hl_a{l-1}"
+    buf(dst, W, H, 1, fmt_out)
+    add("hilite", "doub", pc, par, [src, coarse, dst], Qw, Qh)
+
+
 def oracle_params(p):
     """tests.oracle_py.OrDevelop for the same graph."""
     from oracle_py import OrDevelop
@@ -243,4 +286,6 @@ def oracle_params(p):
     o.fc_light, o.fc_contrast, o.fc_bias, o.fc_colour = p.light, p.contrast, p.bias, p.fc_colour
     o.ll_sigma, o.ll_shadows, o.ll_hilights, o.ll_clarity = p.sigma, p.shadows, p.hilights, p.clarity
     o.ll_num_gamma, o.ll_max_levels = p.num_gamma, p.max_levels
+    o.hl_on = 1 if p.hilite else 0
+    o.hl_white, o.hl_desat, o.hl_soft = p.hl_white, p.hl_desat, p.hl_soft
     return o

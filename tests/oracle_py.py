@@ -31,6 +31,7 @@ class OrDevelop(C.Structure):
         ("fc_light", C.c_float), ("fc_contrast", C.c_float), ("fc_bias", C.c_float), ("fc_colour", C.c_int32),
         ("ll_sigma", C.c_float), ("ll_shadows", C.c_float), ("ll_hilights", C.c_float), ("ll_clarity", C.c_float),
         ("ll_num_gamma", C.c_int32), ("ll_max_levels", C.c_int32),
+        ("hl_on", C.c_int32), ("hl_white", C.c_float), ("hl_desat", C.c_float), ("hl_soft", C.c_float),
     ]
 
 

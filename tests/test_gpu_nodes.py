@@ -22,7 +22,11 @@ from vkdt_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("bayer", 256, 192, 7), ("xtrans", 258, 198, 11)]
+# (kind, W, H, seed, hilite): the last two insert hilite between denoise and
+# demosaic (bin/default-darkroom.i-raw:17-18)
+CASES = [("bayer", 256, 192, 7, False), ("xtrans", 258, 198, 11, False),
+         ("bayer", 256, 192, 17, True), ("xtrans", 258, 198, 19, True)]
+IDS = [c[0] + ("-hilite" if c[4] else "") for c in CASES]
 
 
 def run_gpu_graph(hip, bufs, nodes, raw):
@@ -61,14 +65,15 @@ def f16_ulps(a, b):
     return d
 
 
-@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+@pytest.fixture(scope="module", params=CASES, ids=IDS)
 def graph_run(request, hip):
-    kind, W, H, seed = request.param
+    kind, W, H, seed, hilite = request.param
     cfa, _ = synth.mosaic(W, H, seed, kind)
     na, nb = synth.noise_params(2e-4, 1e-6)
-    p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb)
+    wb = (1.8, 1.0, 1.4, 1.0) if hilite else (1.0, 1.0, 1.0, 1.0)
+    p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb, hilite=hilite, wb=wb)
     bufs, nodes = gs.build(W, H, p)
-    print(f"[gpu_nodes] {kind} {W}x{H} seed {seed}: {len(nodes)} nodes", flush=True)
+    print(f"[gpu_nodes] {kind} {W}x{H} seed {seed} hilite {hilite}: {len(nodes)} nodes", flush=True)
     host = run_gpu_graph(hip, bufs, nodes, cfa)
     return kind, W, H, cfa, p, bufs, nodes, host
 

@@ -9,25 +9,26 @@ import graphspec as gs
 from vkdt_amd import pipe, synth
 
 
-def planned(tmp_path, kind, W, H, extra=()):
+def planned(tmp_path, kind, W, H, extra=(), hilite=False):
     cfa, _ = synth.mosaic(W, H, 1, kind)
     path = str(tmp_path / "p.dng")
     synth.write_dng(path, cfa, kind, as_shot_neutral=(0.45, 1.0, 0.7))
     na, nb = synth.noise_params(2e-4, 1e-6)
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(path, noise_a=na, noise_b=nb, extra=extra))
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(path, noise_a=na, noise_b=nb, extra=extra, hilite=hilite))
     g.export("o-pfm", str(tmp_path / "out"))
     g.plan()
     g.commit()
     return g, np.float32(na), np.float32(nb)
 
 
-@pytest.mark.parametrize("kind,W,H", [("bayer", 200, 150), ("xtrans", 201, 150)])
-def test_nodes_match_restatement(tmp_path, kind, W, H):
-    g, na, nb = planned(tmp_path, kind, W, H)
+@pytest.mark.parametrize("kind,W,H,hilite", [("bayer", 200, 150, False), ("xtrans", 201, 150, False),
+                                             ("bayer", 200, 150, True), ("xtrans", 201, 150, True)])
+def test_nodes_match_restatement(tmp_path, kind, W, H, hilite):
+    g, na, nb = planned(tmp_path, kind, W, H, hilite=hilite)
     ip = g.img_param("i-raw", "main")
     W, H = ip["full"]
     p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb, cam=np.array(ip["cam"], np.float32).reshape(3, 3),
-                         wb=tuple(ip["wb"][:3]) + (ip["wb"][3],))
+                         wb=tuple(ip["wb"][:3]) + (ip["wb"][3],), hilite=hilite)
     _, ref = gs.build(W, H, p)
     nodes = {n["id"]: n for n in g.nodes()}
     live = [nodes[k] for k in g.exec_nodes() if nodes[k]["module"] not in ("i-raw", "o-pfm")]
@@ -119,3 +120,17 @@ def test_cfg_file_round(tmp_path):
     ip = g.img_param("i-raw", "main")
     assert ip["full"] == (64, 48)
     assert np.allclose(ip["noise"], [1e-7, 2e-5], rtol=1e-6)
+
+
+def test_hilite_formats_follow_the_module_connectors(tmp_path):
+    """hilite's half/doub declare ui16 mosaics but dt_connector_copy gives them
+    the module's negotiated format: f16 behind denoise (api.h:88-121)"""
+    g, _, _ = planned(tmp_path, "bayer", 96, 64, hilite=True)
+    nodes = g.nodes()
+    half = [n for n in nodes if n["module"] == "hilite" and n["kernel"] == "half"][0]
+    doub = [n for n in nodes if n["module"] == "hilite" and n["kernel"] == "doub"][0]
+    assert half["conns"][0]["format"] == "f16"
+    assert doub["conns"][0]["format"] == "f16" and doub["conns"][2]["format"] == "f16"
+    red = [n for n in nodes if n["module"] == "hilite" and n["kernel"] == "reduce"]
+    # block grid 48x32: levels 24x16 ... until the next would be 1 texel
+    assert [(n["wd"], n["ht"]) for n in red] == [(24, 16), (12, 8), (6, 4), (3, 2)]

@@ -22,7 +22,11 @@ struct dimg
   int      pitch;
   int      chan;
   long     pstride;
+  int      fmt;       // 1 ui16 (UNORM), 2 f16, 3 f32 (the VKDT_FMT_* codes)
 };
+#define DIMG_UI16 1
+#define DIMG_F16  2
+#define DIMG_F32  3
 
 DEV float clampf(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
 DEV float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
@@ -64,6 +68,12 @@ DEV float ld_u16(const dimg &b, int x, int y)
 { // R16_UNORM: the float32 value v/65535
   const uint16_t v = ((const uint16_t *)b.p)[(long)y * b.pitch + x];
   return (float)v / 65535.0f;
+}
+
+DEV void st_u16(const dimg &b, int x, int y, float v)
+{ // float -> R16_UNORM store: round(clamp(v,0,1) * 65535), ties to even
+  const float c = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;   // NaN -> 0
+  ((uint16_t *)b.p)[(long)y * b.pitch + x] = (uint16_t)__builtin_rintf(c * 65535.0f);
 }
 
 DEV float ld_h1(const dimg &b, int x, int y, int pl = 0)
@@ -117,6 +127,15 @@ DEV float fetch_u16(const dimg &b, int x, int y)
 // texture() at a texel centre
 DEV float4 centre_h4(const dimg &b, int x, int y) { return ld_h4(b, mirror(x, b.wd), mirror(y, b.ht)); }
 DEV float  centre_h1(const dimg &b, int x, int y, int pl = 0) { return ld_h1(b, mirror(x, b.wd), mirror(y, b.ht), pl); }
+
+// a single-channel mosaic stored as f16 or ui16 (a hilite output is UNORM16)
+DEV float ld_m(const dimg &b, int x, int y) { return b.fmt == DIMG_UI16 ? ld_u16(b, x, y) : ld_h1(b, x, y); }
+DEV float fetch_m(const dimg &b, int x, int y)
+{
+  if(x < 0 || y < 0 || x >= b.wd || y >= b.ht) return 0.0f;
+  return ld_m(b, x, y);
+}
+DEV float centre_m(const dimg &b, int x, int y) { return ld_m(b, mirror(x, b.wd), mirror(y, b.ht)); }
 DEV float  centre_u16(const dimg &b, int x, int y) { return ld_u16(b, mirror(x, b.wd), mirror(y, b.ht)); }
 
 // texture() at texture-space (s,t): taps floor(s-.5), +1, weights frac

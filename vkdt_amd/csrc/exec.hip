@@ -36,6 +36,13 @@ int check_blob(const node_call &c, size_t pc_min, size_t params_min, const char 
   return VKDT_HIP_OK;
 }
 
+int check_conn2(const node_call &c, int idx, uint32_t fmt_a, uint32_t fmt_b, uint32_t chan, uint32_t min_planes,
+                const char *what)
+{
+  if(idx < c.nconn && c.conn[idx].format == fmt_b) return check_conn(c, idx, fmt_b, chan, min_planes, what);
+  return check_conn(c, idx, fmt_a, chan, min_planes, what);
+}
+
 int check_conn(const node_call &c, int idx, uint32_t fmt, uint32_t chan, uint32_t min_planes, const char *what)
 {
   if(idx >= c.nconn) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "%s: connector %d missing", what, idx);
@@ -84,6 +91,10 @@ static const reg_entry g_registry[] = {
   { tkn("llap"),     tkn("reduce"),   launch_llap_reduce },
   { tkn("llap"),     tkn("assemble"), launch_llap_assemble },
   { tkn("llap"),     tkn("colour"),   launch_llap_colour },
+  { tkn("hilite"),   tkn("half"),     launch_hilite_half },
+  { tkn("hilite"),   tkn("reduce"),   launch_hilite_reduce },
+  { tkn("hilite"),   tkn("assemble"), launch_hilite_assemble },
+  { tkn("hilite"),   tkn("doub"),     launch_hilite_doub },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)

@@ -77,7 +77,7 @@ DEV dimg rfl(const dimg &d)
   dimg r;
   r.p = (void *)rfl((long)d.p);
   r.wd = rfl(d.wd); r.ht = rfl(d.ht); r.pitch = rfl(d.pitch); r.chan = rfl(d.chan);
-  r.pstride = rfl(d.pstride);
+  r.pstride = rfl(d.pstride); r.fmt = rfl(d.fmt);
   return r;
 }
 DEV bt_job rfl(const bt_job &j) { return bt_job{ rfl(j.img), rfl(j.tiles_x), rfl(j.start), rfl(j.level) }; }
@@ -582,10 +582,10 @@ size_t image_scratch(const dims_t &m)
   return s;
 }
 
-dimg mk(void *p, int wd, int ht, int pitch, int chan, long pstride)
+dimg mk(void *p, int wd, int ht, int pitch, int chan, long pstride, int fmt = DIMG_F16)
 {
   dimg d;
-  d.p = p; d.wd = wd; d.ht = ht; d.pitch = pitch; d.chan = chan; d.pstride = pstride;
+  d.p = p; d.wd = wd; d.ht = ht; d.pitch = pitch; d.chan = chan; d.pstride = pstride; d.fmt = fmt;
   return d;
 }
 
@@ -654,8 +654,8 @@ static void dn_common(dn_args &a, const vkdt_develop_image_t &d)
 static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m, char *&sp, int idx)
 {
   b = bt_img{};
-  b.cfa = mk(const_cast<void *>(d.cfa), d.cfa_wd, d.cfa_ht, d.cfa_pitch, 1, 0);
-  b.out = mk(d.out, m.W, m.H, m.W, 4, 0);
+  b.cfa = mk(const_cast<void *>(d.cfa), d.cfa_wd, d.cfa_ht, d.cfa_pitch, 1, 0, DIMG_UI16);
+  b.out = mk(d.out, m.W, m.H, m.W, 4, 0, DIMG_F32);
   const size_t q = align256((size_t)m.qw * m.qh * 8);
   b.qh = mk(sp, m.qw, m.qh, m.qw, 4, 0); sp += q;
   for(int l = 0; l < 4; l++) { b.qd[l] = mk(sp, m.qw, m.qh, m.qw, 4, 0); sp += q; }

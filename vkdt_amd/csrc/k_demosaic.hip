@@ -29,13 +29,13 @@ __global__ void __launch_bounds__(256) k_dm_down(dimg in, dimg out, dm_args a)
   {
     float s = 0.0f;
     // texelFetch order of the reference: (0,0) (0,1) (0,2) (1,0) ...
-    for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) s += fetch_h1(in, 3 * x + i, 3 * y + j);
+    for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) s += fetch_m(in, 3 * x + i, 3 * y + j);
     lum = s / 9.0f;
   }
   else
   {
-    const float c0 = fetch_h1(in, 2 * x, 2 * y), c1 = fetch_h1(in, 2 * x, 2 * y + 1);
-    const float c2 = fetch_h1(in, 2 * x + 1, 2 * y), c3 = fetch_h1(in, 2 * x + 1, 2 * y + 1);
+    const float c0 = fetch_m(in, 2 * x, 2 * y), c1 = fetch_m(in, 2 * x, 2 * y + 1);
+    const float c2 = fetch_m(in, 2 * x + 1, 2 * y), c3 = fetch_m(in, 2 * x + 1, 2 * y + 1);
     lum = (c0 + c1 + c2 + c3) * 0.25f;
   }
   st_h1(out, x, y, lum);
@@ -90,7 +90,7 @@ int launch_demosaic_down(const node_call &c)
 {
   dm_args a;
   if(int e = decode(a, c, "demosaic/down")) return e;
-  if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, 1, "demosaic/down input")) return e;
+  if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/down input")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, 1, "demosaic/down output")) return e;
   hipLaunchKernelGGL(k_dm_down, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
                      to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
@@ -101,7 +101,7 @@ int launch_demosaic_gauss(const node_call &c)
 {
   dm_args a;
   if(int e = decode(a, c, "demosaic/gauss")) return e;
-  if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, 1, "demosaic/gauss orig")) return e;
+  if(int e = check_conn2(c, 1, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/gauss orig")) return e;
   if(int e = check_conn(c, 2, VKDT_FMT_F16, 4, 1, "demosaic/gauss output")) return e;
   if(a.filters == 9)
     hipLaunchKernelGGL(k_dm_gauss<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
@@ -116,7 +116,7 @@ int launch_demosaic_splat(const node_call &c)
 {
   dm_args a;
   if(int e = decode(a, c, "demosaic/splat")) return e;
-  if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, 1, "demosaic/splat input")) return e;
+  if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/splat input")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "demosaic/splat gauss")) return e;
   if(int e = check_conn(c, 2, VKDT_FMT_F16, 1, 1, "demosaic/splat output")) return e;
   // the +-6 border wrap of eval_gauss stays inside images of at least 6x6
@@ -135,7 +135,7 @@ int launch_demosaic_fix(const node_call &c)
 {
   dm_args a;
   if(int e = decode(a, c, "demosaic/fix")) return e;
-  if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, 1, "demosaic/fix input")) return e;
+  if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/fix input")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, 1, "demosaic/fix green")) return e;
   if(int e = check_conn(c, 2, VKDT_FMT_F16, 4, 1, "demosaic/fix cov")) return e;
   if(int e = check_conn(c, 3, VKDT_FMT_F16, 4, 1, "demosaic/fix output")) return e;

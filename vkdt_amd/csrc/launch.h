@@ -36,12 +36,17 @@ static inline dimg to_dimg(const vkdt_buf_t &b)
   d.chan = (int)b.chan;
   const size_t esz = (b.format == VKDT_FMT_F32 ? 4 : 2) * (size_t)b.chan;
   d.pstride = (long)(b.plane_stride / (esz / b.chan));  // in scalar elements
+  d.fmt = (int)b.format;
   return d;
 }
 
 // host-side validation of one connector before any launch: the kernels
 // index within [0,wd) x [0,ht) x [0,array_length) and rely on these.
 int check_conn(const node_call &c, int idx, uint32_t fmt, uint32_t chan, uint32_t min_planes, const char *what);
+
+// the same for a connector that may hold either of two formats
+int check_conn2(const node_call &c, int idx, uint32_t fmt_a, uint32_t fmt_b, uint32_t chan, uint32_t min_planes,
+                const char *what);
 
 // pc/params blob size checks
 int check_blob(const node_call &c, size_t pc_min, size_t params_min, const char *what);
@@ -73,3 +78,7 @@ int launch_llap_curve(const node_call &);
 int launch_llap_reduce(const node_call &);
 int launch_llap_assemble(const node_call &);
 int launch_llap_colour(const node_call &);
+int launch_hilite_half(const node_call &);
+int launch_hilite_reduce(const node_call &);
+int launch_hilite_assemble(const node_call &);
+int launch_hilite_doub(const node_call &);

@@ -439,7 +439,7 @@ DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
     for(int i = n0; i < n1; i++)
     {
       if(XT ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
-      px[n] = centre_h1(orig, bs * x + i, bs * y + j);
+      px[n] = centre_m(orig, bs * x + i, bs * y + j);
       rp[n] = 1.0f / px[n];
       pi[n] = i; pj[n] = j;
       n++;
@@ -562,7 +562,7 @@ DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
 template<int XT>
 DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
 {
-  return dm_splat_at<XT>([&](int px, int py) { return fetch_h1(in, px, py); }, in.wd, in.ht, ga, x, y);
+  return dm_splat_at<XT>([&](int px, int py) { return fetch_m(in, px, py); }, in.wd, in.ht, ga, x, y);
 }
 
 // fix.comp:21-135: rgb at (x,y) from the mosaic, the green plane (through
@@ -654,7 +654,7 @@ DEV float4 dm_fix_r(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt gree
 template<int XT, typename GreenAt>
 DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
 {
-  return dm_fix_r<XT>([&](int px, int py) { return centre_h1(in, px, py); }, cv, r, x, y, green);
+  return dm_fix_r<XT>([&](int px, int py) { return centre_m(in, px, py); }, cv, r, x, y, green);
 }
 
 // ==========================================================================

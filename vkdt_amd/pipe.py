@@ -205,13 +205,18 @@ class Graph:
 
 
 def raw_graph_cfg(dng, strength=0.5, noise_a=0.0, noise_b=0.0, clarity=0.2, numgamma=10, levels=12,
-                  extra=()):
-    """cfg lines of the raw-development graph i-raw -> ... -> llap -> display."""
+                  extra=(), hilite=False):
+    """cfg lines of the raw-development graph i-raw -> denoise -> [hilite ->]
+    demosaic -> colour -> filmcurv -> llap -> display (with hilite: the
+    chain of bin/default-darkroom.i-raw up to llap)."""
+    mid = (["connect:denoise:01:output:hilite:01:input", "connect:hilite:01:output:demosaic:01:input"]
+           if hilite else ["connect:denoise:01:output:demosaic:01:input"])
     lines = [
         "module:i-raw:main", "module:denoise:01", "module:demosaic:01", "module:colour:01",
         "module:filmcurv:01", "module:llap:01", "module:display:main",
+    ] + (["module:hilite:01"] if hilite else []) + [
         "connect:i-raw:main:output:denoise:01:input",
-        "connect:denoise:01:output:demosaic:01:input",
+    ] + mid + [
         "connect:demosaic:01:output:colour:01:input",
         "connect:colour:01:output:filmcurv:01:input",
         "connect:filmcurv:01:output:llap:01:input",
