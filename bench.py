@@ -101,9 +101,10 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
-def plain_graph(dng, kind, noise_a, noise_b, out_dir, extra=()):
+def plain_graph(dng, kind, noise_a, noise_b, out_dir, extra=(), hilite=False):
     from vkdt_amd import pipe
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=noise_a, noise_b=noise_b, extra=list(extra)))
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=noise_a, noise_b=noise_b, extra=list(extra),
+                                            hilite=hilite))
     g.export("o-pfm", os.path.join(out_dir, "out"))
     return g
 
@@ -121,7 +122,9 @@ def run_plain(args):
     dng = os.path.join(tmp, "still.dng")
     synth.write_dng(dng, cfa, args.kind)
     extra = ["param:llap:01:numgamma:8", "param:llap:01:levels:6"] if args.kind == "xtrans" else []
-    g = plain_graph(dng, args.kind, na, nb, tmp, extra)
+    g = plain_graph(dng, args.kind, na, nb, tmp, extra, hilite=args.hilite)
+    if args.hilite:
+        workload += ", hilite between denoise and demosaic (the stock darkroom chain)"
     g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)        # plan, allocate, upload, first pass
     ip = g.img_param("i-raw", "main")
     P = ip["full"][0] * ip["full"][1]
@@ -178,7 +181,8 @@ def run_plain(args):
         "data": "synthetic (seeded scene + Poisson-Gaussian noise, uncompressed DNG)",
         "config": {"workload": workload, "width": W, "height": H, "cfa_pixels": P,
                    "path": "plain per-image: one HIP launch per node", "storage": "f16 intermediates",
-                   "graph": "i-raw>denoise(strength .5)>demosaic>colour>filmcurv>llap(clarity .2)>o-pfm"},
+                   "graph": "i-raw>denoise(strength .5)>" + ("hilite>" if args.hilite else "")
+                   + "demosaic>colour>filmcurv>llap(clarity .2)>o-pfm"},
         "roofline": roof, "graph_roofline": graph_roof, "cpu_baseline": cpu,
     }
 
@@ -472,6 +476,8 @@ def main():
                     help="batch: 64 mixed stills in one batched hipGraph vs the plain path; "
                          "plain: the per-image path on one still")
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
+    ap.add_argument("--hilite", action="store_true",
+                    help="plain mode: insert hilite between denoise and demosaic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batch mode: skip timing the plain path beside it")
     ap.add_argument("--serve-seconds", type=float, default=10.0,

@@ -19,11 +19,34 @@ __global__ void __launch_bounds__(256) k_hl_half(dimg in, dimg out, float white)
   st_h4(out, x, y, r);
 }
 
+// reduce: a work group covers BX x BY coarse texels.  Their fine
+// footprint (2BX+3) x (2BY+3) is loaded once (mirrored addresses) and each
+// fine texel prepared once in LDS -- the reference evaluates the
+// desaturation of every fine texel for each of the up to 9 coarse texels
+// whose 5x5 window covers it.
+#define HLR_W (2 * BX + 3)
+#define HLR_H (2 * BY + 3)
 __global__ void __launch_bounds__(256) k_hl_reduce(dimg in, dimg out, hl_args a)
 {
-  const int x = tid_x(), y = tid_y();
+  __shared__ float s_r[HLR_H * HLR_W], s_g[HLR_H * HLR_W], s_b[HLR_H * HLR_W];
+  __shared__ float s_l[HLR_H * HLR_W], s_v[HLR_H * HLR_W];
+  const int2 blk = xcd_block();
+  const int X0 = blk.x * BX, Y0 = blk.y * BY;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for(int i = tid; i < HLR_W * HLR_H; i += BX * BY)
+  {
+    const int lx = i % HLR_W, ly = i / HLR_W;
+    const hl_tap t = hl_prep(centre_h4(in, 2 * X0 - 2 + lx, 2 * Y0 - 2 + ly), a);
+    s_r[i] = t.r; s_g[i] = t.g; s_b[i] = t.b; s_l[i] = t.l; s_v[i] = t.valid;
+  }
+  __syncthreads();
+  const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, hl_reduce_px([&](int px, int py) { return centre_h4(in, px, py); }, x, y, a));
+  const int base = 2 * threadIdx.y * HLR_W + 2 * threadIdx.x;
+  st_h4(out, x, y, hl_reduce_sum([&](int ii, int jj) {
+    const int k = base + jj * HLR_W + ii;
+    return hl_tap{ s_r[k], s_g[k], s_b[k], s_l[k], s_v[k] };
+  }));
 }
 
 __global__ void __launch_bounds__(256) k_hl_assemble(dimg fine, dimg coarse, dimg out, hl_args a)

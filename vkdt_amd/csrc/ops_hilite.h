@@ -75,10 +75,41 @@ DEV float4 hl_half_px(MosAt at, int x, int y, float white)
   return r;
 }
 
-// reduce at coarse texel (x,y); Tap(i,j) returns the fine level's rgba at
-// the mirrored address of (i,j)
+// per fine texel of a reduce: its luminance (edge term) and, when no
+// channel reaches white, the desaturated colour that enters the blur;
+// valid = 0 marks a clipped texel, whose colour is then 0 (reduce.comp:37-54)
+struct hl_tap
+{
+  float r, g, b, l, valid;
+};
+
+DEV hl_tap hl_prep(float4 t, const hl_args &a)
+{
+  hl_tap o;
+  float r = t.x, g = t.y, b = t.z;
+  o.l = lum2020(r, g, b);
+  const float rw = r * a.wb[0], gw = g * a.wb[1], bw = b * a.wb[2];
+  const float cmax = fmaxf(rw, fmaxf(gw, bw)), cmin = fminf(rw, fminf(gw, bw));
+  const float sat = (cmax - cmin) / fmaxf(1e-3f, cmax);
+  if(r < a.white_pos && g < a.white_pos && b < a.white_pos)
+  {
+    const float s = smoothstepf(0.2f, 1.0f, fmaxf(r, fmaxf(g, b)) / a.white_pos);
+    float tt = smoothstepf(0.15f, 0.9f, sat);
+    tt = clampf(5.0f * a.ds * a.ds * s * tt, 0.0f, 1.0f);
+    const float m = cmax + cmin;
+    o.r = mixf(r, m / a.wb[0] * 0.5f, tt);
+    o.g = mixf(g, m / a.wb[1] * 0.5f, tt);
+    o.b = mixf(b, m / a.wb[2] * 0.5f, tt);
+    o.valid = 1.0f;
+  }
+  else o.r = o.g = o.b = o.valid = 0.0f;
+  return o;
+}
+
+// the 5x5 sums of a reduce over prepared taps, jj outer and ii inner as the
+// reference loops; Tap(ii,jj) returns the hl_tap at fine offset (ii-2,jj-2)
 template<class Tap>
-DEV float4 hl_reduce_px(Tap tap, int x, int y, const hl_args &a)
+DEV float4 hl_reduce_sum(Tap tap)
 {
   float ex = 0.0f, ey = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, wgt = 0.0f;
 #pragma unroll
@@ -86,27 +117,15 @@ DEV float4 hl_reduce_px(Tap tap, int x, int y, const hl_args &a)
 #pragma unroll
     for(int ii = 0; ii < 5; ii++)
     {
-      const float4 t = tap(2 * x + ii - 2, 2 * y + jj - 2);
-      float r = t.x, g = t.y, b = t.z;
-      const float l = lum2020(r, g, b);
-      ex += (hl_w(jj) * hl_sw(ii)) * l;
-      ey += (hl_w(ii) * hl_sw(jj)) * l;
-      const float u = hl_w(ii) * hl_w(jj);
-      const float rw = r * a.wb[0], gw = g * a.wb[1], bw = b * a.wb[2];
-      const float cmax = fmaxf(rw, fmaxf(gw, bw)), cmin = fminf(rw, fminf(gw, bw));
-      const float sat = (cmax - cmin) / fmaxf(1e-3f, cmax);
-      if(r < a.white_pos && g < a.white_pos && b < a.white_pos)
+      const hl_tap t = tap(ii, jj);
+      ex += (hl_w(jj) * hl_sw(ii)) * t.l;
+      ey += (hl_w(ii) * hl_sw(jj)) * t.l;
+      if(t.valid != 0.0f)
       {
-        const float s = smoothstepf(0.2f, 1.0f, fmaxf(r, fmaxf(g, b)) / a.white_pos);
-        float tt = smoothstepf(0.15f, 0.9f, sat);
-        tt = clampf(5.0f * a.ds * a.ds * s * tt, 0.0f, 1.0f);
-        const float m = cmax + cmin;
-        r = mixf(r, m / a.wb[0] * 0.5f, tt);
-        g = mixf(g, m / a.wb[1] * 0.5f, tt);
-        b = mixf(b, m / a.wb[2] * 0.5f, tt);
-        cr += r * u;
-        cg += g * u;
-        cb += b * u;
+        const float u = hl_w(ii) * hl_w(jj);
+        cr += t.r * u;
+        cg += t.g * u;
+        cb += t.b * u;
         wgt += u;
       }
     }
