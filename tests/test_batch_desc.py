@@ -91,6 +91,19 @@ def test_refuses_graphs_the_batch_does_not_run(tmp_path):
     g.close()
 
 
+def test_graph_with_hilite_is_left_to_the_per_node_path(tmp_path):
+    """hilite between denoise and demosaic is outside the batch's fused chain"""
+    cfa, _ = synth.mosaic(128, 96, 7, "bayer")
+    dng = str(tmp_path / "h.dng")
+    synth.write_dng(dng, cfa, "bayer")
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, hilite=True))
+    g.export("o-pfm", str(tmp_path / "out3"))
+    g.plan()
+    with pytest.raises(pipe.PipeError, match="chain"):
+        batch.describe(g)
+    g.close()
+
+
 def test_create_rejects_bad_input_without_touching_the_device(tmp_path):
     from vkdt_amd import hip
     g, _ = planned(tmp_path, "bayer", 128, 96)
