@@ -122,7 +122,12 @@ def run_plain(args):
     dng = os.path.join(tmp, "still.dng")
     synth.write_dng(dng, cfa, args.kind)
     extra = ["param:llap:01:numgamma:8", "param:llap:01:levels:6"] if args.kind == "xtrans" else []
-    g = plain_graph(dng, args.kind, na, nb, tmp, extra, hilite=args.hilite)
+    if args.noise_profile:
+        # i-raw -> rawhist -> nprof: the FP64 noise statistics of the frame
+        g = pipe.Graph(lines=pipe.noise_profile_cfg(dng))
+        workload = workload.replace("plain per-image path", "noise profile (rawhist collect, FP64 moments)")
+    else:
+        g = plain_graph(dng, args.kind, na, nb, tmp, extra, hilite=args.hilite)
     if args.hilite:
         workload += ", hilite between denoise and demosaic (the stock darkroom chain)"
     g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)        # plan, allocate, upload, first pass
@@ -142,7 +147,7 @@ def run_plain(args):
         g.run(run_flags)                             # syncs the stream (per-node events)
         for k, ms in zip(order, g.exec_ms()):
             n = nodes[k]
-            if n["module"] in ("i-raw", "o-pfm"):
+            if n["module"] in ("i-raw", "o-pfm", "nprof"):
                 continue
             key = n["module"] + "/" + n["kernel"]
             e = per.setdefault(key, {"ms": 0.0, "bytes": 0, "launches": 0})
@@ -155,7 +160,7 @@ def run_plain(args):
     ms_step = wall * 1e3 / args.steps
     gpu_ms = sum(e["ms"] for e in per.values()) / args.steps
     value = P / (ms_step * 1e-3) / 1e6
-    graph_bytes = sum(node_bytes(nodes[k]) for k in order if nodes[k]["module"] not in ("i-raw", "o-pfm"))
+    graph_bytes = sum(node_bytes(nodes[k]) for k in order if nodes[k]["module"] not in ("i-raw", "o-pfm", "nprof"))
 
     dom = max(per, key=lambda k: per[k]["ms"])
     e = per[dom]
@@ -181,7 +186,8 @@ def run_plain(args):
         "data": "synthetic (seeded scene + Poisson-Gaussian noise, uncompressed DNG)",
         "config": {"workload": workload, "width": W, "height": H, "cfa_pixels": P,
                    "path": "plain per-image: one HIP launch per node", "storage": "f16 intermediates",
-                   "graph": "i-raw>denoise(strength .5)>" + ("hilite>" if args.hilite else "")
+                   "graph": "i-raw>rawhist>nprof" if args.noise_profile else
+                   "i-raw>denoise(strength .5)>" + ("hilite>" if args.hilite else "")
                    + "demosaic>colour>filmcurv>llap(clarity .2)>o-pfm"},
         "roofline": roof, "graph_roofline": graph_roof, "cpu_baseline": cpu,
     }
@@ -478,6 +484,8 @@ def main():
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
     ap.add_argument("--hilite", action="store_true",
                     help="plain mode: insert hilite between denoise and demosaic")
+    ap.add_argument("--noise-profile", action="store_true",
+                    help="plain mode: time the noise-profile graph i-raw > rawhist > nprof instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batch mode: skip timing the plain path beside it")
     ap.add_argument("--serve-seconds", type=float, default=10.0,

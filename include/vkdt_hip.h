@@ -35,12 +35,15 @@ enum
 
 // buffer formats, as the connector "format" tokens of the reference
 // (graph-run-nodes-allocate.h:39-96): ui16 = R16_UNORM, f16 = R16F/RGBA16F,
-// f32 = R32F/RGBA32F.
+// f32 = R32F/RGBA32F.  atom: the accumulators that kernels add into with
+// atomics (the reference's R32F float atomics, :53), held here as FP64 --
+// 8 bytes per channel -- so per-frame sums do not lose precision.
 enum
 {
   VKDT_FMT_UI16 = 1,
   VKDT_FMT_F16  = 2,
   VKDT_FMT_F32  = 3,
+  VKDT_FMT_ATOM = 4,
 };
 
 // one connector's device buffer.  Replaces the descriptor-set image
@@ -103,6 +106,11 @@ int vkdt_hip_free(void *ptr);
 // graph-run-nodes-download.h:3-32).
 int vkdt_hip_host_alloc(void **ptr, size_t bytes);
 int vkdt_hip_host_free(void *ptr);
+
+// Zero every plane of `buf` on `stream` (a fill kernel, so the call may be
+// captured into a hipGraph).  Takes over the clear of connectors flagged
+// s_conn_clear before their node runs (graph-run-nodes-record-cmd.h:100-175).
+int vkdt_hip_buf_clear(const vkdt_buf_t *buf, void *stream);
 
 // Asynchronous copies on `stream`.  Take over the staging<->image copies
 // recorded in graph-run-nodes-record-cmd.h:255-320.

@@ -27,6 +27,7 @@
 typedef float real;
 #define EXP   expf
 #define EXP2  exp2f
+#define LOG2  log2f
 #define POW   powf
 #define SQRT  sqrtf
 #define FABS  fabsf
@@ -36,6 +37,7 @@ typedef float real;
 typedef double real;
 #define EXP   exp
 #define EXP2  exp2
+#define LOG2  log2
 #define POW   pow
 #define SQRT  sqrt
 #define FABS  fabs
@@ -1275,6 +1277,144 @@ static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int 
 }
 
 /* ------------------------------------------------------------------------
+ * rawhist (src/pipe/modules/rawhist) and nprof (src/pipe/modules/nprof):
+ * noise statistics of the raw mosaic and the noise-model fit.
+ * params (committed, rawhist/main.c:19-29): c, dx (float), filters, black,
+ * white (uint32, raw units).  The accumulators are a double buffer of
+ * bins x rows (rows 0..2: count, sum of block means, sum of block second
+ * moments), added into in invocation order.
+ * --------------------------------------------------------------------- */
+
+static inline real rh_norm(real v, real black, real white)
+{
+  return FMIN(FMAX((v - black) / (white - black), R(0)), R(1));
+}
+
+// rawhist/collect.comp:19-72 over every invocation of the ceil(wd/8) x
+// ceil(ht/8) groups of 8x8 (block index (x,y), dispatch extent wd x ht)
+static int rh_collect(const float *par, const or_buf_t *c, int wd, int ht)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const uint32_t *pu = (const uint32_t *)(par + 2);
+  const uint32_t filters = pu[0];
+  const real black = (real)(float)pu[1] / R(65536.0), white = (real)(float)pu[2] / R(65536.0);
+  const int bins = out->wd;
+  double *h = (double *)out->data;
+  const int nx = (wd + 7) / 8 * 8, ny = (ht + 7) / 8 * 8;
+  for(int y = 0; y < ny; y++) for(int x = 0; x < nx; x++)
+  {
+    real mean = 0, mom2 = 0, px = 0;
+    if(filters == 9u)
+    { // greens at the corners and the centre of the 3x3 block
+      const int ix = 3 * x, iy = 3 * y;
+      if(ix >= in->wd || iy >= in->ht) continue;
+      const real v0 = rh_norm(fetch(in, ix, iy, 0).x, black, white);
+      const real v1 = rh_norm(fetch(in, ix + 2, iy, 0).x, black, white);
+      const real v2 = rh_norm(fetch(in, ix + 1, iy + 1, 0).x, black, white);
+      const real v3 = rh_norm(fetch(in, ix, iy + 2, 0).x, black, white);
+      const real v4 = rh_norm(fetch(in, ix + 2, iy + 2, 0).x, black, white);
+      mean = (v0 + v1 + v2 + v3 + v4) / R(5.0);
+      mom2 = (v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3 + v4 * v4) / R(5.0);
+      px = (real)bins * LOG2(mean + R(1.0));
+    }
+    else if(filters != 0u)
+    { // greens at (1,0) and (0,1) of the RGGB block
+      const int ix = 2 * x, iy = 2 * y;
+      if(ix >= in->wd || iy >= in->ht) continue;
+      const real v0 = rh_norm(fetch(in, ix + 1, iy, 0).x, black, white);
+      const real v1 = rh_norm(fetch(in, ix, iy + 1, 0).x, black, white);
+      mean = (v0 + v1) / R(2.0);
+      mom2 = (v0 * v0 + v1 * v1) / R(2.0);
+      px = (real)bins * LOG2(mean + R(1.0));
+    }
+    const int b = (int)px;
+    if(b < 0 || b >= bins) continue;   // imageAtomicAdd outside the image
+    h[b] += 1.0;
+    h[bins + b] += (double)mean;
+    h[2 * bins + b] += (double)mom2;
+  }
+  return 0;
+}
+
+// rawhist/map.comp:20-52: the plot of count, variance and the noise model
+static int rh_map(const float *par, const or_buf_t *c)
+{
+  const or_buf_t *hist = c + 0, *out = c + 1;
+  const double *h = (const double *)hist->data;
+  const int bins = hist->wd;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const real cnt = x < bins ? (real)(float)h[x] : 0, mean = x < bins ? (real)(float)h[bins + x] : 0;
+    const real mom2 = x < bins ? (real)(float)h[2 * bins + x] : 0;
+    const real var = mom2 / cnt - mean / cnt * mean / cnt;
+    const real yy = (real)(out->ht - y - 1) / (real)out->ht;
+    const real lin = mean / cnt;
+    const real fit = R(par[0]) + R(par[1]) * lin;
+    v4 o = { 0, 0, 0, R(1.0) };
+    if(FABS(yy - SQRT(var) * R(20.0)) < R(0.015))
+    { o.x = R(0.5) * (o.x + R(1.0)); o.y = R(0.5) * (o.y + R(0.2)); o.z = R(0.5) * (o.z + R(0.2)); }
+    if(yy < LOG2(cnt) * R(0.05)) { o.x += R(0.5); o.y += R(0.5); o.z += R(0.5); }
+    const real df = FABS(yy - SQRT(fit) * R(20.0));
+    if(df < R(0.005))
+    {
+      const real s = smoothstepr(R(0.0), R(1.0), R(1.0) - df / R(0.005));
+      o.x += s; o.y += s; o.z += s;
+    }
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// nprof/main.c:84-140: screen bins by pairs, then a count-weighted line
+int or_nprof_fit(const double *h, int wd, float *a_out, float *b_out)
+{
+  int *ok = (int *)malloc(sizeof(int) * (wd > 0 ? wd : 1));
+  int nok = 0;
+  float a = 0.0f, b = 0.0f;
+  for(int f = 0; f < 7; f++)
+  {
+    const double fk = 1.0 / pow(2.0, f);
+    for(int i = 0; i < wd; i++)
+    {
+      int score = 0;
+      for(int j = i + 1; j < wd * (0.8 - fk); j++)
+      {
+        if(h[i] < fk * 64 || h[j] < fk * 64) continue;
+        const double xi = h[wd + i] / h[i], yi = h[2 * wd + i] / h[i] - xi * xi;
+        const double xj = h[wd + j] / h[j], yj = h[2 * wd + j] / h[j] - xj * xj;
+        if(yi <= 0 || yj <= 0) continue;
+        const double slope = (yj - yi) / (xj - xi), icpt = yi - xi * slope;
+        if(!(slope > 0.0) || !(icpt > 0.0)) continue;
+        if(++score > fk * 32) { a = (float)icpt; b = (float)slope; ok[nok++] = i; break; }
+      }
+    }
+    if(nok) break;
+  }
+  double n = 0, sx = 0, sxx = 0, sy = 0, sxy = 0;
+  for(int k = 0; k < nok; k++)
+  {
+    const int i = ok[k];
+    const double cnt = h[i], m1 = h[wd + i], m2 = h[2 * wd + i];
+    const double x = m1 / cnt;
+    double y = m2 / cnt - m1 / cnt * m1 / cnt;
+    if(0.0 > y) y = 0.0;
+    n += cnt; sx += x * cnt; sxx += x * x * cnt; sy += y * cnt; sxy += x * y * cnt;
+  }
+  const float den = n * sxx - sx * sx;
+  if(fabs(den) > 1e-10f)
+  {
+    const double nb = (n * sxy - sx * sy) / den, na = (sy * sxx - sx * sxy) / den;
+    b = 0.0 > nb ? 0.0f : (float)nb;
+    a = 0.0 > na ? 0.0f : (float)na;
+  }
+  if(nok <= 0) { a = 100.0f; b = 1.0f; }
+  free(ok);
+  *a_out = a;
+  *b_out = b;
+  return nok;
+}
+
+/* ------------------------------------------------------------------------
  * node dispatch
  * --------------------------------------------------------------------- */
 
@@ -1307,6 +1447,8 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("hilite", "reduce"))    return nconn >= 2 ? hl_reduce(pcf, par, conn) : -1;
   if(IS("hilite", "assemble"))  return nconn >= 3 ? hl_assemble(par, conn) : -1;
   if(IS("hilite", "doub"))      return nconn >= 3 ? hl_doub(pci, par, conn, wd, ht) : -1;
+  if(IS("rawhist", "collect"))  return nconn >= 2 ? rh_collect(par, conn, wd, ht) : -1;
+  if(IS("rawhist", "map"))      return nconn >= 2 ? rh_map(par, conn) : -1;
 #undef IS
   return -1;
 }

@@ -60,6 +60,11 @@ void or_gather(const uint16_t *img, int wd, int ht, double u, double v, double o
 // 2x2 symmetric eigen decomposition as the reference computes it
 void or_evd2x2(const double M[4], double eval[2], double evec0[2], double evec1[2]);
 
+// nprof's noise-model fit (nprof/main.c:84-140) on rawhist accumulators h
+// (wd bins x 3 rows: count, sum of means, sum of second moments); returns
+// the number of bins that entered the line, a = intercept, b = slope
+int or_nprof_fit(const double *h, int wd, float *a, float *b);
+
 // whole-graph restatement of i-raw -> denoise -> demosaic -> colour ->
 // filmcurv -> llap for one uncompressed CFA image (already block rounded,
 // CFA offset normalised).  parameters follow the modules' params files.

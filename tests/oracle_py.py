@@ -56,6 +56,7 @@ def lib(bits=64):
         L.or_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.or_evd2x2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_set_storage_f16.argtypes = [C.c_int]
+        L.or_nprof_fit.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         _libs[bits] = L
     return _libs[bits]
 
@@ -108,3 +109,26 @@ def develop(bits, cfa, p, stages=False, storage_f16=True):
     if rc != 0:
         raise RuntimeError(f"oracle develop failed: {rc}")
     return (out, st) if stages else out
+
+
+def rawhist_collect(bits, raw, filters, black, white, bins=1000, rows=5, noise=(0.0, 0.0), node_wd=None,
+                    node_ht=None):
+    """rawhist/collect on the oracle: accumulators rows x bins (float64)"""
+    H, W = raw.shape
+    blk = 3 if filters == 9 else 2
+    raw = np.ascontiguousarray(raw, np.uint16)
+    acc = np.zeros((rows, bins), np.float64)
+    par = np.zeros(5, np.uint32)
+    par[0:2] = np.array(noise, np.float32).view(np.uint32)
+    par[2:5] = [filters, black, white]
+    node(bits, "rawhist", "collect", None, par, [orbuf(raw, OR_UI16, 1), orbuf(acc, OR_F32, 1)],
+         W // blk if node_wd is None else node_wd, H // blk if node_ht is None else node_ht)
+    return acc
+
+
+def nprof_fit(bits, acc):
+    """nprof's fit on accumulators (rows x bins): (a, b, bins kept)"""
+    acc = np.ascontiguousarray(acc, np.float64)
+    a, b = C.c_float(), C.c_float()
+    n = lib(bits).or_nprof_fit(acc.ctypes.data, acc.shape[1], C.byref(a), C.byref(b))
+    return a.value, b.value, n

@@ -134,3 +134,22 @@ def test_hilite_formats_follow_the_module_connectors(tmp_path):
     red = [n for n in nodes if n["module"] == "hilite" and n["kernel"] == "reduce"]
     # block grid 48x32: levels 24x16 ... until the next would be 1 texel
     assert [(n["wd"], n["ht"]) for n in red] == [(24, 16), (12, 8), (6, 4), (3, 2)]
+
+
+def test_noise_profile_graph_plan(tmp_path):
+    """i-raw -> rawhist -> nprof (bin/presets/noise-profile.pst): collect runs on
+    the block grid into a cleared 1000x5 accumulator; the plot feeds nothing"""
+    cfa, _ = synth.mosaic(200, 150, 1, "bayer")
+    path = str(tmp_path / "n.dng")
+    synth.write_dng(path, cfa, "bayer")
+    g = pipe.Graph(lines=pipe.noise_profile_cfg(path))
+    g.plan()
+    nodes = g.nodes()
+    col = [n for n in nodes if n["module"] == "rawhist" and n["kernel"] == "collect"][0]
+    assert (col["wd"], col["ht"]) == (100, 75) and not col["culled"]
+    assert col["conns"][1]["format"] == "atom"
+    assert [n["culled"] for n in nodes if n["kernel"] == "map"] == [True]
+    g.commit()
+    col = [n for n in g.nodes() if n["module"] == "rawhist" and n["kernel"] == "collect"][0]
+    par = np.frombuffer(col["params"], np.uint32)
+    assert par[2] == synth.BAYER_FILTERS and par[3] == 512 and par[4] == 16383

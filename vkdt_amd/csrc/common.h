@@ -22,11 +22,12 @@ struct dimg
   int      pitch;
   int      chan;
   long     pstride;
-  int      fmt;       // 1 ui16 (UNORM), 2 f16, 3 f32 (the VKDT_FMT_* codes)
+  int      fmt;       // 1 ui16 (UNORM), 2 f16, 3 f32, 4 FP64 accumulators (the VKDT_FMT_* codes)
 };
 #define DIMG_UI16 1
 #define DIMG_F16  2
 #define DIMG_F32  3
+#define DIMG_ATOM 4
 
 DEV float clampf(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
 DEV float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }

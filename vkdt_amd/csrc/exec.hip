@@ -55,7 +55,7 @@ int check_conn(const node_call &c, int idx, uint32_t fmt, uint32_t chan, uint32_
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "%s: connector %d extent %ux%u pitch %u", what, idx, b.wd, b.ht, b.pitch);
   if(b.array_length < (min_planes ? min_planes : 1))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "%s: connector %d has %u planes, need %u", what, idx, b.array_length, min_planes);
-  const size_t esz = (fmt == VKDT_FMT_F32 ? 4 : 2) * (size_t)chan;
+  const size_t esz = vkdt_fmt_bytes(fmt) * (size_t)chan;
   if(b.array_length > 1 && b.plane_stride < (size_t)b.pitch * b.ht * esz)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "%s: connector %d plane stride too small", what, idx);
   if(b.array_length > 1 && b.plane_stride % esz)
@@ -95,6 +95,8 @@ static const reg_entry g_registry[] = {
   { tkn("hilite"),   tkn("reduce"),   launch_hilite_reduce },
   { tkn("hilite"),   tkn("assemble"), launch_hilite_assemble },
   { tkn("hilite"),   tkn("doub"),     launch_hilite_doub },
+  { tkn("rawhist"),  tkn("collect"),  launch_rawhist_collect },
+  { tkn("rawhist"),  tkn("map"),      launch_rawhist_map },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)
@@ -151,6 +153,29 @@ int vkdt_hip_upload(void *dst_dev, const void *src_host, size_t bytes, void *str
 int vkdt_hip_download(void *dst_host, const void *src_dev, size_t bytes, void *stream)
 {
   return hip_fail(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "download");
+}
+
+// zero fill: 16-byte stores over the aligned body, 2-byte stores for the rest
+__global__ void __launch_bounds__(256) k_zero(uint4 *body, size_t n16, uint16_t *tail, size_t n2)
+{
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) body[i] = make_uint4(0, 0, 0, 0);
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) tail[i] = 0;
+}
+
+int vkdt_hip_buf_clear(const vkdt_buf_t *buf, void *stream)
+{
+  if(!buf || !buf->ptr) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "clear: no buffer");
+  const size_t bytes = buf->plane_stride * (buf->array_length ? buf->array_length : 1);
+  if(bytes % 2) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "clear: odd byte count %zu", bytes);
+  const int aligned = ((uintptr_t)buf->ptr % 16) == 0;
+  const size_t n16 = aligned ? bytes / 16 : 0;
+  uint16_t *tail = (uint16_t *)((uint8_t *)buf->ptr + n16 * 16);
+  const size_t n2 = (bytes - n16 * 16) / 2;
+  const size_t n = n16 > n2 ? n16 : n2;
+  const unsigned blocks = (unsigned)(n / 256 + 1 < 4096 ? n / 256 + 1 : 4096);
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4 *)buf->ptr, n16, tail, n2);
+  return launch_post("clear");
 }
 
 int vkdt_hip_stream_create(void **stream)

@@ -25,6 +25,12 @@ typedef int (*launch_fn)(const node_call &);
 // records the error text for vkdt_hip_last_error() and returns `code`
 int vkdt_set_error(int code, const char *fmt, ...);
 
+// bytes per channel of a buffer format
+static inline size_t vkdt_fmt_bytes(uint32_t fmt)
+{
+  return fmt == VKDT_FMT_ATOM ? 8 : (fmt == VKDT_FMT_F32 ? 4 : 2);
+}
+
 // element-pitch device view of a connector
 static inline dimg to_dimg(const vkdt_buf_t &b)
 {
@@ -34,7 +40,7 @@ static inline dimg to_dimg(const vkdt_buf_t &b)
   d.ht = (int)b.ht;
   d.pitch = (int)b.pitch;
   d.chan = (int)b.chan;
-  const size_t esz = (b.format == VKDT_FMT_F32 ? 4 : 2) * (size_t)b.chan;
+  const size_t esz = vkdt_fmt_bytes(b.format) * (size_t)b.chan;
   d.pstride = (long)(b.plane_stride / (esz / b.chan));  // in scalar elements
   d.fmt = (int)b.format;
   return d;
@@ -82,3 +88,5 @@ int launch_hilite_half(const node_call &);
 int launch_hilite_reduce(const node_call &);
 int launch_hilite_assemble(const node_call &);
 int launch_hilite_doub(const node_call &);
+int launch_rawhist_collect(const node_call &);
+int launch_rawhist_map(const node_call &);

@@ -12,7 +12,7 @@ import numpy as np
 from . import lib_path
 
 OK, INCOMPLETE, OOM, DEVICE = 0, -1, -2, -3
-FMT_UI16, FMT_F16, FMT_F32 = 1, 2, 3
+FMT_UI16, FMT_F16, FMT_F32, FMT_ATOM = 1, 2, 3, 4   # atom: FP64 accumulators
 
 
 class Buf(C.Structure):
@@ -57,6 +57,7 @@ def lib():
         L.vkdt_hip_host_free.argtypes = [vp]
         L.vkdt_hip_upload.argtypes = [vp, vp, sz, vp]
         L.vkdt_hip_download.argtypes = [vp, vp, sz, vp]
+        L.vkdt_hip_buf_clear.argtypes = [C.POINTER(Buf), vp]
         L.vkdt_hip_stream_create.argtypes = [C.POINTER(vp)]
         L.vkdt_hip_stream_destroy.argtypes = [vp]
         L.vkdt_hip_stream_sync.argtypes = [vp]
@@ -86,7 +87,7 @@ class DeviceBuffer:
 
     def __init__(self, wd, ht, chan, fmt, planes=1):
         self.wd, self.ht, self.chan, self.fmt, self.planes = wd, ht, chan, fmt, planes
-        self.esize = (4 if fmt == FMT_F32 else 2) * chan
+        self.esize = {FMT_ATOM: 8, FMT_F32: 4}.get(fmt, 2) * chan
         self.plane_bytes = wd * ht * self.esize
         self.nbytes = self.plane_bytes * planes
         p = C.c_void_p()
@@ -97,11 +98,16 @@ class DeviceBuffer:
         return Buf(self.ptr, self.wd, self.ht, self.wd, self.chan, self.fmt, self.planes, self.plane_bytes)
 
     def dtype(self):
-        return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32}[self.fmt]
+        return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32, FMT_ATOM: np.float64}[self.fmt]
 
     def shape(self):
         s = (self.planes, self.ht, self.wd) if self.planes > 1 else (self.ht, self.wd)
         return s + ((self.chan,) if self.chan > 1 else ())
+
+    def clear(self, stream=None):
+        b = self.buf()
+        check(lib().vkdt_hip_buf_clear(C.byref(b), stream), "clear")
+        sync(stream)
 
     def upload(self, arr, stream=None):
         a = np.ascontiguousarray(arr, dtype=self.dtype())

@@ -204,6 +204,16 @@ class Graph:
             pass
 
 
+def noise_profile_cfg(dng):
+    """cfg lines of the noise-profiling graph i-raw -> rawhist -> nprof
+    (bin/presets/noise-profile.pst without its display): nprof writes
+    <maker>-<model>-<iso>.nprof into the working directory"""
+    return ["module:i-raw:main", "module:rawhist:np", "module:nprof:np",
+            "connect:i-raw:main:output:rawhist:np:input",
+            "connect:rawhist:np:data:nprof:np:input",
+            f"param:i-raw:main:filename:{dng}"]
+
+
 def raw_graph_cfg(dng, strength=0.5, noise_a=0.0, noise_b=0.0, clarity=0.2, numgamma=10, levels=12,
                   extra=(), hilite=False):
     """cfg lines of the raw-development graph i-raw -> denoise -> [hilite ->]

@@ -79,8 +79,7 @@ int dt_graph_node_conn(const dt_graph_t *g, int n, int c, uint64_t tok[4], int l
   buf->ht = own->roi.ht;
   buf->pitch = own->roi.wd;
   buf->chan = dt_connector_channels(own);
-  buf->format = own->format == dt_token("ui16") ? VKDT_FMT_UI16 : own->format == dt_token("f16") ? VKDT_FMT_F16
-              : own->format == dt_token("f32") ? VKDT_FMT_F32 : 0;
+  buf->format = dt_connector_hip_format(own);
   buf->array_length = own->array_length > 1 ? own->array_length : 1;
   buf->plane_stride = dt_connector_bufsize(own, own->roi.wd, own->roi.ht);
   return 0;

@@ -84,7 +84,12 @@ static const dt_cid_t s_cid_unset = { -1, -1 };
 static inline dt_cid_t dt_cid(int i, int c) { dt_cid_t r = { i, c }; return r; }
 static inline int dt_cid_unset(dt_cid_t c) { return c.i < 0 || c.c < 0; }
 
-enum { s_conn_none = 0, s_conn_clear = 1, s_conn_feedback = 2 };
+// connector flags (connector.h:60-71)
+enum
+{
+  s_conn_none = 0, s_conn_smooth = 1, s_conn_clear = 2, s_conn_feedback = 4, s_conn_dynamic_array = 8,
+  s_conn_protected = 16, s_conn_double_buffer = 32, s_conn_mipmap = 64, s_conn_clear_once = 128,
+};
 
 typedef struct dt_connector_t
 {
@@ -126,6 +131,7 @@ dt_connector_error_t;
 int dt_connector_channels(const dt_connector_t *c);
 size_t dt_connector_bytes_per_channel(const dt_connector_t *c);
 size_t dt_connector_bufsize(const dt_connector_t *c, uint32_t wd, uint32_t ht);
+uint32_t dt_connector_hip_format(const dt_connector_t *c);   // VKDT_FMT_* of a connector
 static inline int dt_connector_owner(const dt_connector_t *c)
 { return c->type == dt_token("write") || c->type == dt_token("source"); }
 static inline int dt_connector_input(const dt_connector_t *c)

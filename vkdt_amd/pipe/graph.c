@@ -46,8 +46,8 @@ int dt_connector_channels(const dt_connector_t *c)
 
 size_t dt_connector_bytes_per_channel(const dt_connector_t *c)
 {
-  if(c->format == dt_token("f32") || c->format == dt_token("ui32") || c->format == dt_token("u32") ||
-     c->format == dt_token("atom")) return 4;
+  if(c->format == dt_token("atom")) return 8;   // FP64 accumulators (vkdt_hip.h VKDT_FMT_ATOM)
+  if(c->format == dt_token("f32") || c->format == dt_token("ui32") || c->format == dt_token("u32")) return 4;
   if(c->format == dt_token("f16") || c->format == dt_token("ui16") || c->format == dt_token("u16")) return 2;
   if(c->format == dt_token("ui8") || c->format == dt_token("u8")) return 1;
   return 4;
@@ -59,11 +59,12 @@ size_t dt_connector_bufsize(const dt_connector_t *c, uint32_t wd, uint32_t ht)
   return n > 0 ? n : 16;
 }
 
-static uint32_t hip_format(const dt_connector_t *c)
+uint32_t dt_connector_hip_format(const dt_connector_t *c)
 {
   if(c->format == dt_token("ui16") || c->format == dt_token("u16")) return VKDT_FMT_UI16;
   if(c->format == dt_token("f16")) return VKDT_FMT_F16;
   if(c->format == dt_token("f32")) return VKDT_FMT_F32;
+  if(c->format == dt_token("atom")) return VKDT_FMT_ATOM;
   return 0;
 }
 
@@ -899,7 +900,7 @@ static void fill_buf(const dt_graph_t *g, const dt_node_t *n, int c, vkdt_buf_t 
   b->ht = own->roi.ht;
   b->pitch = own->roi.wd;
   b->chan = dt_connector_channels(own);
-  b->format = hip_format(own);
+  b->format = dt_connector_hip_format(own);
   b->array_length = MAX(1, own->array_length);
   b->plane_stride = dt_connector_bufsize(own, own->roi.wd, own->roi.ht);
 }
@@ -1044,6 +1045,10 @@ int dt_graph_run(dt_graph_t *g, dt_graph_run_t run)
     {
       vkdt_buf_t conn[DT_MAX_CONNECTORS];
       for(int c = 0; c < n->num_connectors; c++) fill_buf(g, n, c, conn + c);
+      for(int c = 0; c < n->num_connectors; c++)
+        if((n->connector[c].flags & s_conn_clear) && dt_connector_output(n->connector + c) && conn[c].ptr &&
+           vkdt_hip_buf_clear(conn + c, g->stream))
+          return gerr(g, -3, "clear %" PRItkn ": %s", dt_token_str(n->name), vkdt_hip_last_error());
       const dt_module_t *m = n->module;
       const void *params = m->committed_param_size ? (const void *)m->committed_param : (const void *)m->param;
       const size_t psize = m->committed_param_size ? (size_t)m->committed_param_size : (size_t)m->param_size;
