@@ -13,6 +13,7 @@
 #include "launch.h"
 
 #define RH_MAX_BINS 2048
+#define RH_BATCH 4
 
 // the module's committed uniform (rawhist/main.c:19-29): noise model
 // c + dx * x, filters, black and white in raw units
@@ -73,16 +74,31 @@ __global__ void __launch_bounds__(256) k_rh_collect(dimg in, double *hist, int b
     s_m2[i] = 0.0;
   }
   __syncthreads();
-  const long n = (long)nx * ny;
-  for(long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+  // each work group takes one contiguous run of blocks (a band of block
+  // rows), so its histogram spans the brightness of one band and its flush
+  // below touches few bins; RH_BATCH blocks per thread and pass have their
+  // loads in flight before the first LDS atomic
+  const int n = nx * ny, per = (n + gridDim.x - 1) / gridDim.x;
+  const int beg = blockIdx.x * per, end = min(n, beg + per), stride = blockDim.x;
+  for(int t0 = beg + threadIdx.x; t0 < end; t0 += RH_BATCH * stride)
   {
-    const int bx = (int)(t % nx), by = (int)(t / nx);
-    int bin;
-    float mean, mom2;
-    if(!rh_block(in, bx, by, filters, black, white, bins, bin, mean, mom2)) continue;
-    atomicAdd(&s_cnt[bin], 1u);
-    unsafeAtomicAdd(&s_m1[bin], (double)mean);
-    unsafeAtomicAdd(&s_m2[bin], (double)mom2);
+    int bin[RH_BATCH];
+    float mean[RH_BATCH], mom2[RH_BATCH];
+    bool ok[RH_BATCH];
+#pragma unroll
+    for(int k = 0; k < RH_BATCH; k++)
+    {
+      const int t = t0 + k * stride;
+      ok[k] = t < end && rh_block(in, t % nx, t / nx, filters, black, white, bins, bin[k], mean[k], mom2[k]);
+    }
+#pragma unroll
+    for(int k = 0; k < RH_BATCH; k++)
+    {
+      if(!ok[k]) continue;
+      atomicAdd(&s_cnt[bin[k]], 1u);
+      unsafeAtomicAdd(&s_m1[bin[k]], (double)mean[k]);
+      unsafeAtomicAdd(&s_m2[bin[k]], (double)mom2[k]);
+    }
   }
   __syncthreads();
   for(int i = threadIdx.x; i < bins; i += blockDim.x)
@@ -147,6 +163,7 @@ int launch_rawhist_collect(const node_call &c)
   // every invocation of the ceil(wd/8) x ceil(ht/8) groups of 8x8 runs
   const int nx = (int)((c.wd + 7) / 8 * 8), ny = (int)((c.ht + 7) / 8 * 8);
   const long n = (long)nx * ny;
+  if(n >= (1l << 30)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "rawhist/collect: %ld blocks", n);
   const int groups = (int)(n / 256 + 1 < 1024 ? n / 256 + 1 : 1024);
   // collect.comp:28-29: levels in raw units over 65536
   const float black = (float)p->black / 65536.0f, white = (float)p->white / 65536.0f;
