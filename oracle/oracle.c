@@ -1277,6 +1277,71 @@ static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int 
 }
 
 /* ------------------------------------------------------------------------
+ * demosaic/halfsize (halfsize.comp:19-50) and shared/resample
+ * (resample.comp:23-35 with sample_catmull_rom, shared.glsl:46-95)
+ * --------------------------------------------------------------------- */
+
+static int dm_halfsize(const int32_t *pci, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    v4 o;
+    if(pci[4] == 9)
+    {
+      real cc[9];
+      for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) cc[3 * i + j] = fetch(in, 3 * x + i, 3 * y + j, 0).x;
+      o.y = (cc[0] + cc[2] + cc[4] + cc[6] + cc[8]) * R(1.0) / R(5.0);
+      const real p = (cc[1] + cc[7]) * R(0.5), q = (cc[3] + cc[5]) * R(0.5);
+      if(((x + y) & 1) > 0) { o.x = p; o.z = q; }
+      else                  { o.z = p; o.x = q; }
+      o.w = R(1.0);   // undefined in the reference (alpha never written)
+    }
+    else
+    {
+      const v4 g = gather2(in, x, y);
+      o.x = g.w; o.y = (g.x + g.z) / R(2.0); o.z = g.y; o.w = R(1.0);
+    }
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+static int sh_resample(const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const real W = (real)in->wd, H = (real)in->ht;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const real u = ((real)x + R(0.5)) / (real)out->wd, v = ((real)y + R(0.5)) / (real)out->ht;
+    const real s[2] = { u * W, v * H };
+    real tp[2][3], w[2][3];
+    for(int a = 0; a < 2; a++)
+    { // catmull-rom weights of the four taps around the sample, the middle two merged
+      const real sz = a ? H : W;
+      const real t1 = floor(s[a] - R(0.5)) + R(0.5), f = s[a] - t1;
+      const real w0 = f * (R(-0.5) + f * (R(1.0) - R(0.5) * f));
+      const real w1 = R(1.0) + f * f * (R(-2.5) + R(1.5) * f);
+      const real w2 = f * (R(0.5) + f * (R(2.0) - R(1.5) * f));
+      const real w3 = f * f * (R(-0.5) + R(0.5) * f);
+      const real off = w2 / (w1 + w2);
+      tp[a][0] = (t1 - R(1.0)) / sz * sz; w[a][0] = w0;
+      tp[a][1] = (t1 + off) / sz * sz;    w[a][1] = w1 + w2;
+      tp[a][2] = (t1 + R(2.0)) / sz * sz; w[a][2] = w3;
+    }
+    v4 r = { 0, 0, 0, 0 };
+    for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++)
+    {
+      const v4 t = bilin(in, tp[0][i], tp[1][j], 0);   // tap * wx * wy, left to right
+      r.x += t.x * w[0][i] * w[1][j]; r.y += t.y * w[0][i] * w[1][j];
+      r.z += t.z * w[0][i] * w[1][j]; r.w += t.w * w[0][i] * w[1][j];
+    }
+    store(out, x, y, 0, r);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
  * rawhist (src/pipe/modules/rawhist) and nprof (src/pipe/modules/nprof):
  * noise statistics of the raw mosaic and the noise-model fit.
  * params (committed, rawhist/main.c:19-29): c, dx (float), filters, black,
@@ -1447,6 +1512,8 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("hilite", "reduce"))    return nconn >= 2 ? hl_reduce(pcf, par, conn) : -1;
   if(IS("hilite", "assemble"))  return nconn >= 3 ? hl_assemble(par, conn) : -1;
   if(IS("hilite", "doub"))      return nconn >= 3 ? hl_doub(pci, par, conn, wd, ht) : -1;
+  if(IS("demosaic", "halfsize")) return nconn >= 2 ? dm_halfsize(pci, conn) : -1;
+  if(IS("shared", "resample"))  return nconn >= 2 ? sh_resample(conn) : -1;
   if(IS("rawhist", "collect"))  return nconn >= 2 ? rh_collect(par, conn, wd, ht) : -1;
   if(IS("rawhist", "map"))      return nconn >= 2 ? rh_map(par, conn) : -1;
 #undef IS

@@ -71,6 +71,7 @@ class DevelopParams:
     detail: float = 1.0
     edges: tuple = (0.0, 0.0, 0.0, 0.0)
     dm_colour: int = 0
+    dm_method: int = 0          # 0 gaussian splats, 2 half size
     exposure: float = 0.0
     co_white: tuple = (0.0, 0.0, 0.0)
     light: float = 3.0
@@ -155,9 +156,20 @@ def colour_params(p):
     return cp
 
 
-def build(W, H, p):
+def export_extent(W, H, max_wd=0, max_ht=0):
+    """the extent a sink asks for when the export is limited to max_wd x
+    max_ht (graph-run-modules.h:466-470, float32 as the runtime computes it)"""
+    f32 = np.float32
+    sx = f32(W) / f32(max_wd) if max_wd > 0 else f32(1)
+    sy = f32(H) / f32(max_ht) if max_ht > 0 else f32(1)
+    s = max(sx, sy)
+    return int(f32(W) / s + f32(0.5)), int(f32(H) / s + f32(0.5))
+
+
+def build(W, H, p, max_wd=0, max_ht=0):
     """(buffers, nodes): buffers name -> (wd, ht, chan, fmt, planes);
-    nodes are dicts with module, kernel, pc, params, conn (buffer names), wd, ht, dp."""
+    nodes are dicts with module, kernel, pc, params, conn (buffer names), wd, ht, dp.
+    max_wd/max_ht: the export size limit (vkdt-cli --width/--height)."""
     xt = p.kind == "xtrans"
     block = 3 if xt else 2
     Qw, Qh = W // block, H // block
@@ -197,13 +209,34 @@ def build(W, H, p):
         add_hilite(W, H, p, buf, add, "dn", "hl")
 
     dmpc = mixed_blob([("f", v) for v in wb] + [("i", p.filters)])
-    dmpar = mixed_blob([("i", p.dm_colour), ("i", 0)])
-    buf("cov", Qw, Qh, 4, F16)
-    add("demosaic", "gauss", dmpc, dmpar, [mos, mos, "cov"], Qw, Qh)
-    buf("green", W, H, 1, F16)
-    add("demosaic", "splat", dmpc, dmpar, [mos, "cov", "green"], W, H)
-    buf("dm", W, H, 4, F16)
-    add("demosaic", "fix", dmpc, dmpar, [mos, "green", "cov", "dm"], W, H)
+    dmpar = mixed_blob([("i", p.dm_colour), ("i", p.dm_method)])
+    # demosaic's output: half size for method 2 (demosaic/main.c:28-33), then
+    # scaled to what the sink asks for; half size again once the scale
+    # reaches 1.5 blocks, and a resample to the requested extent
+    fw, fh = ((W + 1) // 2, (H + 1) // 2) if p.dm_method == 2 else (W, H)
+    ow, oh = export_extent(fw, fh, max_wd, max_ht)
+    scale = float(np.float32(W) / np.float32(ow))
+    if scale >= 1.5 * block or p.dm_method == 2:
+        buf("dmh", Qw, Qh, 4, F16)
+        add("demosaic", "halfsize", dmpc, dmpar, [mos, "dmh"], Qw, Qh)
+        if float(block) != scale:
+            buf("dm", ow, oh, 4, F16)
+            add("shared", "resample", None, None, ["dmh", "dm"], ow, oh)
+        else:
+            bufs["dm"] = bufs.pop("dmh")
+            nodes[-1]["conn"] = [mos, "dm"]
+    else:
+        buf("cov", Qw, Qh, 4, F16)
+        add("demosaic", "gauss", dmpc, dmpar, [mos, mos, "cov"], Qw, Qh)
+        buf("green", W, H, 1, F16)
+        add("demosaic", "splat", dmpc, dmpar, [mos, "cov", "green"], W, H)
+        fix_out = "dm" if (ow, oh) == (W, H) else "dmf"
+        buf(fix_out, W, H, 4, F16)
+        add("demosaic", "fix", dmpc, dmpar, [mos, "green", "cov", fix_out], W, H)
+        if fix_out != "dm":
+            buf("dm", ow, oh, 4, F16)
+            add("shared", "resample", None, None, ["dmf", "dm"], ow, oh)
+    W, H = ow, oh      # colour, filmcurv and llap run at the output extent
 
     buf("co", W, H, 4, F16)
     add("colour", "main", mixed_blob([("i", 0)] * 3), colour_params(p), ["dm", "co"], W, H)

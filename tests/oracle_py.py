@@ -132,3 +132,18 @@ def nprof_fit(bits, acc):
     a, b = C.c_float(), C.c_float()
     n = lib(bits).or_nprof_fit(acc.ctypes.data, acc.shape[1], C.byref(a), C.byref(b))
     return a.value, b.value, n
+
+
+def run_nodes(bits, bufs, nodes, raw, storage_f16=True):
+    """a restated node list (graphspec) on the oracle from the ui16 mosaic
+    `raw`; returns every buffer (float64, uint16 for ui16 buffers)"""
+    data = {}
+    for name, (wd, ht, chan, fmt, planes) in bufs.items():
+        shape = ((planes,) if planes > 1 else ()) + (ht, wd) + ((chan,) if chan > 1 else ())
+        data[name] = np.zeros(shape, np.uint16 if fmt == OR_UI16 else np.float64)
+    data["raw"][...] = raw
+    for n in nodes:
+        ob = [orbuf(data[c], bufs[c][3], bufs[c][2], bufs[c][4]) for c in n["conn"]]
+        node(bits, n["module"], n["kernel"], n["pc"], n["params"], ob, n["wd"], n["ht"], n["dp"],
+             storage_f16=storage_f16)
+    return data

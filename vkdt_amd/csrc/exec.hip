@@ -97,6 +97,8 @@ static const reg_entry g_registry[] = {
   { tkn("hilite"),   tkn("doub"),     launch_hilite_doub },
   { tkn("rawhist"),  tkn("collect"),  launch_rawhist_collect },
   { tkn("rawhist"),  tkn("map"),      launch_rawhist_map },
+  { tkn("demosaic"), tkn("halfsize"), launch_demosaic_halfsize },
+  { tkn("shared"),   tkn("resample"), launch_shared_resample },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)

@@ -76,6 +76,37 @@ __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg 
   st_h4(out, x, y, dm_fix_px<XT>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
 }
 
+// ---- halfsize (halfsize.comp:19-50): one rgba texel per CFA block --------
+
+template<int XT>
+__global__ void __launch_bounds__(256) k_dm_halfsize(dimg in, dimg out)
+{
+  const int x = tid_x(), y = tid_y();
+  if(x >= out.wd || y >= out.ht) return;
+  float4 o;
+  if(XT)
+  { // c[3i+j] = texel (3x+i, 3y+j), texelFetch (zero outside)
+    float c[9];
+#pragma unroll
+    for(int i = 0; i < 3; i++)
+#pragma unroll
+      for(int j = 0; j < 3; j++) c[3 * i + j] = fetch_m(in, 3 * x + i, 3 * y + j);
+    o.y = (c[0] + c[2] + c[4] + c[6] + c[8]) * 1.0f / 5.0f;
+    const float p = (c[1] + c[7]) * 0.5f, q = (c[3] + c[5]) * 0.5f;
+    const bool odd = ((x + y) & 1) != 0;
+    o.x = odd ? p : q;
+    o.z = odd ? q : p;
+    o.w = 1.0f;   // left unset by the reference (undefined alpha)
+  }
+  else
+  { // textureGather at the block centre: r = (0,0), g = mean of (0,1),(1,0), b = (1,1)
+    const float gx = centre_m(in, 2 * x, 2 * y + 1), gy = centre_m(in, 2 * x + 1, 2 * y + 1);
+    const float gz = centre_m(in, 2 * x + 1, 2 * y), gw = centre_m(in, 2 * x, 2 * y);
+    o = make_float4(gw, (gx + gz) / 2.0f, gy, 1.0f);
+  }
+  st_h4(out, x, y, o);
+}
+
 // ---- host launchers --------------------------------------------------------
 
 static int decode(dm_args &a, const node_call &c, const char *what)
@@ -149,4 +180,19 @@ int launch_demosaic_fix(const node_call &c)
     hipLaunchKernelGGL(k_dm_fix<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
                        to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), r);
   return launch_post("demosaic/fix");
+}
+
+int launch_demosaic_halfsize(const node_call &c)
+{
+  dm_args a;
+  if(int e = decode(a, c, "demosaic/halfsize")) return e;
+  if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/halfsize input")) return e;
+  if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "demosaic/halfsize output")) return e;
+  if(a.filters == 9)
+    hipLaunchKernelGGL(k_dm_halfsize<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]));
+  else
+    hipLaunchKernelGGL(k_dm_halfsize<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]));
+  return launch_post("demosaic/halfsize");
 }

@@ -90,3 +90,5 @@ int launch_hilite_assemble(const node_call &);
 int launch_hilite_doub(const node_call &);
 int launch_rawhist_collect(const node_call &);
 int launch_rawhist_map(const node_call &);
+int launch_demosaic_halfsize(const node_call &);
+int launch_shared_resample(const node_call &);

@@ -2,10 +2,12 @@
  * demosaic: gaussian-splat demosaicing of Bayer and X-Trans mosaics.
  *
  * ROI callbacks and node graph as in src/pipe/modules/demosaic/main.c:
- * method 0 (default) builds down -> gauss -> splat -> fix (:158-202).
- * RCD (method 1) and half-size (method 2) build the reference's node
- * names; their kernels are not part of this path, so running them fails
- * at launch with "no HIP kernel".
+ * method 0 (default) builds down -> gauss -> splat -> fix (:158-202);
+ * half-size (method 2, or an output scaled down by 1.5 blocks or more)
+ * builds halfsize [-> shared/resample] (:92-110); a scaled output of the
+ * full-resolution paths ends in shared/resample.  RCD (method 1) builds the
+ * reference's node names; its kernels are not part of this path, so running
+ * it fails at launch with "no HIP kernel".
  */
 #include "pipe/api.h"
 
@@ -53,6 +55,29 @@ dt_graph_run_t check_params(dt_module_t *module, uint32_t parid, uint32_t num, v
   return s_graph_run_record_cmd_buf;
 }
 
+// a shared/resample node to the module's output extent, wired to the
+// module output (demosaic/main.c:98-107,144-152)
+static int add_resample(dt_graph_t *graph, dt_module_t *module)
+{
+  const int id = dt_node_add(graph, module, "shared", "resample", module->connector[1].roi.wd,
+                             module->connector[1].roi.ht, 1, 0, 0, 2,
+                             "input", "read", "rgba", "f16", dt_no_roi,
+                             "output", "write", "rgba", "f16", &module->connector[1].roi);
+  dt_connector_copy(graph, module, 1, id, 1);
+  return id;
+}
+
+// the full-resolution paths end in a resample when the output extent differs
+// from the input's (a scaled export).  The reference keys this on a hard roi
+// marker on the output (demosaic/main.c:144,195), which its roi negotiation
+// sets on this chain; at equal extents that resample is the identity
+// (catmull-rom weights 0 1 0 0) and is left out here.
+static int output_scaled(const dt_module_t *module)
+{
+  return module->connector[0].roi.wd != module->connector[1].roi.wd ||
+         module->connector[0].roi.ht != module->connector[1].roi.ht;
+}
+
 void create_nodes(dt_graph_t *graph, dt_module_t *module)
 {
   const dt_image_params_t *ip = dt_module_get_input_img_param(graph, module, dt_token("input"));
@@ -78,12 +103,17 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
   const float scale = (float)module->connector[0].roi.wd / (float)module->connector[1].roi.wd;
   const int halfsize = (scale >= 1.5 * block) || (method == 2);
   if(halfsize)
-  {
+  { // block means, then resampled to the output unless that is the block grid
     const int id = dt_node_add(graph, module, "demosaic", "halfsize", rhalf.wd, rhalf.ht, 1, sizeof(pc), pc, 2,
                                "input", "read", "rggb", "*", dt_no_roi,
                                "output", "write", "rgba", "f16", &rhalf);
     dt_connector_copy(graph, module, 0, id, 0);
-    dt_connector_copy(graph, module, 1, id, 1);
+    if(block != scale)
+    {
+      const int id_res = add_resample(graph, module);
+      CONN(dt_node_connect(graph, id, 1, id_res, 0));
+    }
+    else dt_connector_copy(graph, module, 1, id, 1);
     return;
   }
   if(method == 1)
@@ -108,7 +138,12 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
     CONN(dt_node_connect_named(graph, id_conv, "lp", id_fill, "lp"));
     dt_connector_copy(graph, module, 0, id_conv, 0);
     dt_connector_copy(graph, module, 0, id_fill, 0);
-    dt_connector_copy(graph, module, 1, id_fill, 4);
+    if(output_scaled(module))
+    {
+      const int id_res = add_resample(graph, module);
+      CONN(dt_node_connect(graph, id_fill, 4, id_res, 0));
+    }
+    else dt_connector_copy(graph, module, 1, id_fill, 4);
     return;
   }
   // gaussian splats
@@ -136,5 +171,10 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
   dt_connector_copy(graph, module, 0, id_fix, 0);
   CONN(dt_node_connect(graph, id_splat, 2, id_fix, 1));
   CONN(dt_node_connect(graph, id_gauss, 2, id_fix, 2));
-  dt_connector_copy(graph, module, 1, id_fix, 3);
+  if(output_scaled(module))
+  {
+    const int id_res = add_resample(graph, module);
+    CONN(dt_node_connect(graph, id_fix, 3, id_res, 0));
+  }
+  else dt_connector_copy(graph, module, 1, id_fix, 3);
 }

@@ -35,6 +35,7 @@ def lib():
         L.dt_graph_plan.argtypes = [vp, u32]
         L.dt_graph_replace_display.argtypes = [vp, u64, u64]
         L.dt_graph_disconnect_display_modules.argtypes = [vp]
+        L.dt_graph_set_export_size.argtypes = [vp, i32, i32, i32]
         L.dt_graph_last_error.argtypes = [vp]
         L.dt_graph_last_error.restype = cp
         L.dt_graph_set_perf.argtypes = [vp, i32]
@@ -96,13 +97,17 @@ class Graph:
     def error(self):
         return self.L.dt_graph_last_error(self.g).decode(errors="replace")
 
-    def export(self, output_module="o-pfm", filename=None, inst="main"):
+    def export(self, output_module="o-pfm", filename=None, inst="main", max_wd=0, max_ht=0):
+        """replace display:inst by an output module; max_wd/max_ht scale the
+        image to fit (vkdt-cli --width/--height)"""
         m = self.L.dt_graph_replace_display(self.g, token(inst), token(output_module))
         if m < 0:
             raise PipeError(f"replace display:{inst} failed ({m}): {self.error()}")
         self.L.dt_graph_disconnect_display_modules(self.g)
         if filename is not None:
             self.set_param(output_module, inst, "filename", filename)
+        if max_wd or max_ht:
+            self.L.dt_graph_set_export_size(self.g, m, int(max_wd), int(max_ht))
         return m
 
     def set_param(self, module, inst, param, value):

@@ -85,6 +85,16 @@ int dt_graph_node_conn(const dt_graph_t *g, int n, int c, uint64_t tok[4], int l
   return 0;
 }
 
+// export size limit on a sink module's input (graph-export.c:207-208):
+// the image is scaled to fit max_wd x max_ht (0: no limit on that axis)
+int dt_graph_set_export_size(dt_graph_t *g, int m, int max_wd, int max_ht)
+{
+  if(m < 0 || m >= g->num_modules || g->module[m].connector[0].type != dt_token("sink")) return -1;
+  g->module[m].connector[0].max_wd = max_wd > 0 ? max_wd : 0;
+  g->module[m].connector[0].max_ht = max_ht > 0 ? max_ht : 0;
+  return 0;
+}
+
 // module lookup and parameter setting by name, for scripted graphs
 int dt_graph_module(dt_graph_t *g, const char *name, const char *inst)
 {

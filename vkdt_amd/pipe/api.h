@@ -54,19 +54,19 @@ typedef struct dt_graph_t dt_graph_t;
 typedef struct dt_module_t dt_module_t;
 typedef struct dt_node_t dt_node_t;
 
-/* ---- regions of interest (connector.h:24-30) --------------------------- */
+/* ---- regions of interest (connector.h:11-23) --------------------------- */
 typedef enum dt_roi_mark_t
 {
   s_roi_mark_uninited  = 0,
   s_roi_mark_soft      = 1,
   s_roi_mark_hard      = 2,
-  s_roi_mark_fwd       = 4,
-  s_roi_mark_bck       = 8,
-  s_roi_mark_soft_fwd  = 1 | 4,
-  s_roi_mark_soft_bck  = 1 | 8,
-  s_roi_mark_hard_fwd  = 2 | 4,
-  s_roi_mark_hard_bck  = 2 | 8,
-  s_roi_mark_dontcare  = 16,
+  s_roi_mark_bck       = 4,
+  s_roi_mark_fwd       = 8,
+  s_roi_mark_soft_bck  = 1 | 4,
+  s_roi_mark_hard_bck  = 2 | 4,
+  s_roi_mark_soft_fwd  = 1 | 8,
+  s_roi_mark_hard_fwd  = 2 | 8,
+  s_roi_mark_dontcare  = 0xff,
 }
 dt_roi_mark_t;
 
@@ -337,6 +337,7 @@ int  dt_module_set_param_int(dt_module_t *m, dt_token_t param, int32_t val);
 // replace display:<inst> by an output module (graph-export.c:24-96)
 int  dt_graph_replace_display(dt_graph_t *graph, dt_token_t inst, dt_token_t mod);
 void dt_graph_disconnect_display_modules(dt_graph_t *graph);
+int  dt_graph_set_export_size(dt_graph_t *graph, int module, int max_wd, int max_ht);
 // resolve a resource relative to the graph's search path, then basedir
 FILE *dt_graph_open_resource(dt_graph_t *graph, int frame, const char *fname, const char *mode);
 int  dt_graph_get_resource_filename(dt_module_t *mod, const char *fname, int frame, char *out, size_t size);

@@ -29,7 +29,7 @@ static double now_ms(void)
 int main(int argc, char *argv[])
 {
   const char *cfg = 0, *fmt = "o-pfm", *fname = "output", *prim = "bt2020", *trc = "linear";
-  int perf = 0, dev = 0;
+  int perf = 0, dev = 0, max_wd = 0, max_ht = 0;
   for(int i = 1; i < argc; i++)
   {
     if(!strcmp(argv[i], "-g") && i + 1 < argc) cfg = argv[++i];
@@ -39,11 +39,14 @@ int main(int argc, char *argv[])
     else if(!strcmp(argv[i], "--colour-trc") && i + 1 < argc) trc = argv[++i];
     else if(!strcmp(argv[i], "-d") && i + 1 < argc) { if(!strcmp(argv[++i], "perf") || !strcmp(argv[i], "all")) perf = 1; }
     else if(!strcmp(argv[i], "--device") && i + 1 < argc) dev = atoi(argv[++i]);
+    else if(!strcmp(argv[i], "--width") && i + 1 < argc) max_wd = (int)atof(argv[++i]);
+    else if(!strcmp(argv[i], "--height") && i + 1 < argc) max_ht = (int)atof(argv[++i]);
     else if(!strcmp(argv[i], "--version")) { printf("vkdt-cli (raw-development path, HIP gfx950)\n"); return 0; }
   }
   if(!cfg)
   {
-    fprintf(stderr, "usage: %s -g <graph.cfg> [--format o-pfm] [--filename <base>] [-d perf]\n", argv[0]);
+    fprintf(stderr, "usage: %s -g <graph.cfg> [--format o-pfm] [--filename <base>] [--width <px>] [--height <px>]"
+                    " [-d perf]\n", argv[0]);
     return 1;
   }
   if(strcmp(fmt, "o-pfm")) { fprintf(stderr, "[cli] output format %s not supported (o-pfm only)\n", fmt); return 1; }
@@ -65,6 +68,7 @@ int main(int argc, char *argv[])
   if(m < 0) { fprintf(stderr, "[cli] graph has no connected display:main\n"); goto out; }
   dt_graph_disconnect_display_modules(g);
   dt_module_set_param_string(g->module + m, dt_token("filename"), fname);
+  dt_graph_set_export_size(g, m, max_wd, max_ht);
   const double t0 = now_ms();
   if(dt_graph_run(g, s_graph_run_all)) { fprintf(stderr, "[cli] run failed: %s\n", dt_graph_last_error(g)); goto out; }
   const double t1 = now_ms();

@@ -667,11 +667,16 @@ static void roi_in(dt_graph_t *g, dt_module_t *m)
   {
     int output = dt_module_get_connector(m, dt_token("output"));
     if(output < 0 && m->connector[0].type == dt_token("sink"))
-    { // sinks ask for the full image (no LOD scaling on this path)
+    { // sinks ask for the full image, scaled to fit a requested export size
+      // (graph-run-modules.h:459-474)
       output = 0;
       dt_roi_t *r = &m->connector[0].roi;
-      r->wd = r->full_wd;
-      r->ht = r->full_ht;
+      const float max_wd = m->connector[0].max_wd, max_ht = m->connector[0].max_ht;
+      const float sx = max_wd > 0 ? r->full_wd / max_wd : 1.0f;
+      const float sy = max_ht > 0 ? r->full_ht / max_ht : 1.0f;
+      const float scale = sx > sy ? sx : sy;
+      r->wd = (int)(r->full_wd / scale + 0.5f);
+      r->ht = (int)(r->full_ht / scale + 0.5f);
       r->marker = m->inst == dt_token("main") ? s_roi_mark_hard_bck : s_roi_mark_soft_bck;
     }
     if(output >= 0)
