@@ -28,6 +28,7 @@ typedef float real;
 #define EXP   expf
 #define EXP2  exp2f
 #define LOG2  log2f
+#define LOG   logf
 #define POW   powf
 #define SQRT  sqrtf
 #define FABS  fabsf
@@ -38,6 +39,7 @@ typedef double real;
 #define EXP   exp
 #define EXP2  exp2
 #define LOG2  log2
+#define LOG   log
 #define POW   pow
 #define SQRT  sqrt
 #define FABS  fabs
@@ -1277,6 +1279,90 @@ static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int 
 }
 
 /* ------------------------------------------------------------------------
+ * colenc/main (colenc/main.comp:15-79): rec2020 -> export primaries + trc
+ * params: prim, trc (int)
+ * --------------------------------------------------------------------- */
+
+// rec2020 -> primaries, rows (modules/matrices.h); 0 where the reference
+// leaves the colour in rec2020
+static const double *enc_matrix(int prim)
+{
+  static const double rec709[9] = { 1.66022677, -0.58754761, -0.07283825, -0.12455334, 1.13292605, -0.00834963,
+                                    -0.01815514, -0.10060303, 1.11899817 };
+  static const double adobe[9] = { 1.15194302, -0.09753232, -0.05448118, -0.12454585, 1.13290963, -0.00837122,
+                                   -0.02253539, -0.04979918, 1.07275365 };
+  static const double p3[9] = { 1.34353337, -0.28218904, -0.06142427, -0.06530851, 1.07578268, -0.01048453,
+                                0.00282971, -0.01961215, 1.01717851 };
+  static const double xyz[9] = { 0.636958048301290991, 0.144616903586208406, 0.168880975164172054,
+                                 0.26270021201126692, 0.677998071518871148, 0.0593017164698619384,
+                                 4.9999999999999999e-17, 0.0280726930490874452, 1.06098505771079066 };
+  static const double ap0[9] = { 6.68685575e-01, 1.51817679e-01, 1.77189677e-01, 4.49002044e-02, 8.62145497e-01,
+                                 1.01922441e-01, -2.66851927e-09, 2.78271109e-02, 1.05170358 };
+  static const double ap1[9] = { 9.62918591e-01, 1.16137050e-02, 2.55863361e-02, 4.16800770e-04, 9.99378426e-01,
+                                 -8.82457347e-05, 5.31123331e-03, 2.18655328e-02, 9.75907920e-01 };
+  static const double red[9] = { 0.853263, 0.079695, 0.067042, 0.029375, 0.809195, 0.161430, 0.051575, 0.208097,
+                                 0.740329 };
+  switch(prim)
+  {
+    case 1: return rec709;
+    case 3: return adobe;
+    case 4: return p3;
+    case 5: return xyz;
+    case 6: return ap0;
+    case 7: return ap1;
+    case 10: return red;
+    default: return 0;
+  }
+}
+
+static real enc_trc(real v, int trc)
+{
+  if(trc == 1)
+  {
+    const real a = R(1.09929682680944), b = R(0.018053968510807);
+    return v > b ? POW(v, R(1.0) / R(2.2)) * a - (a - R(1.0)) : v * R(4.5);
+  }
+  if(trc == 2) return v > R(0.0031308) ? POW(v, R(1.0) / R(2.4)) * R(1.055) - R(0.055) : v * R(12.92);
+  if(trc == 3)
+  {
+    const real c3 = R(2392.0) / R(128.0), c2 = R(2413.0) / R(128.0), c1 = c3 - c2 + R(1.0);
+    const real m1 = R(1305.0) / R(8192.0), m2 = R(2523.0) / R(32.0);
+    const real p = POW(FMAX(R(0.0), v), m1);
+    const real num = (c1 - R(1.0)) + (c2 - c3) * p, den = R(1.0) + c3 * p;
+    return POW(R(1.0) + num / den, m2);
+  }
+  if(trc == 4) return POW(v, R(1.0) / R(2.6));
+  if(trc == 5)
+  {
+    const real a = R(0.17883277), b = R(1.0) - R(4.0) * a, c = R(0.5) - a * LOG(R(4.0) * a);
+    return v > R(1.0) / R(12.0) ? a * LOG(R(12.0) * v - b) + c : SQRT(R(3.0) * v);
+  }
+  if(trc == 6) return POW(v, R(1.0) / R(2.2));
+  return v;
+}
+
+static int co_colenc(const int32_t *par, const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const double *M = enc_matrix(par[0]);
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const v4 t = fetch(in, x, y, 0);
+    real r = t.x, g = t.y, b = t.z;
+    if(M)
+    {
+      const real r2 = R(M[0]) * r + R(M[1]) * g + R(M[2]) * b;
+      const real g2 = R(M[3]) * r + R(M[4]) * g + R(M[5]) * b;
+      const real b2 = R(M[6]) * r + R(M[7]) * g + R(M[8]) * b;
+      r = r2; g = g2; b = b2;
+    }
+    const v4 o = { enc_trc(r, par[1]), enc_trc(g, par[1]), enc_trc(b, par[1]), R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
  * demosaic/halfsize (halfsize.comp:19-50) and shared/resample
  * (resample.comp:23-35 with sample_catmull_rom, shared.glsl:46-95)
  * --------------------------------------------------------------------- */
@@ -1512,6 +1598,7 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("hilite", "reduce"))    return nconn >= 2 ? hl_reduce(pcf, par, conn) : -1;
   if(IS("hilite", "assemble"))  return nconn >= 3 ? hl_assemble(par, conn) : -1;
   if(IS("hilite", "doub"))      return nconn >= 3 ? hl_doub(pci, par, conn, wd, ht) : -1;
+  if(IS("colenc", "main"))      return nconn >= 2 ? co_colenc((const int32_t *)params, conn) : -1;
   if(IS("demosaic", "halfsize")) return nconn >= 2 ? dm_halfsize(pci, conn) : -1;
   if(IS("shared", "resample"))  return nconn >= 2 ? sh_resample(conn) : -1;
   if(IS("rawhist", "collect"))  return nconn >= 2 ? rh_collect(par, conn, wd, ht) : -1;

@@ -166,10 +166,11 @@ def export_extent(W, H, max_wd=0, max_ht=0):
     return int(f32(W) / s + f32(0.5)), int(f32(H) / s + f32(0.5))
 
 
-def build(W, H, p, max_wd=0, max_ht=0):
+def build(W, H, p, max_wd=0, max_ht=0, prim=2, trc=0):
     """(buffers, nodes): buffers name -> (wd, ht, chan, fmt, planes);
     nodes are dicts with module, kernel, pc, params, conn (buffer names), wd, ht, dp.
-    max_wd/max_ht: the export size limit (vkdt-cli --width/--height)."""
+    max_wd/max_ht: the export size limit (vkdt-cli --width/--height);
+    prim/trc: the export encoding (a colenc node unless linear rec2020)."""
     xt = p.kind == "xtrans"
     block = 3 if xt else 2
     Qw, Qh = W // block, H // block
@@ -258,8 +259,14 @@ def build(W, H, p, max_wd=0, max_ht=0):
         coarse = "pyr0" if l == nl - 1 else f"out{l}"
         add("llap", "assemble", mixed_blob([("i", ng), ("i", 1 if l == nl - 1 else 0)]), lpar,
             [coarse, f"pyr{l-1}", f"pyr{l}", f"out{l-1}"], lw[l - 1], lh[l - 1])
-    buf("final", W, H, 4, F32)
-    add("llap", "colour", None, lpar, ["out0", "fc", "final"], W, H)
+    if (prim, trc) == (2, 0):
+        buf("final", W, H, 4, F32)
+        add("llap", "colour", None, lpar, ["out0", "fc", "final"], W, H)
+    else:   # graph-export.c:66-86: llap keeps f16, colenc writes the sink's f32
+        buf("lin", W, H, 4, F16)
+        add("llap", "colour", None, lpar, ["out0", "fc", "lin"], W, H)
+        buf("final", W, H, 4, F32)
+        add("colenc", "main", None, mixed_blob([("i", prim), ("i", trc)]), ["lin", "final"], W, H)
     return bufs, nodes
 
 

@@ -9,13 +9,13 @@ import graphspec as gs
 from vkdt_amd import pipe, synth
 
 
-def planned(tmp_path, kind, W, H, extra=(), hilite=False, max_wd=0, max_ht=0):
+def planned(tmp_path, kind, W, H, extra=(), hilite=False, max_wd=0, max_ht=0, prim=2, trc=0):
     cfa, _ = synth.mosaic(W, H, 1, kind)
     path = str(tmp_path / "p.dng")
     synth.write_dng(path, cfa, kind, as_shot_neutral=(0.45, 1.0, 0.7))
     na, nb = synth.noise_params(2e-4, 1e-6)
     g = pipe.Graph(lines=pipe.raw_graph_cfg(path, noise_a=na, noise_b=nb, extra=extra, hilite=hilite))
-    g.export("o-pfm", str(tmp_path / "out"), max_wd=max_wd, max_ht=max_ht)
+    g.export("o-pfm", str(tmp_path / "out"), max_wd=max_wd, max_ht=max_ht, prim=prim, trc=trc)
     g.plan()
     g.commit()
     return g, np.float32(na), np.float32(nb)
@@ -29,18 +29,23 @@ PLAN_CASES = [("bayer", 200, 150, False, 0, 0, 0), ("xtrans", 201, 150, False, 0
               ("bayer", 200, 150, False, 0, 150, 0),    # scale 1.3: full demosaic + resample
               ("xtrans", 201, 150, False, 0, 0, 40),    # scale 3.7 < 4.5: full + resample
               ("xtrans", 201, 150, False, 2, 0, 0),     # method 2: half size + resample
-              ("bayer", 200, 150, False, 2, 0, 0)]      # method 2 on the block grid
+              ("bayer", 200, 150, False, 2, 0, 0),      # method 2 on the block grid
+              ("bayer", 200, 150, False, 0, 0, 0, 1, 2),    # sRGB export: colenc
+              ("xtrans", 201, 150, True, 0, 80, 0, 4, 3)]   # P3 + PQ, hilite, scaled
 
 
-@pytest.mark.parametrize("kind,W,H,hilite,method,max_wd,max_ht", PLAN_CASES)
-def test_nodes_match_restatement(tmp_path, kind, W, H, hilite, method, max_wd, max_ht):
+@pytest.mark.parametrize("case", PLAN_CASES, ids=[str(i) for i in range(len(PLAN_CASES))])
+def test_nodes_match_restatement(tmp_path, case):
+    kind, W, H, hilite, method, max_wd, max_ht = case[:7]
+    prim, trc = case[7:] if len(case) > 7 else (2, 0)
     extra = [f"param:demosaic:01:method:{method}"] if method else []
-    g, na, nb = planned(tmp_path, kind, W, H, hilite=hilite, extra=extra, max_wd=max_wd, max_ht=max_ht)
+    g, na, nb = planned(tmp_path, kind, W, H, hilite=hilite, extra=extra, max_wd=max_wd, max_ht=max_ht,
+                        prim=prim, trc=trc)
     ip = g.img_param("i-raw", "main")
     W, H = ip["full"]
     p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb, cam=np.array(ip["cam"], np.float32).reshape(3, 3),
                          wb=tuple(ip["wb"][:3]) + (ip["wb"][3],), hilite=hilite, dm_method=method)
-    _, ref = gs.build(W, H, p, max_wd, max_ht)
+    _, ref = gs.build(W, H, p, max_wd, max_ht, prim, trc)
     nodes = {n["id"]: n for n in g.nodes()}
     live = [nodes[k] for k in g.exec_nodes() if nodes[k]["module"] not in ("i-raw", "o-pfm")]
     assert [(n["module"], n["kernel"]) for n in live] == [(r["module"], r["kernel"]) for r in ref]

@@ -99,6 +99,7 @@ static const reg_entry g_registry[] = {
   { tkn("rawhist"),  tkn("map"),      launch_rawhist_map },
   { tkn("demosaic"), tkn("halfsize"), launch_demosaic_halfsize },
   { tkn("shared"),   tkn("resample"), launch_shared_resample },
+  { tkn("colenc"),   tkn("main"),     launch_colenc_main },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)

@@ -92,3 +92,4 @@ int launch_rawhist_collect(const node_call &);
 int launch_rawhist_map(const node_call &);
 int launch_demosaic_halfsize(const node_call &);
 int launch_shared_resample(const node_call &);
+int launch_colenc_main(const node_call &);

@@ -36,6 +36,7 @@ def lib():
         L.dt_graph_replace_display.argtypes = [vp, u64, u64]
         L.dt_graph_disconnect_display_modules.argtypes = [vp]
         L.dt_graph_set_export_size.argtypes = [vp, i32, i32, i32]
+        L.dt_graph_replace_display_colour.argtypes = [vp, u64, u64, i32, i32]
         L.dt_graph_last_error.argtypes = [vp]
         L.dt_graph_last_error.restype = cp
         L.dt_graph_set_perf.argtypes = [vp, i32]
@@ -97,10 +98,11 @@ class Graph:
     def error(self):
         return self.L.dt_graph_last_error(self.g).decode(errors="replace")
 
-    def export(self, output_module="o-pfm", filename=None, inst="main", max_wd=0, max_ht=0):
+    def export(self, output_module="o-pfm", filename=None, inst="main", max_wd=0, max_ht=0, prim=2, trc=0):
         """replace display:inst by an output module; max_wd/max_ht scale the
-        image to fit (vkdt-cli --width/--height)"""
-        m = self.L.dt_graph_replace_display(self.g, token(inst), token(output_module))
+        image to fit (vkdt-cli --width/--height); prim/trc other than linear
+        rec2020 (2, 0) put a colenc module in between (--colour-prim/-trc)"""
+        m = self.L.dt_graph_replace_display_colour(self.g, token(inst), token(output_module), prim, trc)
         if m < 0:
             raise PipeError(f"replace display:{inst} failed ({m}): {self.error()}")
         self.L.dt_graph_disconnect_display_modules(self.g)

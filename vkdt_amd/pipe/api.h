@@ -140,21 +140,32 @@ static inline int dt_connector_output(const dt_connector_t *c)
 { return c->type == dt_token("write") || c->type == dt_token("source") || c->type == dt_token("modify"); }
 
 /* ---- image metadata travelling with the graph (module.h:55-100) ------- */
-typedef enum dt_colour_primaries_t
+typedef enum dt_colour_primaries_t   // module.h:21-42
 {
-  s_colour_primaries_custom = 0,
-  s_colour_primaries_srgb   = 1,
-  s_colour_primaries_2020   = 2,
-  s_colour_primaries_XYZ    = 5,
-  s_colour_primaries_unknown = 0xffff,
+  s_colour_primaries_custom       = 0,
+  s_colour_primaries_srgb         = 1,
+  s_colour_primaries_2020         = 2,
+  s_colour_primaries_adobe        = 3,
+  s_colour_primaries_P3           = 4,
+  s_colour_primaries_XYZ          = 5,
+  s_colour_primaries_aces2065_1   = 6,
+  s_colour_primaries_acescg       = 7,
+  s_colour_primaries_arri3        = 8,
+  s_colour_primaries_arri4        = 9,
+  s_colour_primaries_red          = 10,
+  s_colour_primaries_unknown      = 0xffff,
 }
 dt_colour_primaries_t;
 
-typedef enum dt_colour_trc_t
+typedef enum dt_colour_trc_t          // module.h:44-60
 {
   s_colour_trc_linear  = 0,
   s_colour_trc_709     = 1,
   s_colour_trc_srgb    = 2,
+  s_colour_trc_PQ      = 3,
+  s_colour_trc_DCI     = 4,
+  s_colour_trc_HLG     = 5,
+  s_colour_trc_gamma   = 6,
   s_colour_trc_unknown = 0xffff,
 }
 dt_colour_trc_t;
@@ -336,6 +347,9 @@ int  dt_module_set_param_float(dt_module_t *m, dt_token_t param, float val);
 int  dt_module_set_param_int(dt_module_t *m, dt_token_t param, int32_t val);
 // replace display:<inst> by an output module (graph-export.c:24-96)
 int  dt_graph_replace_display(dt_graph_t *graph, dt_token_t inst, dt_token_t mod);
+// the same with an output colour encoding: a colenc module goes in between
+// unless the encoding is linear rec2020 (graph-export.c:60-90)
+int  dt_graph_replace_display_colour(dt_graph_t *graph, dt_token_t inst, dt_token_t mod, int prim, int trc);
 void dt_graph_disconnect_display_modules(dt_graph_t *graph);
 int  dt_graph_set_export_size(dt_graph_t *graph, int module, int max_wd, int max_ht);
 // resolve a resource relative to the graph's search path, then basedir

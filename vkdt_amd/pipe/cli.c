@@ -6,8 +6,11 @@
  *   -g <graph.cfg>            graph to process
  *   --format o-pfm            output module (o-pfm only)
  *   --filename <base>         output file name without extension
- *   --colour-prim bt2020      only linear rec2020 output is produced
- *   --colour-trc linear
+ *   --colour-prim <p>         sRGB bt2020 adobergb P3 XYZ (default sRGB)
+ *   --colour-trc <t>          linear 709 sRGB PQ DCI gamma2.2; the
+ *                             reference's default is its parse of "sRGB" as
+ *                             primaries, i.e. trc 1 (709) -- kept as is
+ *   --width/--height <px>     scale the export to fit
  *   -d perf                   per-node and total GPU time
  *   --device <n>              HIP device
  */
@@ -15,9 +18,32 @@
 #include "vkdt_hip.h"
 
 #include <libgen.h>
+#include <strings.h>
 #include <stdlib.h>
 #include <time.h>
 #include <unistd.h>
+
+// cli/main.c:14-35
+static int parse_prim(const char *s)
+{
+  if(!strcasecmp(s, "sRGB")) return s_colour_primaries_srgb;
+  if(!strcasecmp(s, "bt2020")) return s_colour_primaries_2020;
+  if(!strcasecmp(s, "adobergb")) return s_colour_primaries_adobe;
+  if(!strcasecmp(s, "P3")) return s_colour_primaries_P3;
+  if(!strcasecmp(s, "XYZ")) return s_colour_primaries_XYZ;
+  return s_colour_primaries_unknown;
+}
+
+static int parse_trc(const char *s)
+{
+  if(!strcasecmp(s, "linear")) return s_colour_trc_linear;
+  if(!strcasecmp(s, "709")) return s_colour_trc_709;
+  if(!strcasecmp(s, "sRGB")) return s_colour_trc_srgb;
+  if(!strcasecmp(s, "PQ")) return s_colour_trc_PQ;
+  if(!strcasecmp(s, "DCI")) return s_colour_trc_DCI;
+  if(!strcasecmp(s, "gamma2.2")) return s_colour_trc_gamma;
+  return s_colour_trc_unknown;
+}
 
 static double now_ms(void)
 {
@@ -28,15 +54,16 @@ static double now_ms(void)
 
 int main(int argc, char *argv[])
 {
-  const char *cfg = 0, *fmt = "o-pfm", *fname = "output", *prim = "bt2020", *trc = "linear";
+  const char *cfg = 0, *fmt = "o-pfm", *fname = "output";
+  int prim = parse_prim("sRGB"), trc = parse_prim("sRGB");   // cli/main.c:58-59
   int perf = 0, dev = 0, max_wd = 0, max_ht = 0;
   for(int i = 1; i < argc; i++)
   {
     if(!strcmp(argv[i], "-g") && i + 1 < argc) cfg = argv[++i];
     else if(!strcmp(argv[i], "--format") && i + 1 < argc) fmt = argv[++i];
     else if(!strcmp(argv[i], "--filename") && i + 1 < argc) fname = argv[++i];
-    else if(!strcmp(argv[i], "--colour-prim") && i + 1 < argc) prim = argv[++i];
-    else if(!strcmp(argv[i], "--colour-trc") && i + 1 < argc) trc = argv[++i];
+    else if(!strcmp(argv[i], "--colour-prim") && i + 1 < argc) prim = parse_prim(argv[++i]);
+    else if(!strcmp(argv[i], "--colour-trc") && i + 1 < argc) trc = parse_trc(argv[++i]);
     else if(!strcmp(argv[i], "-d") && i + 1 < argc) { if(!strcmp(argv[++i], "perf") || !strcmp(argv[i], "all")) perf = 1; }
     else if(!strcmp(argv[i], "--device") && i + 1 < argc) dev = atoi(argv[++i]);
     else if(!strcmp(argv[i], "--width") && i + 1 < argc) max_wd = (int)atof(argv[++i]);
@@ -50,8 +77,6 @@ int main(int argc, char *argv[])
     return 1;
   }
   if(strcmp(fmt, "o-pfm")) { fprintf(stderr, "[cli] output format %s not supported (o-pfm only)\n", fmt); return 1; }
-  if(strcmp(prim, "bt2020") || strcmp(trc, "linear"))
-  { fprintf(stderr, "[cli] only --colour-prim bt2020 --colour-trc linear is supported\n"); return 1; }
 
   char exe[4096] = {0}, base[4096];
   if(readlink("/proc/self/exe", exe, sizeof(exe) - 1) <= 0) snprintf(exe, sizeof(exe), "%s", argv[0]);
@@ -64,7 +89,7 @@ int main(int argc, char *argv[])
   g->perf = perf;
   int ret = 1;
   if(dt_graph_read_config_ascii(g, cfg)) goto out;
-  const int m = dt_graph_replace_display(g, dt_token("main"), dt_token("o-pfm"));
+  const int m = dt_graph_replace_display_colour(g, dt_token("main"), dt_token("o-pfm"), prim, trc);
   if(m < 0) { fprintf(stderr, "[cli] graph has no connected display:main\n"); goto out; }
   dt_graph_disconnect_display_modules(g);
   dt_module_set_param_string(g->module + m, dt_token("filename"), fname);

@@ -559,6 +559,11 @@ int dt_graph_read_config_ascii(dt_graph_t *g, const char *filename)
 
 int dt_graph_replace_display(dt_graph_t *g, dt_token_t inst, dt_token_t mod)
 {
+  return dt_graph_replace_display_colour(g, inst, mod, s_colour_primaries_2020, s_colour_trc_linear);
+}
+
+int dt_graph_replace_display_colour(dt_graph_t *g, dt_token_t inst, dt_token_t mod, int prim, int trc)
+{
   if(inst == 0) inst = dt_token("main");
   const int mid = dt_module_get(g, dt_token("display"), inst);
   if(mid < 0) return -1;
@@ -569,6 +574,25 @@ int dt_graph_replace_display(dt_graph_t *g, dt_token_t inst, dt_token_t mod)
   const int m2 = dt_module_add(g, mod, inst);
   if(m2 < 0) return -3;
   const int i2 = dt_module_get_connector(g->module + m2, dt_token("input"));
+  if(g->module[m2].connector[i2].format == dt_token("ui8") || prim != s_colour_primaries_2020 ||
+     trc != s_colour_trc_linear)
+  { // encode on the way out: m0 -> colenc -> output module
+    const int m1 = dt_module_add(g, dt_token("colenc"), inst);
+    if(m1 < 0) return gerr(g, -3, "no colenc module");
+    const int i1 = dt_module_get_connector(g->module + m1, dt_token("input"));
+    const int o1 = dt_module_get_connector(g->module + m1, dt_token("output"));
+    if(prim == s_colour_primaries_unknown) prim = g->module[m0].img_param.colour_primaries;
+    if(trc == s_colour_trc_unknown) trc = g->module[m0].img_param.colour_trc;
+    dt_module_t *enc = g->module + m1;
+    *(int32_t *)dt_module_param_int(enc, dt_module_get_param(enc->so, dt_token("prim"))) = prim;
+    *(int32_t *)dt_module_param_int(enc, dt_module_get_param(enc->so, dt_token("trc"))) = trc;
+    g->module[m1].connector[o1].format = g->module[m2].connector[i2].format;
+    g->module[m1].connector[o1].chan = g->module[m2].connector[i2].chan;
+    int err = dt_module_connect(g, m0, o0, m1, i1);
+    if(!err) err = dt_module_connect(g, m1, o1, m2, i2);
+    if(err) return gerr(g, -4, "connecting colenc failed: %d", err);
+    return m2;
+  }
   // linear rec2020 output: no colour encoding module needed, the last
   // module writes the sink's format directly
   g->module[m0].connector[o0].format = g->module[m2].connector[i2].format;
