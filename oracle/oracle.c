@@ -1279,6 +1279,175 @@ static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int 
 }
 
 /* ------------------------------------------------------------------------
+ * demosaic RCD (rcd_conv.comp:11-49, rcd_fill.comp:24-161), RGGB only.
+ * conn: cfa, vh (W x H), pq, lp (W/2 x H) [, output rgba]
+ * The fill runs tile by tile (64x32, 3-site border) with the tile arrays in
+ * binary16; array reads outside a tile's 64x32 entries give 0.
+ * --------------------------------------------------------------------- */
+
+static inline real rcd_c(const or_buf_t *b, int x, int y) { return centre(b, x, y, 0).x; }
+
+static int dm_rcd_conv(const or_buf_t *c)
+{
+  const or_buf_t *cfa = c + 0, *vh = c + 1, *pq = c + 2, *lp = c + 3;
+  for(int y = 0; y < cfa->ht; y++) for(int x = 0; x < cfa->wd; x++)
+  {
+#define CF(dx, dy) rcd_c(cfa, x + (dx), y + (dy))
+    real vx = 0, vy = 0;
+    for(int i = -1; i <= 1; i++)
+    {
+      vx += CF(i, -3) - R(3.0) * CF(i, -2) - CF(i, -1) + R(6.0) * CF(i, 0) - CF(i, 1) - R(3.0) * CF(i, 2) + CF(i, 3);
+      vy += CF(-3, i) - R(3.0) * CF(-2, i) - CF(-1, i) + R(6.0) * CF(0, i) - CF(1, i) - R(3.0) * CF(2, i) + CF(3, i);
+    }
+    vx *= vx; vy *= vy;
+    const v4 o = { vx / (R(1e-5) + vx + vy), 0, 0, 1 };
+    store(vh, x, y, 0, o);
+    if(((x + y) & 1) == 0)
+    {
+      real px = R(1e-5), py = R(1e-5);
+      for(int i = -1; i <= 1; i++)
+      {
+        px += CF(-3 + i, -3 + i) - CF(1 + i, -1 + i) - CF(1 + i, 1 + i) + CF(3 + i, 3 + i)
+              - R(3.0) * (CF(-2 + i, -2 + i) + CF(2 + i, 2 + i)) + R(6.0) * CF(i, i);
+        py += CF(3 + i, -3 - i) - CF(1 + i, -1 - i) - CF(-1 + i, 1 - i) + CF(-3 + i, 3 - i)
+              - R(3.0) * (CF(2 + i, -2 - i) + CF(-2 + i, 2 - i)) + R(6.0) * CF(i, -i);
+      }
+      px *= px; py *= py;
+      const v4 q = { px / (px + py), 0, 0, 1 };
+      store(pq, x / 2, y, 0, q);
+    }
+    else
+    {
+      const int off = (x & 1) ? -1 : 1;
+      const real w[3] = { R(0.5), R(1.0), R(0.5) };
+      real l = 0;
+      for(int j = -1; j <= 1; j++) for(int i = -1; i <= 1; i++) l += w[j + 1] * w[i + 1] * CF(i + off, j);
+      const v4 q = { FMAX(R(1e-6), l), 0, 0, 1 };
+      store(lp, x / 2, y, 0, q);
+    }
+#undef CF
+  }
+  return 0;
+}
+
+static inline int rcd_col(int x, int y) { return ((x + y) & 1) ? 1 : ((y & 1) ? 2 : 0); }
+static inline real rcd_sh(const real *a, int i) { return (i >= 0 && i < 64 * 32) ? a[i] : R(0); }
+static inline real rcd_h16(real v) { return (real)or_f16_round((double)v); }
+static inline real rcd_discr(real c, real n) { return FABS(R(0.5) - c) < FABS(R(0.5) - n) ? n : c; }
+static inline real rcd_half(const or_buf_t *b, int x, int y) { return rcd_c(b, (x + ((y & 1) ? 0 : 1)) / 2, y); }
+
+static int dm_rcd_fill(const float *pcf, const or_buf_t *c, int wd, int ht)
+{
+  const or_buf_t *cfa = c + 0, *vh = c + 1, *pq = c + 2, *lp = c + 3, *out = c + 4;
+  const real wb[3] = { R(pcf[0]), R(pcf[1]), R(pcf[2]) }, eps = R(1e-5);
+  static real sr[64 * 32], sg[64 * 32], sb[64 * 32];
+  const int ntx = (wd + 7) / 8, nty = (ht + 7) / 8;   // one tile per 8x8 group of the node extent
+  for(int ty = 0; ty < nty; ty++) for(int tx = 0; tx < ntx; tx++)
+  {
+    const int ox = tx * 58, oy = ty * 26;
+    if(ox >= out->wd || oy >= out->ht) continue;
+#define PX(lx) (ox + (lx) - 3)
+#define PY(ly) (oy + (ly) - 3)
+    for(int ly = 0; ly < 32; ly++) for(int lx = 0; lx < 64; lx++)
+    { // fill
+      const int x = PX(lx), y = PY(ly), col = rcd_col(x, y);
+      const real v = FMIN(FMAX(fetch(cfa, x, y, 0).x, R(0)), R(65535.0));
+      sr[64 * ly + lx] = rcd_h16(col == 0 ? wb[0] * v : R(0));
+      sg[64 * ly + lx] = rcd_h16(col == 1 ? wb[1] * v : R(0));
+      sb[64 * ly + lx] = rcd_h16(col == 2 ? wb[2] * v : R(0));
+    }
+    for(int ly = 0; ly < 32; ly++) for(int lx = 0; lx < 64; lx += 2)
+    { // green at the red/blue site of each pair
+      const int k = rcd_col(PX(lx), PY(ly)) == 1 ? 1 : 0;
+      const int x = PX(lx) + k, y = PY(ly), i = 64 * ly + lx + k;
+#define CF(dx, dy) fetch(cfa, x + (dx), y + (dy), 0).x
+#define VH(dx, dy) rcd_c(vh, x + (dx), y + (dy))
+#define LP(dx, dy) rcd_half(lp, x + (dx), y + (dy))
+      const real vhc = VH(0, 0), vhn = R(0.25) * (VH(-1, -1) + VH(1, -1) + VH(-1, 1) + VH(1, 1));
+      const real d = rcd_discr(vhc, vhn);
+      const real Ng = eps + FABS(CF(0, -1) - CF(0, 1)) + FABS(CF(0, 0) - CF(0, -2)) + FABS(CF(0, -1) - CF(0, -3)) + FABS(CF(0, -2) - CF(0, -4));
+      const real Sg = eps + FABS(CF(0, -1) - CF(0, 1)) + FABS(CF(0, 0) - CF(0, 2)) + FABS(CF(0, 1) - CF(0, 3)) + FABS(CF(0, 2) - CF(0, 4));
+      const real Wg = eps + FABS(CF(-1, 0) - CF(1, 0)) + FABS(CF(0, 0) - CF(-2, 0)) + FABS(CF(-1, 0) - CF(-3, 0)) + FABS(CF(-2, 0) - CF(-4, 0));
+      const real Eg = eps + FABS(CF(-1, 0) - CF(1, 0)) + FABS(CF(0, 0) - CF(2, 0)) + FABS(CF(1, 0) - CF(3, 0)) + FABS(CF(2, 0) - CF(4, 0));
+      const real l0 = LP(0, 0);
+      const real Ne = CF(0, -1) * R(2.0) * l0 / (eps + l0 + LP(0, -2));
+      const real Se = CF(0, 1) * R(2.0) * l0 / (eps + l0 + LP(0, 2));
+      const real We = CF(-1, 0) * R(2.0) * l0 / (eps + l0 + LP(-2, 0));
+      const real Ee = CF(1, 0) * R(2.0) * l0 / (eps + l0 + LP(2, 0));
+      const real ve = FMIN(FMAX((Sg * Ne + Ng * Se) / (Ng + Sg), R(0)), R(65534.0));
+      const real he = FMIN(FMAX((Wg * Ee + Eg * We) / (Eg + Wg), R(0)), R(65534.0));
+      sg[i] = rcd_h16(mixr(ve, he, d));
+#undef CF
+#undef VH
+#undef LP
+    }
+    for(int ly = 0; ly < 32; ly++) for(int lx = 0; lx < 64; lx += 2)
+    { // the other of red/blue at the red/blue site
+      const int k = rcd_col(PX(lx), PY(ly)) == 1 ? 1 : 0;
+      const int x = PX(lx) + k, y = PY(ly), i = 64 * ly + lx + k;
+      const int red = rcd_col(x, y) == 0;
+      const real *oc = red ? sb : sr;
+#define PQ(dx, dy) rcd_half(pq, x + (dx), y + (dy))
+#define G(dx, dy) rcd_sh(sg, i + 64 * (dy) + (dx))
+#define C(dx, dy) rcd_sh(oc, i + 64 * (dy) + (dx))
+      const real pqc = PQ(0, 0), pqn = R(0.25) * (PQ(-1, -1) + PQ(1, -1) + PQ(-1, 1) + PQ(1, 1));
+      const real d = rcd_discr(pqc, pqn);
+      const real NWg = eps + FABS(C(-1, -1) - C(1, 1)) + FABS(C(-1, -1) - C(-3, -3)) + FABS(G(0, 0) - G(-2, -2));
+      const real NEg = eps + FABS(C(1, -1) - C(-1, 1)) + FABS(C(1, -1) - C(3, -3)) + FABS(G(0, 0) - G(2, -2));
+      const real SWg = eps + FABS(C(1, -1) - C(-1, 1)) + FABS(C(-1, 1) - C(-3, 3)) + FABS(G(0, 0) - G(-2, 2));
+      const real SEg = eps + FABS(C(-1, -1) - C(1, 1)) + FABS(C(1, 1) - C(3, 3)) + FABS(G(0, 0) - G(2, 2));
+      const real NWe = C(-1, -1) - G(-1, -1), NEe = C(1, -1) - G(1, -1);
+      const real SWe = C(-1, 1) - G(-1, 1), SEe = C(1, 1) - G(1, 1);
+      const real pe = (NWg * SEe + SEg * NWe) / (NWg + SEg);
+      const real qe = (NEg * SWe + SWg * NEe) / (NEg + SWg);
+      const real v = rcd_h16(FMIN(FMAX(G(0, 0) + mixr(pe, qe, d), R(0)), R(65535.0)));
+      (red ? sb : sr)[i] = v;
+#undef PQ
+#undef C
+    }
+    for(int ly = 0; ly < 32; ly++) for(int lx = 0; lx < 64; lx += 2)
+    { // red and blue at the green site
+      const int k = rcd_col(PX(lx), PY(ly)) == 1 ? 0 : 1;
+      const int x = PX(lx) + k, y = PY(ly), i = 64 * ly + lx + k;
+#define VH(dx, dy) rcd_c(vh, x + (dx), y + (dy))
+      const real vhc = VH(0, 0), vhn = R(0.25) * (VH(-1, -1) + VH(1, -1) + VH(-1, 1) + VH(1, 1));
+      const real d = rcd_discr(vhc, vhn);
+#undef VH
+      const real N1 = eps + FABS(G(0, 0) - G(0, -2)), S1 = eps + FABS(G(0, 0) - G(0, 2));
+      const real W1 = eps + FABS(G(0, 0) - G(-2, 0)), E1 = eps + FABS(G(0, 0) - G(2, 0));
+      real res[2];
+      for(int ch = 0; ch < 2; ch++)
+      {
+        const real *a = ch == 0 ? sr : sb;
+#define C(dx, dy) rcd_sh(a, i + 64 * (dy) + (dx))
+        const real SN = FABS(C(0, -1) - C(0, 1)), EW = FABS(C(-1, 0) - C(1, 0));
+        const real Ng = N1 + SN + FABS(C(0, -1) - C(0, -3)), Sg = S1 + SN + FABS(C(0, 1) - C(0, 3));
+        const real Wg = W1 + EW + FABS(C(-1, 0) - C(-3, 0)), Eg = E1 + EW + FABS(C(1, 0) - C(3, 0));
+        const real Ne = C(0, -1) - G(0, -1), Se = C(0, 1) - G(0, 1), We = C(-1, 0) - G(-1, 0), Ee = C(1, 0) - G(1, 0);
+        const real ve = (Ng * Se + Sg * Ne) / (Ng + Sg), he = (Eg * We + Wg * Ee) / (Eg + Wg);
+        res[ch] = FMIN(FMAX(G(0, 0) + mixr(ve, he, d), R(0)), R(65535.0));
+#undef C
+      }
+      // stored after both passes: a pass reads this site's own channel of
+      // neither colour (the neighbours and the sites three away only)
+      sr[i] = rcd_h16(res[0]);
+      sb[i] = rcd_h16(res[1]);
+#undef G
+    }
+    for(int ly = 3; ly < 29; ly++) for(int lx = 3; lx < 61; lx++)
+    {
+      const int x = PX(lx), y = PY(ly), i = 64 * ly + lx;
+      if(x >= out->wd || y >= out->ht) continue;
+      const v4 o = { sr[i] / wb[0], sg[i] / wb[1], sb[i] / wb[2], R(1.0) };
+      store(out, x, y, 0, o);
+    }
+#undef PX
+#undef PY
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
  * colenc/main (colenc/main.comp:15-79): rec2020 -> export primaries + trc
  * params: prim, trc (int)
  * --------------------------------------------------------------------- */
@@ -1598,6 +1767,8 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("hilite", "reduce"))    return nconn >= 2 ? hl_reduce(pcf, par, conn) : -1;
   if(IS("hilite", "assemble"))  return nconn >= 3 ? hl_assemble(par, conn) : -1;
   if(IS("hilite", "doub"))      return nconn >= 3 ? hl_doub(pci, par, conn, wd, ht) : -1;
+  if(IS("demosaic", "rcd_conv")) return nconn >= 4 ? dm_rcd_conv(conn) : -1;
+  if(IS("demosaic", "rcd_fill")) return nconn >= 5 ? dm_rcd_fill(pcf, conn, wd, ht) : -1;
   if(IS("colenc", "main"))      return nconn >= 2 ? co_colenc((const int32_t *)params, conn) : -1;
   if(IS("demosaic", "halfsize")) return nconn >= 2 ? dm_halfsize(pci, conn) : -1;
   if(IS("shared", "resample"))  return nconn >= 2 ? sh_resample(conn) : -1;

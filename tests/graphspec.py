@@ -71,7 +71,7 @@ class DevelopParams:
     detail: float = 1.0
     edges: tuple = (0.0, 0.0, 0.0, 0.0)
     dm_colour: int = 0
-    dm_method: int = 0          # 0 gaussian splats, 2 half size
+    dm_method: int = 0          # 0 gaussian splats, 1 RCD, 2 half size
     exposure: float = 0.0
     co_white: tuple = (0.0, 0.0, 0.0)
     light: float = 3.0
@@ -226,6 +226,18 @@ def build(W, H, p, max_wd=0, max_ht=0, prim=2, trc=0):
         else:
             bufs["dm"] = bufs.pop("dmh")
             nodes[-1]["conn"] = [mos, "dm"]
+    elif p.dm_method == 1:   # RCD (demosaic/main.c:112-155)
+        buf("vh", W, H, 1, F16)
+        buf("pq", W // 2, H, 1, F16)
+        buf("lp", W // 2, H, 1, F16)
+        add("demosaic", "rcd_conv", None, dmpar, [mos, "vh", "pq", "lp"], W, H)
+        fill_out = "dm" if (ow, oh) == (W, H) else "dmf"
+        buf(fill_out, W, H, 4, F16)
+        nx, ny = (W + 57) // 58 * 8, (H + 25) // 26 * 8
+        add("demosaic", "rcd_fill", dmpc, dmpar, [mos, "vh", "pq", "lp", fill_out], nx, ny)
+        if fill_out != "dm":
+            buf("dm", ow, oh, 4, F16)
+            add("shared", "resample", None, None, ["dmf", "dm"], ow, oh)
     else:
         buf("cov", Qw, Qh, 4, F16)
         add("demosaic", "gauss", dmpc, dmpar, [mos, mos, "cov"], Qw, Qh)

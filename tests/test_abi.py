@@ -38,9 +38,14 @@ def test_abi_version_and_registry():
     for m, k in [("denoise", "half"), ("denoise", "downcov"), ("denoise", "down"), ("denoise", "assemble"),
                  ("denoise", "doub"), ("denoise", "noop"), ("demosaic", "gauss"), ("demosaic", "splat"),
                  ("demosaic", "fix"), ("demosaic", "down"), ("colour", "main"), ("filmcurv", "main"),
-                 ("llap", "curve"), ("llap", "reduce"), ("llap", "assemble"), ("llap", "colour")]:
+                 ("llap", "curve"), ("llap", "reduce"), ("llap", "assemble"), ("llap", "colour"),
+                 ("hilite", "half"), ("hilite", "reduce"), ("hilite", "assemble"), ("hilite", "doub"),
+                 ("rawhist", "collect"), ("rawhist", "map"), ("demosaic", "halfsize"), ("shared", "resample"),
+                 ("colenc", "main"), ("demosaic", "rcd_conv"), ("demosaic", "rcd_fill")]:
         assert lib.vkdt_hip_node_supported(tok(m), tok(k)) == 1, (m, k)
-    assert lib.vkdt_hip_node_supported(tok("demosaic"), tok("rcd_conv")) == 0
+    # kernels of modules outside this path are refused by name
+    assert lib.vkdt_hip_node_supported(tok("demosaic"), tok("coldumb")) == 0
+    assert lib.vkdt_hip_node_supported(tok("i-mcraw"), tok("decode")) == 0
 
 
 def test_launch_rejects_bad_nodes_without_touching_the_device():

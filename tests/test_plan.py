@@ -31,6 +31,8 @@ PLAN_CASES = [("bayer", 200, 150, False, 0, 0, 0), ("xtrans", 201, 150, False, 0
               ("xtrans", 201, 150, False, 2, 0, 0),     # method 2: half size + resample
               ("bayer", 200, 150, False, 2, 0, 0),      # method 2 on the block grid
               ("bayer", 200, 150, False, 0, 0, 0, 1, 2),    # sRGB export: colenc
+              ("bayer", 200, 150, False, 1, 0, 0),      # RCD
+              ("bayer", 200, 150, True, 1, 150, 0),     # RCD behind hilite, scaled: + resample
               ("xtrans", 201, 150, True, 0, 80, 0, 4, 3)]   # P3 + PQ, hilite, scaled
 
 

@@ -100,6 +100,8 @@ static const reg_entry g_registry[] = {
   { tkn("demosaic"), tkn("halfsize"), launch_demosaic_halfsize },
   { tkn("shared"),   tkn("resample"), launch_shared_resample },
   { tkn("colenc"),   tkn("main"),     launch_colenc_main },
+  { tkn("demosaic"), tkn("rcd_conv"), launch_demosaic_rcd_conv },
+  { tkn("demosaic"), tkn("rcd_fill"), launch_demosaic_rcd_fill },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)

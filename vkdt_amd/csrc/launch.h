@@ -93,3 +93,5 @@ int launch_rawhist_map(const node_call &);
 int launch_demosaic_halfsize(const node_call &);
 int launch_shared_resample(const node_call &);
 int launch_colenc_main(const node_call &);
+int launch_demosaic_rcd_conv(const node_call &);
+int launch_demosaic_rcd_fill(const node_call &);
