@@ -139,6 +139,19 @@ DEV float fetch_m(const dimg &b, int x, int y)
 DEV float centre_m(const dimg &b, int x, int y) { return ld_m(b, mirror(x, b.wd), mirror(y, b.ht)); }
 DEV float  centre_u16(const dimg &b, int x, int y) { return ld_u16(b, mirror(x, b.wd), mirror(y, b.ht)); }
 
+// the same with the format fixed at compile time (U16: R16_UNORM, else f16),
+// for the hot kernels that are instantiated per mosaic format
+template<int U16> DEV float ld_mt(const dimg &b, int x, int y) { return U16 ? ld_u16(b, x, y) : ld_h1(b, x, y); }
+template<int U16> DEV float fetch_mt(const dimg &b, int x, int y)
+{
+  if(x < 0 || y < 0 || x >= b.wd || y >= b.ht) return 0.0f;
+  return ld_mt<U16>(b, x, y);
+}
+template<int U16> DEV float centre_mt(const dimg &b, int x, int y)
+{
+  return ld_mt<U16>(b, mirror(x, b.wd), mirror(y, b.ht));
+}
+
 // texture() at texture-space (s,t): taps floor(s-.5), +1, weights frac
 DEV float4 bilin_h4(const dimg &b, float s, float t)
 {

@@ -41,12 +41,12 @@ __global__ void __launch_bounds__(256) k_dm_down(dimg in, dimg out, dm_args a)
   st_h1(out, x, y, lum);
 }
 
-template<int XT>
+template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, dm_gauss_px<XT>(orig, x, y));
+  st_h4(out, x, y, dm_gauss_px<XT, U16>(orig, x, y));
 }
 
 // texel of this thread: Bayer tiles are handed out colour-uniform per wave
@@ -58,22 +58,22 @@ DEV int2 cfa_xy()
   return bayer_wave_xy(t.x * BX, t.y * BY);
 }
 
-template<int XT>
+template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 {
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  st_h1(out, x, y, dm_splat_px<XT>(in, ga, x, y));
+  st_h1(out, x, y, dm_splat_px<XT, U16>(in, ga, x, y));
 }
 
-template<int XT>
+template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg out, int r)
 {
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, dm_fix_px<XT>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
+  st_h4(out, x, y, dm_fix_px<XT, U16>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
 }
 
 // ---- halfsize (halfsize.comp:19-50): one rgba texel per CFA block --------
@@ -109,6 +109,10 @@ __global__ void __launch_bounds__(256) k_dm_halfsize(dimg in, dimg out)
 
 // ---- host launchers --------------------------------------------------------
 
+// the <XT, U16> instantiation of a demosaic kernel: CFA and mosaic format
+#define DM_PICK(kern, xt, u16) \
+  ((xt) ? ((u16) ? kern<1, 1> : kern<1, 0>) : ((u16) ? kern<0, 1> : kern<0, 0>))
+
 static int decode(dm_args &a, const node_call &c, const char *what)
 {
   if(int e = check_blob(c, 5 * 4, 0, what)) return e;
@@ -134,12 +138,9 @@ int launch_demosaic_gauss(const node_call &c)
   if(int e = decode(a, c, "demosaic/gauss")) return e;
   if(int e = check_conn2(c, 1, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "demosaic/gauss orig")) return e;
   if(int e = check_conn(c, 2, VKDT_FMT_F16, 4, 1, "demosaic/gauss output")) return e;
-  if(a.filters == 9)
-    hipLaunchKernelGGL(k_dm_gauss<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[1]), to_dimg(c.conn[2]));
-  else
-    hipLaunchKernelGGL(k_dm_gauss<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[1]), to_dimg(c.conn[2]));
+  const int u16 = c.conn[1].format == VKDT_FMT_UI16;
+  hipLaunchKernelGGL(DM_PICK(k_dm_gauss, a.filters == 9, u16), grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                     to_dimg(c.conn[1]), to_dimg(c.conn[2]));
   return launch_post("demosaic/gauss");
 }
 
@@ -153,12 +154,9 @@ int launch_demosaic_splat(const node_call &c)
   // the +-6 border wrap of eval_gauss stays inside images of at least 6x6
   if(c.conn[0].wd < 6 || c.conn[0].ht < 6)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "demosaic/splat: input smaller than 6x6");
-  if(a.filters == 9)
-    hipLaunchKernelGGL(k_dm_splat<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]));
-  else
-    hipLaunchKernelGGL(k_dm_splat<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]));
+  const int u16 = c.conn[0].format == VKDT_FMT_UI16;
+  hipLaunchKernelGGL(DM_PICK(k_dm_splat, a.filters == 9, u16), grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]));
   return launch_post("demosaic/splat");
 }
 
@@ -173,12 +171,9 @@ int launch_demosaic_fix(const node_call &c)
   int r;
   if(a.filters == 9) { r = a.fixup + 2; r = r < 2 ? 2 : (r > 3 ? 3 : r); }
   else               { r = a.fixup + 1; r = r < 1 ? 1 : (r > 2 ? 2 : r); }
-  if(a.filters == 9)
-    hipLaunchKernelGGL(k_dm_fix<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), r);
-  else
-    hipLaunchKernelGGL(k_dm_fix<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), r);
+  const int u16 = c.conn[0].format == VKDT_FMT_UI16;
+  hipLaunchKernelGGL(DM_PICK(k_dm_fix, a.filters == 9, u16), grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), r);
   return launch_post("demosaic/fix");
 }
 

@@ -424,7 +424,7 @@ DEV float dn_doub_px(const dimg &in, const dimg &crs0, const dimg &crs1b, int ow
 // and feed a 2x2 eigen solve whose discriminant vanishes on flat or clipped
 // blocks: they keep IEEE division so the decision follows the written
 // arithmetic.
-template<int XT>
+template<int XT, int U16 = 0>
 DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
 {
   constexpr int n0 = XT ? 0 : -1, n1 = 3, bs = XT ? 3 : 2;
@@ -439,7 +439,7 @@ DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
     for(int i = n0; i < n1; i++)
     {
       if(XT ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
-      px[n] = centre_m(orig, bs * x + i, bs * y + j);
+      px[n] = centre_mt<U16>(orig, bs * x + i, bs * y + j);
       rp[n] = 1.0f / px[n];
       pi[n] = i; pj[n] = j;
       n++;
@@ -559,10 +559,10 @@ DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
   return g * frcp(fmaxf(1e-8f, w));
 }
 
-template<int XT>
+template<int XT, int U16 = 0>
 DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
 {
-  return dm_splat_at<XT>([&](int px, int py) { return fetch_m(in, px, py); }, in.wd, in.ht, ga, x, y);
+  return dm_splat_at<XT>([&](int px, int py) { return fetch_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y);
 }
 
 // fix.comp:21-135: rgb at (x,y) from the mosaic, the green plane (through
@@ -651,10 +651,10 @@ DEV float4 dm_fix_r(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt gree
   return dm_fix_loop<XT>(mos, cv, r, x, y, green);
 }
 
-template<int XT, typename GreenAt>
+template<int XT, int U16 = 0, typename GreenAt>
 DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
 {
-  return dm_fix_r<XT>([&](int px, int py) { return centre_m(in, px, py); }, cv, r, x, y, green);
+  return dm_fix_r<XT>([&](int px, int py) { return centre_mt<U16>(in, px, py); }, cv, r, x, y, green);
 }
 
 // ==========================================================================
