@@ -136,6 +136,42 @@ int vkdt_hip_event_query(void *event);
 // Human readable text of the last error on this thread.
 const char *vkdt_hip_last_error(void);
 
+// ---- lossless JPEG raw data (compressed DNG, Compression = 7) -------------
+// One stream per DNG tile or strip (ITU-T T.81 process 14).  The reference
+// decodes these on the host inside rawler (i-raw/rawloader-c, a dependency
+// outside its tree) and liblj92 (i-mlv/liblj92/lj92.c).  The scan's samples
+// fill the tile row-major; sites outside the image are dropped.
+typedef struct vkdt_lj92_tile_t
+{
+  uint64_t offset, length;    // the stream's bytes in the data buffer
+  uint32_t x0, y0;            // the tile's first site in the image
+  uint32_t tile_wd, tile_ht;  // its extent (TileWidth/TileLength, or width x RowsPerStrip)
+  uint32_t image;             // index into the image list (GPU decode)
+}
+vkdt_lj92_tile_t;
+
+typedef struct vkdt_lj92_image_t
+{
+  void    *out;               // device: uint16 mosaic
+  uint32_t wd, ht, pitch;     // extent and texels per row
+}
+vkdt_lj92_image_t;
+
+// Frame of one stream: samples per line per component, lines, components, bits.
+int vkdt_lj92_info(const uint8_t *data, size_t length, uint32_t *wd, uint32_t *ht, uint32_t *comp, uint32_t *bits);
+
+// Decode one tile on the host into `out` (wd x ht, `pitch` texels per row):
+// i-raw's path, taking over the decode inside rawler::decode_file (rawloader-c/lib.rs:102-124).
+int vkdt_lj92_decode_host(const uint8_t *data, const vkdt_lj92_tile_t *tile, uint16_t *out,
+                          uint32_t wd, uint32_t ht, uint32_t pitch);
+
+// Decode n tiles of nimg images on the GPU, one lane per tile: the host
+// parses the headers and builds the code tables, uploads the compressed
+// bytes (`data`, data_len) and waits for the decode on `stream`.  No
+// reference counterpart (the reference unpacks on the host).
+int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const vkdt_lj92_tile_t *tiles, int n,
+                         const vkdt_lj92_image_t *img, int nimg, void *stream);
+
 // ---- batched develop ------------------------------------------------------
 // The raw-development graph i-raw -> denoise -> demosaic -> colour ->
 // filmcurv -> llap of many stills or frames, of any sizes and CFAs, in one

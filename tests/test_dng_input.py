@@ -69,8 +69,12 @@ def test_rejects_bad_files(tmp_path, iraw):
     rc, _, err = load(iraw, notdng)
     assert rc != 0 and "TIFF" in err
     comp = str(tmp_path / "c.dng")
-    synth.write_dng(comp, cfa, extra_tags=[(259, 3, [7])])   # lossless JPEG
+    synth.write_dng(comp, cfa, extra_tags=[(259, 3, [7])])   # lossless JPEG tag over plain samples
     rc, _, err = load(iraw, comp)
+    assert rc != 0 and "lj92" in err
+    lossy = str(tmp_path / "l.dng")
+    synth.write_dng(lossy, cfa, extra_tags=[(259, 3, [34892])])   # lossy JPEG
+    rc, _, err = load(iraw, lossy)
     assert rc != 0 and "compression" in err
     rc, _, err = load(iraw, str(tmp_path / "missing.dng"))
     assert rc != 0

@@ -26,6 +26,17 @@ class Buf(C.Structure):
     ]
 
 
+class Lj92Tile(C.Structure):
+    """vkdt_lj92_tile_t"""
+    _fields_ = [("offset", C.c_uint64), ("length", C.c_uint64), ("x0", C.c_uint32), ("y0", C.c_uint32),
+                ("tile_wd", C.c_uint32), ("tile_ht", C.c_uint32), ("image", C.c_uint32)]
+
+
+class Lj92Image(C.Structure):
+    """vkdt_lj92_image_t"""
+    _fields_ = [("out", C.c_void_p), ("wd", C.c_uint32), ("ht", C.c_uint32), ("pitch", C.c_uint32)]
+
+
 def token(s):
     """dt_token(): up to 8 chars packed little endian into a uint64."""
     b = s.encode()[:8]
@@ -66,6 +77,9 @@ def lib():
         L.vkdt_hip_event_record.argtypes = [vp, vp]
         L.vkdt_hip_event_elapsed_ms.argtypes = [C.POINTER(C.c_float), vp, vp]
         L.vkdt_hip_event_query.argtypes = [vp]
+        L.vkdt_lj92_info.argtypes = [vp, sz] + [C.POINTER(C.c_uint32)] * 4
+        L.vkdt_lj92_decode_host.argtypes = [vp, C.POINTER(Lj92Tile), vp, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.vkdt_hip_lj92_decode.argtypes = [vp, sz, C.POINTER(Lj92Tile), C.c_int, C.POINTER(Lj92Image), C.c_int, vp]
         L.vkdt_hip_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -182,3 +196,30 @@ class Stream:
 
     def sync(self):
         sync(self.ptr)
+
+
+def lj92_tiles(streams, tw, th, W, H, image=0):
+    """(bytes, [Lj92Tile]) for row-major tile streams covering a W x H image"""
+    ntx = -(-W // tw)
+    tiles, blob, off = [], bytearray(), 0
+    for k, s in enumerate(streams):
+        tiles.append(Lj92Tile(off, len(s), (k % ntx) * tw, (k // ntx) * th, tw, th, image))
+        blob += s
+        off += len(s)
+    return bytes(blob), tiles
+
+
+def lj92_decode_host(blob, tile, W, H):
+    """one stream on the host: uint16 H x W (zero where the tile does not reach)"""
+    out = np.zeros((H, W), np.uint16)
+    buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+    check(lib().vkdt_lj92_decode_host(buf, C.byref(tile), out.ctypes.data, W, H, W), "lj92 host decode")
+    return out
+
+
+def lj92_decode(blob, tiles, images, stream=None):
+    """tiles of several images on the GPU; images: [(device ptr, W, H, pitch)]"""
+    buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+    t = (Lj92Tile * len(tiles))(*tiles)
+    im = (Lj92Image * len(images))(*[Lj92Image(p, w, h, pi) for p, w, h, pi in images])
+    check(lib().vkdt_hip_lj92_decode(buf, len(blob), t, len(tiles), im, len(images), stream), "lj92 decode")

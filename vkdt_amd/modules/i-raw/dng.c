@@ -5,10 +5,12 @@
  * (NewSubFileType 0, PhotometricInterpretation CFA), and extracts the tags
  * i-raw needs: extent, CFA pattern, black/white levels, colour matrices
  * with their calibration illuminants, AsShotNeutral and DefaultCrop.
- * Only 16-bit uncompressed strips are accepted; anything else is refused
- * with a message (compressed DNGs are a later raw-input format).
+ * Raw data: 16-bit uncompressed strips, or lossless-JPEG compressed
+ * (Compression 7) tiles or strips, decoded on the host one stream at a time
+ * (vkdt_lj92_decode_host); anything else is refused with a message.
  */
 #include "dng.h"
+#include "vkdt_hip.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -185,8 +187,32 @@ int dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen)
   const int comp = (int)tagv(&t, ri, 0, 259, 0, 1);
   const int spp = (int)tagv(&t, ri, 0, 277, 0, 1);
   if(W == 0 || H == 0 || W > 65535 || H > 65535) { snprintf(err, errlen, "%s: bad extent %ux%u", filename, W, H); goto done; }
-  if(comp != 1) { snprintf(err, errlen, "%s: compression %d not supported (uncompressed only)", filename, comp); goto done; }
-  if(bps != 16 || spp != 1) { snprintf(err, errlen, "%s: %d bits x %d samples not supported (16 bit CFA only)", filename, bps, spp); goto done; }
+  if(comp != 1 && comp != 7) { snprintf(err, errlen, "%s: compression %d not supported (none or lossless JPEG)", filename, comp); goto done; }
+  if(spp != 1 || (comp == 1 && bps != 16) || bps > 16)
+  { snprintf(err, errlen, "%s: %d bits x %d samples not supported (16 bit CFA, or lossless JPEG up to 16 bit)", filename, bps, spp); goto done; }
+  if(comp == 7)
+  { // lossless JPEG: one stream per tile (TileOffsets) or per strip
+    const entry_t *to = find(ri, 324), *tcnt = find(ri, 325);
+    const int tiled = to && tcnt;
+    if(!tiled) { to = find(ri, 273); tcnt = find(ri, 279); }
+    if(!to || !tcnt || to->cnt != tcnt->cnt) { snprintf(err, errlen, "%s: no tiles or strips", filename); goto done; }
+    const uint32_t tw = tiled ? (uint32_t)tagv(&t, ri, 0, 322, 0, 0) : W;
+    const uint32_t th = tiled ? (uint32_t)tagv(&t, ri, 0, 323, 0, 0) : (uint32_t)tagv(&t, ri, 0, 278, 0, H);
+    if(tw == 0 || th == 0) { snprintf(err, errlen, "%s: bad tile extent %ux%u", filename, tw, th); goto done; }
+    const uint32_t ntx = (W + tw - 1) / tw, nty = (H + th - 1) / th;
+    if(to->cnt < ntx * nty) { snprintf(err, errlen, "%s: %u tiles for %ux%u", filename, to->cnt, ntx, nty); goto done; }
+    img->data = calloc((size_t)W * H, 2);
+    if(!img->data) { snprintf(err, errlen, "out of memory"); goto done; }
+    for(uint32_t k = 0; k < ntx * nty; k++)
+    {
+      const vkdt_lj92_tile_t tile = { (uint64_t)ent_val(&t, to, k), (uint64_t)ent_val(&t, tcnt, k),
+                                      (k % ntx) * tw, (k / ntx) * th, tw, th, 0 };
+      if(tile.offset + tile.length > t.n) { snprintf(err, errlen, "%s: truncated tile %u", filename, k); goto done; }
+      if(vkdt_lj92_decode_host(buf, &tile, img->data, W, H, W))
+      { snprintf(err, errlen, "%s: tile %u: %s", filename, k, vkdt_hip_last_error()); goto done; }
+    }
+    goto decoded;
+  }
   const entry_t *so = find(ri, 273), *sc = find(ri, 279);
   if(!so || !sc || so->cnt != sc->cnt) { snprintf(err, errlen, "%s: no strips (tiled DNGs are not supported)", filename); goto done; }
   const uint32_t rps = (uint32_t)tagv(&t, ri, 0, 278, 0, H);
@@ -207,6 +233,7 @@ int dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen)
     }
     if(row < H) { snprintf(err, errlen, "%s: strips cover %u of %u rows", filename, row, H); goto done; }
   }
+decoded:
   img->width = W;
   img->height = H;
   img->stride = W;

@@ -133,14 +133,25 @@ def _rational(v, signed):
 
 def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(1.0, 1.0, 1.0),
               color_matrix=None, make="vkdt-synth", model="synthetic", pattern=None, illuminant=21,
-              extra_tags=()):
-    """Write ``cfa`` (uint16 H x W) as an uncompressed CFA DNG.
+              extra_tags=(), compress=None):
+    """Write ``cfa`` (uint16 H x W) as a CFA DNG, uncompressed or (``compress``
+    a dict with tile=(wd, ht), components, predictor, bits) as lossless-JPEG
+    tiles (Compression 7), edge tiles padded by repeating the last row/column.
 
     ``pattern`` overrides the CFA layout (a 2x2 or 6x6 array of 0/1/2),
     ``illuminant`` is CalibrationIlluminant1 (21 = D65), ``extra_tags`` are
     (tag, type, values) triples appended to the IFD (for negative tests)."""
     import struct
     H, W = cfa.shape
+    chunks = None
+    if compress is not None:
+        from . import lj92
+        tw, th = compress.get("tile", (256, 256))
+        ntx, nty = -(-W // tw), -(-H // th)
+        padded = np.pad(np.asarray(cfa, np.uint16), ((0, nty * th - H), (0, ntx * tw - W)), mode="edge")
+        chunks = [lj92.encode(padded[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw], compress.get("bits", 16),
+                              compress.get("predictor", 1), compress.get("components", 2))
+                  for ty in range(nty) for tx in range(ntx)]
     pat = cfa_pattern(kind) if pattern is None else np.asarray(pattern, np.uint8)
     n = pat.shape[0]
     cm = xyz_to_cam() if color_matrix is None else np.asarray(color_matrix, np.float64)
@@ -163,15 +174,21 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
     add(254, 4, [0])                      # NewSubFileType: main image
     add(256, 4, [W])                      # ImageWidth
     add(257, 4, [H])                      # ImageLength
-    add(258, 3, [16])                     # BitsPerSample
-    add(259, 3, [1])                      # Compression: none
+    add(258, 3, [compress.get("bits", 16) if chunks else 16])   # BitsPerSample
+    add(259, 3, [7 if chunks else 1])     # Compression: lossless JPEG / none
     add(262, 3, [32803])                  # PhotometricInterpretation: CFA
     add(271, 2, make)                     # Make
     add(272, 2, model)                    # Model
-    add(273, 4, [0])                      # StripOffsets (patched)
+    if chunks:
+        add(322, 4, [tw])                 # TileWidth
+        add(323, 4, [th])                 # TileLength
+        add(324, 4, [0] * len(chunks))    # TileOffsets (patched)
+        add(325, 4, [len(c) for c in chunks])   # TileByteCounts
+    else:
+        add(273, 4, [0])                  # StripOffsets (patched)
+        add(278, 4, [H])                  # RowsPerStrip
+        add(279, 4, [strip_bytes])        # StripByteCounts
     add(277, 3, [1])                      # SamplesPerPixel
-    add(278, 4, [H])                      # RowsPerStrip
-    add(279, 4, [strip_bytes])            # StripByteCounts
     add(284, 3, [1])                      # PlanarConfiguration
     add(33421, 3, [n, n])                 # CFARepeatPatternDim
     add(33422, 1, [int(v) for v in pat.ravel()])  # CFAPattern
@@ -203,6 +220,17 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
         else:
             fixed.append((tag, typ, cnt, data.ljust(4, b"\0"), None))
     strip_off = (cur + 15) & ~15
+    if chunks:   # tile offsets, laid out after the IFD data
+        offs, o = [], strip_off
+        for c in chunks:
+            offs.append(o)
+            o += len(c)
+        tile_data = struct.pack("<%dI" % len(offs), *offs)
+        if len(tile_data) <= 4:
+            fixed = [(tg, ty, cn, tile_data.ljust(4, b"\0") if tg == 324 else il, of) for tg, ty, cn, il, of in fixed]
+        else:
+            at = [of for tg, ty, cn, il, of in fixed if tg == 324][0]
+            blobs = [(of, tile_data if of == at else d) for of, d in blobs]
     out = bytearray(b"II*\0" + struct.pack("<I", ifd_off))
     out += struct.pack("<H", len(fixed))
     for tag, typ, cnt, inline, off in fixed:
@@ -217,5 +245,9 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
     out += b"\0" * (strip_off - len(out))
     with open(path, "wb") as f:
         f.write(out)
-        f.write(np.ascontiguousarray(cfa, dtype="<u2").tobytes())
+        if chunks:
+            for c in chunks:
+                f.write(c)
+        else:
+            f.write(np.ascontiguousarray(cfa, dtype="<u2").tobytes())
     return path
