@@ -93,6 +93,7 @@ int lj92_parse(const uint8_t *d, size_t n, lj92_hdr *h)
       h->pt = s[3 + 2 * ns] & 15;
       if(h->pred < 1 || h->pred > 7) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: predictor %d", h->pred);
       h->scan = i + 2 + L;
+      if(h->scan >= n) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: no entropy-coded data");
       return VKDT_HIP_OK;
     }
     else if(m == 0xd9) break;
@@ -103,19 +104,41 @@ int lj92_parse(const uint8_t *d, size_t n, lj92_hdr *h)
   return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: no scan");
 }
 
-// canonical codes (T.81 C.2) -> 16-bit lookup: entry = category << 8 | length,
-// 0 for prefixes no code matches; fails on tables whose codes overflow
-int lj92_lut(const uint8_t bits[16], const uint8_t *val, uint16_t *lut)
+// a DC table (T.81 C.2, F.2.2.3) in the form the decoder reads: codes of up
+// to LJ92_FAST bits by one lookup of the stream's next LJ92_FAST bits (entry =
+// category << 8 | length, 0 where the code is longer or matches nothing),
+// longer ones by the canonical bounds per length (maxcode -1 where a length
+// has no codes; category = val[code + delta])
+#define LJ92_FAST 9
+struct __attribute__((aligned(16))) lj92_tab
 {
-  memset(lut, 0, sizeof(uint16_t) << 16);
+  uint16_t fast[1 << LJ92_FAST];
+  int32_t  maxcode[17];
+  int32_t  delta[17];
+  uint8_t  val[20];
+};
+
+int lj92_table(const uint8_t bits[16], const uint8_t *val, lj92_tab *t)
+{
+  memset(t, 0, sizeof(*t));
   int code = 0, k = 0;
   for(int len = 1; len <= 16; len++)
   {
-    for(int j = 0; j < bits[len - 1]; j++, k++, code++)
+    t->maxcode[len] = -1;
+    if(bits[len - 1])
     {
-      if(code >= (1 << len) || val[k] > 16) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: bad Huffman table");
-      const int lo = code << (16 - len), cnt = 1 << (16 - len);
-      for(int e = 0; e < cnt; e++) lut[lo + e] = (uint16_t)((val[k] << 8) | len);
+      t->delta[len] = k - code;
+      for(int j = 0; j < bits[len - 1]; j++, k++, code++)
+      {
+        if(code >= (1 << len) || val[k] > 16) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: bad Huffman table");
+        t->val[k] = val[k];
+        if(len <= LJ92_FAST)
+        {
+          const int lo = code << (LJ92_FAST - len), cnt = 1 << (LJ92_FAST - len);
+          for(int e = 0; e < cnt; e++) t->fast[lo + e] = (uint16_t)((val[k] << 8) | len);
+        }
+      }
+      t->maxcode[len] = code - 1;
     }
     code <<= 1;
   }
@@ -128,40 +151,109 @@ int lj92_lut(const uint8_t bits[16], const uint8_t *val, uint16_t *lut)
 
 struct lj92_job
 {
-  const uint8_t *data;     // entropy-coded segment
-  long     len;
-  const uint16_t *lut[4];  // per component
+  const uint8_t *data;     // entropy-coded segment (readable as whole aligned
+  long     len;            //   words: the buffer is padded past its end)
+  const lj92_tab *tab[4];  // per component
+  int      same;           // bit c: component c uses component 0's table
   int      bits, fwd, fht, comp, pred, pt;
   uint16_t *out;           // the image
   int      img_wd, img_ht, pitch;
   int      x0, y0, tile_wd;
-  uint16_t *row;           // fwd * comp samples: the previous line
+  uint16_t *row;           // fwd * comp samples: the previous line (global memory variant)
 };
 
 // bit reader over a segment with 0xFF00 stuffing; a marker ends the data
-// (zeros are fed past it)
+// (zeros are fed past it).  The stuffed bytes arrive a 32-bit word at a time
+// with the next word already requested, so a lane waits on memory once per
+// four bytes at most.  Words are read only up to the one holding the
+// segment's last byte.
+// loads and stores of global memory reached through pointers held in the
+// job: named as global on the device so they do not count against LDS waits
+template<class T> __host__ __device__ static inline T lj92_ld(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(1))) T *)p;
+#else
+  return *p;
+#endif
+}
+
+template<class T> __host__ __device__ static inline void lj92_st(T *p, T v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  *(__attribute__((address_space(1))) T *)p = v;
+#else
+  *p = v;
+#endif
+}
+
 struct lj92_bits
 {
-  const uint8_t *p;
-  long n, i;
-  uint64_t acc;
-  int cnt;
+  const uint32_t *wp;      // next word to request
+  const uint32_t *wlast;   // the word holding the segment's last byte
+  uint32_t pre;            // requested word as loaded
+  int      pre_ok;         // whether it holds segment bytes (else it counts as zero)
+  uint64_t raw;            // stuffed bytes not yet taken, left-justified
+  int      rawn;           // how many
+  long     left;           // segment bytes from raw's first on
+  uint64_t acc;            // unstuffed bits, left-justified
+  int      cnt;
 };
+
+__host__ __device__ static inline uint32_t lj92_bswap(uint32_t w)
+{
+  return (w >> 24) | ((w >> 8) & 0xff00u) | ((w << 8) & 0xff0000u) | (w << 24);
+}
+
+// requests the next word: always a load (of the last word once past it), so
+// no branch joins on its result and the lane waits only where it is used
+__host__ __device__ static inline void lj92_request(lj92_bits &b)
+{
+  b.pre_ok = b.wp <= b.wlast;
+  b.pre = lj92_ld(b.pre_ok ? b.wp : b.wlast);
+  b.wp++;
+}
+
+__host__ __device__ static inline uint32_t lj92_take(const lj92_bits &b)
+{
+  return b.pre_ok ? lj92_bswap(b.pre) : 0u;
+}
+
+// n >= 1 segment bytes at p
+__host__ __device__ static inline void lj92_init(lj92_bits &b, const uint8_t *p, long n)
+{
+  const int skip = (int)((uintptr_t)p & 3);
+  b.wp = (const uint32_t *)(p - skip);
+  b.wlast = (const uint32_t *)((uintptr_t)(p + n - 1) & ~(uintptr_t)3);
+  lj92_request(b);
+  b.raw = ((uint64_t)lj92_take(b) << 32) << (8 * skip);
+  b.rawn = 4 - skip;
+  lj92_request(b);
+  b.left = n;
+  b.acc = 0;
+  b.cnt = 0;
+}
 
 __host__ __device__ static inline void lj92_fill(lj92_bits &b)
 {
   while(b.cnt <= 56)
   {
+    if(b.rawn <= 4)
+    { // five bytes or more in raw: a 0xFF and the byte after it are both there
+      b.raw |= (uint64_t)lj92_take(b) << (32 - 8 * b.rawn);
+      b.rawn += 4;
+      lj92_request(b);
+    }
     uint32_t byte = 0;
-    if(b.i < b.n)
+    if(b.left > 0)
     {
-      byte = b.p[b.i];
+      byte = (uint32_t)(b.raw >> 56);
       if(byte == 0xff)
       {
-        if(b.i + 1 < b.n && b.p[b.i + 1] == 0x00) b.i += 2;
-        else { byte = 0; b.i = b.n; }   // a marker: the segment ends
+        if(b.left >= 2 && ((b.raw >> 48) & 0xff) == 0) { b.raw <<= 16; b.rawn -= 2; b.left -= 2; }
+        else { byte = 0; b.left = 0; }   // a marker: the segment ends
       }
-      else b.i++;
+      else { b.raw <<= 8; b.rawn--; b.left--; }
     }
     b.acc |= (uint64_t)byte << (56 - b.cnt);
     b.cnt += 8;
@@ -176,42 +268,85 @@ __host__ __device__ static inline int lj92_get(lj92_bits &b, int k)
   return v;
 }
 
-// returns 0, or -1 on a prefix no code matches
-__host__ __device__ static int lj92_run(const lj92_job &jg)
+// next category from table t, -1 on a prefix no code matches
+__host__ __device__ static inline int lj92_symbol(lj92_bits &b, const lj92_tab *t)
+{
+  const int peek = (int)(b.acc >> (64 - LJ92_FAST));
+  const uint16_t e = t->fast[peek];
+  if(e)
+  {
+    lj92_get(b, e & 0xff);
+    return e >> 8;
+  }
+  const int p16 = (int)(b.acc >> 48);
+  for(int l = LJ92_FAST + 1; l <= 16; l++)
+  {
+    const int code = p16 >> (16 - l);
+    if(code <= t->maxcode[l])
+    {
+      lj92_get(b, l);
+      return t->val[code + t->delta[l]];
+    }
+  }
+  return -1;
+}
+
+typedef unsigned int lj92_u32x4 __attribute__((ext_vector_type(4)));
+
+// every 16-bit half of w shifted left by pt (the point transform)
+__host__ __device__ static inline unsigned lj92_pt2(unsigned w, int pt)
+{
+  return ((w << pt) & 0xffffu) | ((((w >> 16) << pt) & 0xffffu) << 16);
+}
+
+// decodes one stream into the image; row holds a scan line (fwd * comp
+// samples): the previous one while a line decodes, which it then replaces.
+// Each decoded line is written out after it completes, in 16-byte stores
+// where the tile's lines allow: stores interleaved with the decode would
+// make every wait for the next stream word wait for them too.
+// With SAME every component decodes with tab0 (a copy of their common table
+// in the caller's fastest memory), else with the job's table per component.
+// Returns 0, or -1 on corrupt data.
+template<int SAME>
+__host__ __device__ static inline int lj92_run(const lj92_job &jg, const lj92_tab *tab0, uint16_t *row)
 {
   const lj92_job j = jg;   // in registers: the stores below cannot alias it
-  lj92_bits b = { j.data, j.len, 0, 0, 0 };
-  const int C = j.comp, W = j.fwd;
+  const lj92_tab *const *tab = jg.tab;   // indexed per component: read where it lies
+  lj92_bits b;
+  lj92_init(b, j.data, j.len);
+  const int C = j.comp, W = j.fwd, L = W * C;
   const int base = 1 << (j.bits - j.pt - 1);
-  int tx = 0, ty = 0;      // tile site of the next sample
+  int tx = 0, ty = 0;      // tile site of the next line's first sample
+  long o = (long)j.y0 * j.pitch + j.x0;
+  const bool inside = j.x0 + j.tile_wd <= j.img_wd;   // every column of the tile lands in the image
+  // a scan line is a tile line that lands whole, on 16-byte boundaries
+  const bool vec = L == j.tile_wd && inside && (L & 7) == 0 && (j.pitch & 7) == 0 &&
+                   (((uintptr_t)(j.out + o)) & 15) == 0 && (((uintptr_t)row) & 15) == 0;
   for(int r = 0; r < j.fht; r++)
   {
-    int left[4] = { 0, 0, 0, 0 }, ul[4] = { 0, 0, 0, 0 };
+    int left0 = 0, left1 = 0, left2 = 0, left3 = 0, ul0 = 0, ul1 = 0, ul2 = 0, ul3 = 0;
     for(int x = 0; x < W; x++)
       for(int c = 0; c < C; c++)
       {
         if(b.cnt < 32) lj92_fill(b);
-        const uint16_t e = j.lut[c][(int)(b.acc >> 48)];
-        if(!e) return -1;
-        lj92_get(b, e & 0xff);
-        const int t = e >> 8;
+        const int t = lj92_symbol(b, SAME ? tab0 : lj92_ld(tab + c));
+        if(t < 0) return -1;
         int diff;
         if(t == 0) diff = 0;
         else if(t == 16) diff = 32768;
         else
-        {
-          if(b.cnt < 32) lj92_fill(b);
+        { // the fill above left 32 bits or more: a code of up to 16 and up to 15 more
           diff = lj92_get(b, t);
           if(diff < (1 << (t - 1))) diff -= (1 << t) - 1;
         }
         const int k = x * C + c;
-        const int up = j.row[k];
+        const int up = row[k];
         int p;
-        if(r == 0) p = x == 0 ? base : left[c];
+        if(r == 0) p = x == 0 ? base : left0;
         else if(x == 0) p = up;
         else
         {
-          const int a = left[c], bb = up, cc = ul[c];
+          const int a = left0, bb = up, cc = ul0;
           switch(j.pred)
           {
             case 1: p = a; break;
@@ -224,23 +359,59 @@ __host__ __device__ static int lj92_run(const lj92_job &jg)
           }
         }
         const int v = (p + diff) & 0xffff;
-        ul[c] = up;
-        left[c] = v;
-        j.row[k] = (uint16_t)v;
-        // the scan fills the tile row-major
-        const int ox = j.x0 + tx, oy = j.y0 + ty;
-        if(ox < j.img_wd && oy < j.img_ht) j.out[(long)oy * j.pitch + ox] = (uint16_t)(v << j.pt);
-        if(++tx == j.tile_wd) { tx = 0; ty++; }
+        // left0/ul0 always hold the next sample's component: rotate by one
+        if(C == 1) { left0 = v; ul0 = up; }
+        else if(C == 2) { left0 = left1; left1 = v; ul0 = ul1; ul1 = up; }
+        else if(C == 3) { left0 = left1; left1 = left2; left2 = v; ul0 = ul1; ul1 = ul2; ul2 = up; }
+        else { left0 = left1; left1 = left2; left2 = left3; left3 = v; ul0 = ul1; ul1 = ul2; ul2 = ul3; ul3 = up; }
+        row[k] = (uint16_t)v;
+      }
+    // the scan fills the tile row-major: this line's L samples from (tx, ty) on
+    if(vec)
+    {
+      if(j.y0 + ty < j.img_ht)
+        for(int k = 0; k < L; k += 8)
+        {
+          lj92_u32x4 w = *(const lj92_u32x4 *)(row + k);
+          if(j.pt) w = lj92_u32x4{ lj92_pt2(w.x, j.pt), lj92_pt2(w.y, j.pt), lj92_pt2(w.z, j.pt), lj92_pt2(w.w, j.pt) };
+          lj92_st((lj92_u32x4 *)(j.out + o + k), w);
+        }
+      ty++;
+      o += j.pitch;
+    }
+    else
+      for(int k = 0; k < L; k++)
+      {
+        if(j.y0 + ty < j.img_ht && (inside || j.x0 + tx < j.img_wd))
+          lj92_st(j.out + o + tx, (uint16_t)(row[k] << j.pt));
+        if(++tx == j.tile_wd) { tx = 0; ty++; o += j.pitch; }
       }
   }
   return 0;
 }
 
+// one lane per stream.  For lines of up to LJ92_ROW samples the lane keeps its
+// line in LDS (ROW_LDS), and for streams whose components share one table
+// the lane's copy of it (SAME), so that the only global loads of the decode
+// are the stream's own words.
+#define LJ92_ROW 512
+template<int ROW_LDS, int SAME>
 __global__ void __launch_bounds__(64) k_lj92(const lj92_job *jobs, int n, int *status)
 {
+  __shared__ lj92_tab tabs[SAME ? 64 : 1];
+  __shared__ uint16_t rows[ROW_LDS ? 64 : 1][ROW_LDS ? LJ92_ROW : 1];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  if(lj92_run(jobs[i])) atomicAdd(status, 1);
+  const lj92_job &j = jobs[i];
+  lj92_tab *t = tabs + (SAME ? threadIdx.x : 0);
+  if(SAME)
+  {
+    const lj92_u32x4 *src = (const lj92_u32x4 *)lj92_ld(j.tab);
+    lj92_u32x4 *dst = (lj92_u32x4 *)t;
+#pragma unroll 8
+    for(int k = 0; k < (int)(sizeof(lj92_tab) / (sizeof(lj92_u32x4))); k++) dst[k] = lj92_ld(src + k);
+  }
+  if(lj92_run<SAME>(j, t, ROW_LDS ? rows[threadIdx.x] : lj92_ld(&j.row))) atomicAdd(status, 1);
 }
 
 // ---- host side ---------------------------------------------------------------
@@ -265,20 +436,27 @@ extern "C" int vkdt_lj92_decode_host(const uint8_t *data, const vkdt_lj92_tile_t
   if((long)h.wd * h.comp * h.ht < (long)t->tile_wd * t->tile_ht || t->tile_wd == 0)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: %dx%dx%d samples for a %ux%u tile", h.wd, h.ht, h.comp,
                           t->tile_wd, t->tile_ht);
-  std::vector<uint16_t> luts((size_t)h.comp << 16), row((size_t)h.wd * h.comp, 0);
+  std::vector<lj92_tab> tabs(h.comp);
+  std::vector<uint16_t> row((size_t)h.wd * h.comp, 0);
   lj92_job j = {};
   for(int c = 0; c < h.comp; c++)
   {
-    if(int e = lj92_lut(h.huff_bits[h.tbl[c]], h.huff_val[h.tbl[c]], luts.data() + ((size_t)c << 16))) return e;
-    j.lut[c] = luts.data() + ((size_t)c << 16);
+    if(int e = lj92_table(h.huff_bits[h.tbl[c]], h.huff_val[h.tbl[c]], &tabs[c])) return e;
+    j.tab[c] = &tabs[c];
+    if(h.tbl[c] == h.tbl[0]) j.same |= 1 << c;
   }
-  j.data = data + t->offset + h.scan;
-  j.len = (long)(t->length - h.scan);
+  // the segment in whole aligned words, as the reader takes it
+  const size_t len = t->length - h.scan;
+  std::vector<uint32_t> seg((len + 3) / 4 + 1, 0);
+  memcpy(seg.data(), data + t->offset + h.scan, len);
+  j.data = (const uint8_t *)seg.data();
+  j.len = (long)len;
   j.bits = h.bits; j.fwd = h.wd; j.fht = h.ht; j.comp = h.comp; j.pred = h.pred; j.pt = h.pt;
   j.out = out; j.img_wd = wd; j.img_ht = ht; j.pitch = pitch;
   j.x0 = t->x0; j.y0 = t->y0; j.tile_wd = t->tile_wd;
   j.row = row.data();
-  if(lj92_run(j)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: corrupt entropy-coded data");
+  const int err = j.same == (1 << h.comp) - 1 ? lj92_run<1>(j, &tabs[0], row.data()) : lj92_run<0>(j, nullptr, row.data());
+  if(err) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: corrupt entropy-coded data");
   return VKDT_HIP_OK;
 }
 
@@ -291,6 +469,10 @@ extern "C" int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const 
   std::vector<std::vector<uint8_t>> tables;
   std::vector<int> tid((size_t)n * 4, 0);
   size_t rows = 0;
+  // four kinds of streams, launched apart: line in LDS or not x one shared
+  // table or not
+  std::vector<int> kind(n);
+  int count[4] = { 0, 0, 0, 0 };
   for(int i = 0; i < n; i++)
   {
     const vkdt_lj92_tile_t &t = tiles[i];
@@ -308,55 +490,77 @@ extern "C" int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const 
       if(it == table_id.end()) { it = table_id.emplace(key, (int)tables.size()).first; tables.push_back(key); }
       tid[4 * i + c] = it->second;
     }
-    rows += (size_t)h.wd * h.comp;
+    int same = 1;
+    for(int c = 1; c < h.comp; c++) same &= tid[4 * i + c] == tid[4 * i];
+    const int lds = h.wd * h.comp <= LJ92_ROW;
+    if(!lds) rows += (size_t)h.wd * h.comp;
+    kind[i] = 2 * lds + same;
+    count[kind[i]]++;
   }
-  // device: [ data | luts | row caches | jobs, status ]
-  const size_t lut_bytes = tables.size() * (sizeof(uint16_t) << 16);
-  const size_t off_lut = (data_len + 255) & ~(size_t)255;
-  const size_t off_row = off_lut + lut_bytes;
+  // device: [ data | tables | row caches | jobs, status ]
+  const size_t tab_bytes = tables.size() * sizeof(lj92_tab);
+  const size_t off_tab = (data_len + 255) & ~(size_t)255;
+  const size_t off_row = off_tab + tab_bytes;
   const size_t off_job = (off_row + rows * 2 + 255) & ~(size_t)255;
   const size_t off_st = off_job + sizeof(lj92_job) * n;
   const size_t total = off_st + 256;
-  std::vector<uint16_t> luts(tables.size() << 16);
+  std::vector<lj92_tab> tabs(tables.size());
   for(size_t k = 0; k < tables.size(); k++)
-    if(int e = lj92_lut(tables[k].data(), tables[k].data() + 16, luts.data() + (k << 16))) return e;
+    if(int e = lj92_table(tables[k].data(), tables[k].data() + 16, &tabs[k])) return e;
   uint8_t *dev = nullptr;
   if(int e = hip_fail(hipMalloc((void **)&dev, total), "lj92 scratch")) return e;
-  // the jobs, then a zero status word (the row caches need no clearing: the
-  // first line reads none)
+  // the jobs grouped by kind, then a zero status word; the row caches need
+  // no clearing: the first line reads none
   std::vector<uint8_t> jobbuf(off_st - off_job + sizeof(int), 0);
   lj92_job *jobs = (lj92_job *)jobbuf.data();
+  int first[4], next[4];
+  for(int k = 0, f = 0; k < 4; f += count[k++]) first[k] = next[k] = f;
   size_t r = 0;
   for(int i = 0; i < n; i++)
   {
     const vkdt_lj92_tile_t &t = tiles[i];
     const lj92_hdr &h = hdr[i];
     const vkdt_lj92_image_t &im = img[t.image];
-    lj92_job &j = jobs[i];
+    lj92_job &j = jobs[next[kind[i]]++];
     j.data = dev + t.offset + h.scan;
     j.len = (long)(t.length - h.scan);
-    for(int c = 0; c < 4; c++) j.lut[c] = (const uint16_t *)(dev + off_lut) + ((size_t)tid[4 * i + (c < h.comp ? c : 0)] << 16);
+    for(int c = 0; c < 4; c++)
+    {
+      const int id = tid[4 * i + (c < h.comp ? c : 0)];
+      j.tab[c] = (const lj92_tab *)(dev + off_tab) + id;
+      if(id == tid[4 * i]) j.same |= 1 << c;
+    }
     j.bits = h.bits; j.fwd = h.wd; j.fht = h.ht; j.comp = h.comp; j.pred = h.pred; j.pt = h.pt;
     j.out = (uint16_t *)im.out; j.img_wd = im.wd; j.img_ht = im.ht; j.pitch = im.pitch;
     j.x0 = t.x0; j.y0 = t.y0; j.tile_wd = t.tile_wd;
-    j.row = (uint16_t *)(dev + off_row) + r;
-    r += (size_t)h.wd * h.comp;
+    j.row = nullptr;
+    if(!(kind[i] & 2))
+    {
+      j.row = (uint16_t *)(dev + off_row) + r;
+      r += (size_t)h.wd * h.comp;
+    }
   }
   hipStream_t s = (hipStream_t)stream;
   int err = hip_fail(hipMemcpyAsync(dev, data, data_len, hipMemcpyHostToDevice, s), "lj92 upload");
-  if(!err) err = hip_fail(hipMemcpyAsync(dev + off_lut, luts.data(), lut_bytes, hipMemcpyHostToDevice, s), "lj92 luts");
+  if(!err) err = hip_fail(hipMemcpyAsync(dev + off_tab, tabs.data(), tab_bytes, hipMemcpyHostToDevice, s), "lj92 tables");
   if(!err) err = hip_fail(hipMemcpyAsync(dev + off_job, jobbuf.data(), jobbuf.size(), hipMemcpyHostToDevice, s),
                           "lj92 jobs");
-  if(!err)
+  const lj92_job *dj = (const lj92_job *)(dev + off_job);
+  int *st = (int *)(dev + off_st);
+  for(int k = 0; k < 4 && !err; k++)
   {
-    hipLaunchKernelGGL(k_lj92, dim3((n + 63) / 64), dim3(64), 0, s, (const lj92_job *)(dev + off_job), n,
-                       (int *)(dev + off_st));
+    if(!count[k]) continue;
+    const dim3 grid((count[k] + 63) / 64), block(64);
+    if(k == 3) hipLaunchKernelGGL((k_lj92<1, 1>), grid, block, 0, s, dj + first[k], count[k], st);
+    else if(k == 2) hipLaunchKernelGGL((k_lj92<1, 0>), grid, block, 0, s, dj + first[k], count[k], st);
+    else if(k == 1) hipLaunchKernelGGL((k_lj92<0, 1>), grid, block, 0, s, dj + first[k], count[k], st);
+    else hipLaunchKernelGGL((k_lj92<0, 0>), grid, block, 0, s, dj + first[k], count[k], st);
     err = launch_post("lj92");
   }
   int bad = 0;
   if(!err) err = hip_fail(hipMemcpyAsync(&bad, dev + off_st, sizeof(int), hipMemcpyDeviceToHost, s), "lj92 status");
   if(!err) err = hip_fail(hipStreamSynchronize(s), "lj92 sync");
-  hipFree(dev);
+  (void)hipFree(dev);
   if(err) return err;
   if(bad) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "lj92: %d tiles with corrupt entropy-coded data", bad);
   return VKDT_HIP_OK;
