@@ -1,8 +1,11 @@
 """Lossless JPEG raw data decoded on the GPU (vkdt_hip_lj92_decode): tiles of
 several images with different components, predictors and tile shapes in one
 launch equal the host decoder (i-raw's path) and the original mosaics bit
-for bit, and a graph reading a compressed DNG develops to the same output as
-the uncompressed one."""
+for bit -- streams with one table and predictor 1 take the many-lanes
+decode (1..4 components, scan lines of one or two tile lines, whole-image
+strips), the others one lane each -- corrupt entropy data is refused by
+both decoders, and a graph reading a compressed DNG develops to the same
+output as the uncompressed one."""
 import os
 import time
 
@@ -14,21 +17,30 @@ from vkdt_amd import hip as H, lj92, pfm, pipe, synth
 pytestmark = pytest.mark.gpu
 
 
-def encode_tiles(cfa, tw, th, comp, pred, bits=14):
+def encode_tiles(cfa, tw, th, comp, pred, bits=14, fold=1):
+    """row-major tile streams; fold > 1 frames each tile as th/fold lines of
+    fold tile lines (the scan fills the tile row-major either way)"""
     Hh, W = cfa.shape
     ntx, nty = -(-W // tw), -(-Hh // th)
     pad = np.pad(cfa, ((0, nty * th - Hh), (0, ntx * tw - W)), mode="edge")
-    return [lj92.encode(pad[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw], bits, pred, comp)
+    return [lj92.encode(pad[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw].reshape(th // fold, tw * fold),
+                        bits, pred, comp)
             for ty in range(nty) for tx in range(ntx)]
 
 
+SPECS = [  # kind, W, H, tile w, tile h, components, predictor, tile lines per scan line
+    ("bayer", 1000, 700, 256, 256, 2, 1, 1), ("xtrans", 990, 660, 240, 128, 1, 6, 1),
+    ("bayer", 640, 480, 640, 16, 4, 7, 1), ("bayer", 512, 384, 132, 96, 3, 4, 1),
+    ("bayer", 700, 500, 128, 128, 1, 1, 1), ("bayer", 520, 390, 132, 64, 3, 1, 1),
+    ("bayer", 610, 300, 256, 64, 4, 1, 1), ("bayer", 600, 400, 128, 128, 2, 1, 2),
+    ("xtrans", 606, 402, 606, 402, 2, 1, 1)]
+
+
 def test_many_images_one_launch(hip):
-    specs = [("bayer", 1000, 700, 256, 256, 2, 1), ("xtrans", 990, 660, 240, 128, 1, 6),
-             ("bayer", 640, 480, 640, 16, 4, 7), ("bayer", 512, 384, 132, 96, 3, 4)]
     blob, tiles, images, bufs, cfas = b"", [], [], [], []
-    for i, (kind, W, Hh, tw, th, comp, pred) in enumerate(specs):
+    for i, (kind, W, Hh, tw, th, comp, pred, fold) in enumerate(SPECS):
         cfa, _ = synth.mosaic(W, Hh, 20 + i, kind)
-        streams = encode_tiles(cfa, tw, th, comp, pred)
+        streams = encode_tiles(cfa, tw, th, comp, pred, fold=fold)
         b, t = H.lj92_tiles(streams, tw, th, W, Hh, image=i)
         for x in t:
             x.offset += len(blob)
@@ -38,7 +50,7 @@ def test_many_images_one_launch(hip):
         bufs.append(buf)
         images.append((buf.ptr, W, Hh, W))
         cfas.append(cfa)
-    print(f"[gpu_lj92] {len(tiles)} tiles of {len(specs)} images, {len(blob)} bytes", flush=True)
+    print(f"[gpu_lj92] {len(tiles)} tiles of {len(SPECS)} images, {len(blob)} bytes", flush=True)
     H.lj92_decode(blob, tiles, images)
     for i, (buf, cfa) in enumerate(zip(bufs, cfas)):
         got = buf.download()
@@ -53,6 +65,29 @@ def test_many_images_one_launch(hip):
                 host[m] = part[m]
         assert np.array_equal(host, cfa), i
         buf.free()
+
+
+@pytest.mark.parametrize("pred", [1, 6])
+def test_corrupt_entropy_data_refused(hip, pred):
+    """entropy data of all ones (stuffed 0xff bytes): the tables reserve that
+    code, so the first sample has no valid code on either path"""
+    W, Hh = 64, 32
+    cfa, _ = synth.mosaic(W, Hh, 5, "bayer")
+    good = lj92.encode(cfa, 14, pred, 2)
+    scan = good.index(b"\xff\xda")
+    scan += 2 + int.from_bytes(good[scan + 2:scan + 4], "big")
+    bad = good[:scan] + b"\xff\x00" * ((len(good) - scan - 2) // 2) + b"\xff\xd9"
+    blob, tiles = H.lj92_tiles([bad], W, Hh, W, Hh)
+    with pytest.raises(RuntimeError):
+        H.lj92_decode_host(blob, tiles[0], W, Hh)
+    buf = hip.DeviceBuffer(W, Hh, 1, H.FMT_UI16)
+    with pytest.raises(RuntimeError):
+        H.lj92_decode(blob, tiles, [(buf.ptr, W, Hh, W)])
+    # and the good stream still decodes on both
+    blob, tiles = H.lj92_tiles([good], W, Hh, W, Hh)
+    H.lj92_decode(blob, tiles, [(buf.ptr, W, Hh, W)])
+    assert np.array_equal(buf.download(), cfa)
+    buf.free()
 
 
 def test_throughput_24mp_tiles(hip):

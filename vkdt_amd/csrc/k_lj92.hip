@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(64) k_lj92(const lj92_job *jobs, int n, int *s
 //             its symbol count and where its last symbol ends
 //   sync      while any lane's start differs from its predecessor's end, it
 //             decodes again from there (ping-pong ends; converges in as many
-//             rounds as the longest chain of wrong starts, usually one)
+//             rounds as the longest chain of wrong starts, a few)
 //   scan      symbol offsets per subsequence (one work group per stream)
 //   write     every lane decodes its subsequence once more into the
 //             differences; the last lane of a stream continues past the
@@ -643,7 +643,9 @@ __global__ void __launch_bounds__(256) k_lj92_sync(const lj92_pst *ps, int np, u
   const lj92_pst &p = ps[i];
   const uint32_t k = g - p.sbase;
   const uint32_t e = end_in[g];
-  if(k == 0 || end_in[g - 1] == start[g]) { end_out[g] = e; return; }
+  // past the data a subsequence holds no symbol whatever its start: left alone,
+  // it does not hand a changed start down the rest of the stream one round at a time
+  if(k == 0 || end_in[g - 1] == start[g] || k * LJ92_SUB >= nbits[i]) { end_out[g] = e; return; }
   const uint32_t s = end_in[g - 1], lim = min(k * LJ92_SUB + LJ92_SUB, nbits[i]);
   uint32_t n;
   start[g] = s;
