@@ -152,8 +152,9 @@ vkdt_lj92_tile_t;
 
 typedef struct vkdt_lj92_image_t
 {
-  void    *out;               // device: uint16 mosaic
-  uint32_t wd, ht, pitch;     // extent and texels per row
+  void    *out;               // device: uint16 mosaic, the window's first site
+  uint32_t wd, ht, pitch;     // window extent and texels per row of out
+  uint32_t x0, y0;            // the window's first site in the tiles' image (a crop; 0, 0 for all of it)
 }
 vkdt_lj92_image_t;
 
@@ -165,10 +166,12 @@ int vkdt_lj92_info(const uint8_t *data, size_t length, uint32_t *wd, uint32_t *h
 int vkdt_lj92_decode_host(const uint8_t *data, const vkdt_lj92_tile_t *tile, uint16_t *out,
                           uint32_t wd, uint32_t ht, uint32_t pitch);
 
-// Decode n tiles of nimg images on the GPU, one lane per tile: the host
+// Decode n tiles of nimg images on the GPU into their windows: the host
 // parses the headers and builds the code tables, uploads the compressed
-// bytes (`data`, data_len) and waits for the decode on `stream`.  No
-// reference counterpart (the reference unpacks on the host).
+// bytes (`data`, data_len) and waits for the decode on `stream`.  Streams
+// with one table and predictor 1 are decoded by many lanes each, the others
+// by one lane each.  No reference counterpart (the reference unpacks on the
+// host); i-raw calls it from read_source_hip.
 int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const vkdt_lj92_tile_t *tiles, int n,
                          const vkdt_lj92_image_t *img, int nimg, void *stream);
 

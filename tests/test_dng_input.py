@@ -15,7 +15,8 @@ class DngImage(C.Structure):
                 ("stride", C.c_uint32), ("cpp", C.c_uint32), ("wb_coeffs", C.c_float * 4),
                 ("whitelevels", C.c_float * 4), ("blacklevels", C.c_float * 4), ("xyz_to_cam", C.c_float * 12),
                 ("illuminant", C.c_uint32), ("cfa_w", C.c_uint32), ("cfa_h", C.c_uint32), ("cfa", C.c_uint8 * 36),
-                ("crop_aabb", C.c_uint64 * 4), ("iso", C.c_float), ("data", C.POINTER(C.c_uint16))]
+                ("crop_aabb", C.c_uint64 * 4), ("iso", C.c_float), ("data", C.POINTER(C.c_uint16)),
+                ("packed", C.c_void_p), ("packed_len", C.c_size_t), ("tiles", C.c_void_p), ("ntiles", C.c_uint32)]
 
 
 @pytest.fixture(scope="module")

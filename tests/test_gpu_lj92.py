@@ -118,13 +118,33 @@ def test_throughput_24mp_tiles(hip):
     out.free()
 
 
-def test_graph_from_compressed_dng(tmp_path, hip):
-    W, Hh = 320, 240
-    cfa, _ = synth.mosaic(W, Hh, 31, "bayer")
+@pytest.mark.parametrize("pred,comp", [(1, 2), (6, 1)])
+def test_window(hip, pred, comp):
+    """a window of the tiles' image (x0, y0 and extent) lands at the out
+    pointer, nothing outside it is written"""
+    W, Hh, tw, th = 300, 200, 64, 48
+    cfa, _ = synth.mosaic(W, Hh, 8, "bayer")
+    blob, tiles = H.lj92_tiles(encode_tiles(cfa, tw, th, comp, pred), tw, th, W, Hh)
+    x0, y0, w, h = 3, 5, W - 7, Hh - 9
+    buf = hip.DeviceBuffer(w + 8, h + 2, 1, H.FMT_UI16)
+    buf.upload(np.full((h + 2, w + 8), 7, np.uint16))
+    H.lj92_decode(blob, tiles, [(buf.ptr, w, h, w + 8, x0, y0)])
+    got = buf.download()
+    assert np.array_equal(got[:h, :w], cfa[y0:y0 + h, x0:x0 + w])
+    assert (got[:h, w:] == 7).all() and (got[h:] == 7).all()
+    buf.free()
+
+
+@pytest.mark.parametrize("kind,W,Hh,pattern", [("bayer", 320, 240, None), ("bayer", 331, 247, ((2, 1), (1, 0))),
+                                               ("xtrans", 330, 246, None)], ids=["rggb", "bggr-ragged", "xtrans"])
+def test_graph_from_compressed_dng(tmp_path, hip, kind, W, Hh, pattern):
+    """i-raw decodes a compressed DNG on the GPU (read_source_hip) into the
+    CFA-aligned window; the developed output equals the uncompressed DNG's"""
+    cfa, _ = synth.mosaic(W, Hh, 31, kind)
     outs = []
     for comp in (None, dict(tile=(128, 96), components=2, predictor=1, bits=14)):
         dng = str(tmp_path / f"{'c' if comp else 'u'}.dng")
-        synth.write_dng(dng, cfa, "bayer", compress=comp)
+        synth.write_dng(dng, cfa, kind, compress=comp, pattern=pattern)
         na, nb = synth.noise_params(2e-4, 1e-6)
         g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb))
         out = str(tmp_path / f"o{len(outs)}")

@@ -317,8 +317,8 @@ __host__ __device__ static inline int lj92_run(const lj92_job &jg, const lj92_ta
   const int C = j.comp, W = j.fwd, L = W * C;
   const int base = 1 << (j.bits - j.pt - 1);
   int tx = 0, ty = 0;      // tile site of the next line's first sample
-  long o = (long)j.y0 * j.pitch + j.x0;
-  const bool inside = j.x0 + j.tile_wd <= j.img_wd;   // every column of the tile lands in the image
+  long o = (long)j.y0 * j.pitch + j.x0;               // the tile's origin relative to the window (may be outside)
+  const bool inside = j.x0 >= 0 && j.x0 + j.tile_wd <= j.img_wd;   // every column of the tile lands in the window
   // a scan line is a tile line that lands whole, on 16-byte boundaries
   const bool vec = L == j.tile_wd && inside && (L & 7) == 0 && (j.pitch & 7) == 0 &&
                    (((uintptr_t)(j.out + o)) & 15) == 0 && (((uintptr_t)row) & 15) == 0;
@@ -369,7 +369,7 @@ __host__ __device__ static inline int lj92_run(const lj92_job &jg, const lj92_ta
     // the scan fills the tile row-major: this line's L samples from (tx, ty) on
     if(vec)
     {
-      if(j.y0 + ty < j.img_ht)
+      if(j.y0 + ty >= 0 && j.y0 + ty < j.img_ht)
         for(int k = 0; k < L; k += 8)
         {
           lj92_u32x4 w = *(const lj92_u32x4 *)(row + k);
@@ -382,7 +382,7 @@ __host__ __device__ static inline int lj92_run(const lj92_job &jg, const lj92_ta
     else
       for(int k = 0; k < L; k++)
       {
-        if(j.y0 + ty < j.img_ht && (inside || j.x0 + tx < j.img_wd))
+        if(j.y0 + ty >= 0 && j.y0 + ty < j.img_ht && (inside || (j.x0 + tx >= 0 && j.x0 + tx < j.img_wd)))
           lj92_st(j.out + o + tx, (uint16_t)(row[k] << j.pt));
         if(++tx == j.tile_wd) { tx = 0; ty++; o += j.pitch; }
       }
@@ -773,7 +773,7 @@ __global__ void __launch_bounds__(256) k_lj92_rows(const lj92_pst *ps, int np, u
   const long s0 = (long)r * L;
   int tx = (int)(s0 % p.tile_wd), ty = (int)(s0 / p.tile_wd);
   long o = (long)(p.y0 + ty) * p.pitch + p.x0;
-  const bool inside = p.x0 + p.tile_wd <= p.img_wd;
+  const bool inside = p.x0 >= 0 && p.x0 + p.tile_wd <= p.img_wd;
   for(int k = 0; k < L; k++)
   {
     const int c = k % C;
@@ -787,7 +787,8 @@ __global__ void __launch_bounds__(256) k_lj92_rows(const lj92_pst *ps, int np, u
       else if(c == 2) v = v2 = (v2 + dd) & 0xffff;
       else v = v3 = (v3 + dd) & 0xffff;
     }
-    if(p.y0 + ty < p.img_ht && (inside || p.x0 + tx < p.img_wd)) lj92_st(p.out + o + tx, (uint16_t)(v << p.pt));
+    if(p.y0 + ty >= 0 && p.y0 + ty < p.img_ht && (inside || (p.x0 + tx >= 0 && p.x0 + tx < p.img_wd)))
+      lj92_st(p.out + o + tx, (uint16_t)(v << p.pt));
     if(++tx == p.tile_wd) { tx = 0; ty++; o += p.pitch; }
   }
 }
@@ -812,7 +813,7 @@ __global__ void __launch_bounds__(256) k_lj92_rows4(const lj92_pst *ps, int np, 
   const int r = g - p.rbase, C = p.comp, L = p.fwd * C;
   const uint16_t *d = p.diff + (size_t)r * L;
   const uint64_t c0 = lj92_ld(p.col0 + r);
-  const bool direct = L == p.tile_wd && p.x0 + p.tile_wd <= p.img_wd;   // scan line = tile line inside the image
+  const bool direct = L == p.tile_wd && p.x0 >= 0 && p.x0 + p.tile_wd <= p.img_wd;   // scan line = tile line inside the window
   const long s0 = (long)r * L;
   const int ty0 = (int)(s0 / p.tile_wd);
   const bool wide = (L & 3) == 0;                      // lines start on 8-byte boundaries of the differences
@@ -850,7 +851,7 @@ __global__ void __launch_bounds__(256) k_lj92_rows4(const lj92_pst *ps, int np, 
       v4[j] = ((uint32_t)(run >> (16 * c)) & 0xffff) << p.pt;
     }
     uint16_t *dst = p.out + (long)(p.y0 + ty0) * p.pitch + p.x0 + k;
-    if(direct && k + 3 < L && p.y0 + ty0 < p.img_ht && ((uintptr_t)dst & 7) == 0)
+    if(direct && k + 3 < L && p.y0 + ty0 >= 0 && p.y0 + ty0 < p.img_ht && ((uintptr_t)dst & 7) == 0)
     {
       lj92_st((uint64_t *)dst, (uint64_t)(v4[0] & 0xffff) | ((uint64_t)(v4[1] & 0xffff) << 16) |
                                 ((uint64_t)(v4[2] & 0xffff) << 32) | ((uint64_t)(v4[3] & 0xffff) << 48));
@@ -864,7 +865,7 @@ __global__ void __launch_bounds__(256) k_lj92_rows4(const lj92_pst *ps, int np, 
       int tx, ty;
       if(direct) { tx = kk; ty = ty0; }
       else { const long sm = s0 + kk; tx = (int)(sm % p.tile_wd); ty = (int)(sm / p.tile_wd); }
-      if(p.y0 + ty < p.img_ht && p.x0 + tx < p.img_wd)
+      if(p.y0 + ty >= 0 && p.y0 + ty < p.img_ht && p.x0 + tx >= 0 && p.x0 + tx < p.img_wd)
         lj92_st(p.out + (long)(p.y0 + ty) * p.pitch + p.x0 + tx, (uint16_t)v);
     }
   }
@@ -1033,7 +1034,7 @@ extern "C" int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const 
       p.col0 = (uint64_t *)(dev + off_c0) + rw; rw += h.ht;
       p.fwd = h.wd; p.fht = h.ht; p.comp = h.comp; p.bits = h.bits; p.pt = h.pt;
       p.out = (uint16_t *)im.out; p.img_wd = im.wd; p.img_ht = im.ht; p.pitch = im.pitch;
-      p.x0 = t.x0; p.y0 = t.y0; p.tile_wd = t.tile_wd;
+      p.x0 = (int)t.x0 - (int)im.x0; p.y0 = (int)t.y0 - (int)im.y0; p.tile_wd = t.tile_wd;
       continue;
     }
     lj92_job &j = jobs[next[kind[i]]++];
@@ -1047,7 +1048,7 @@ extern "C" int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const 
     }
     j.bits = h.bits; j.fwd = h.wd; j.fht = h.ht; j.comp = h.comp; j.pred = h.pred; j.pt = h.pt;
     j.out = (uint16_t *)im.out; j.img_wd = im.wd; j.img_ht = im.ht; j.pitch = im.pitch;
-    j.x0 = t.x0; j.y0 = t.y0; j.tile_wd = t.tile_wd;
+    j.x0 = (int)t.x0 - (int)im.x0; j.y0 = (int)t.y0 - (int)im.y0; j.tile_wd = t.tile_wd;
     j.row = nullptr;
     if(!(kind[i] & 2))
     {

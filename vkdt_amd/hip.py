@@ -34,7 +34,8 @@ class Lj92Tile(C.Structure):
 
 class Lj92Image(C.Structure):
     """vkdt_lj92_image_t"""
-    _fields_ = [("out", C.c_void_p), ("wd", C.c_uint32), ("ht", C.c_uint32), ("pitch", C.c_uint32)]
+    _fields_ = [("out", C.c_void_p), ("wd", C.c_uint32), ("ht", C.c_uint32), ("pitch", C.c_uint32),
+                ("x0", C.c_uint32), ("y0", C.c_uint32)]
 
 
 def token(s):
@@ -218,8 +219,9 @@ def lj92_decode_host(blob, tile, W, H):
 
 
 def lj92_decode(blob, tiles, images, stream=None):
-    """tiles of several images on the GPU; images: [(device ptr, W, H, pitch)]"""
+    """tiles of several images on the GPU; images: [(device ptr, W, H, pitch[, x0, y0])], each a
+    window of W x H sites from (x0, y0) on of its tiles' image"""
     buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
     t = (Lj92Tile * len(tiles))(*tiles)
-    im = (Lj92Image * len(images))(*[Lj92Image(p, w, h, pi) for p, w, h, pi in images])
+    im = (Lj92Image * len(images))(*[Lj92Image(*(tuple(i) + (0, 0))[:6]) for i in images])
     check(lib().vkdt_hip_lj92_decode(buf, len(blob), t, len(tiles), im, len(images), stream), "lj92 decode")

@@ -6,8 +6,10 @@
  * i-raw needs: extent, CFA pattern, black/white levels, colour matrices
  * with their calibration illuminants, AsShotNeutral and DefaultCrop.
  * Raw data: 16-bit uncompressed strips, or lossless-JPEG compressed
- * (Compression 7) tiles or strips, decoded on the host one stream at a time
- * (vkdt_lj92_decode_host); anything else is refused with a message.
+ * (Compression 7) tiles or strips, kept compressed with their stream list
+ * for a GPU decode (vkdt_hip_lj92_decode, from i-raw's read_source_hip) or a
+ * host decode on demand (dng_decode_host); anything else is refused with a
+ * message.
  */
 #include "dng.h"
 #include "vkdt_hip.h"
@@ -201,16 +203,24 @@ int dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen)
     if(tw == 0 || th == 0) { snprintf(err, errlen, "%s: bad tile extent %ux%u", filename, tw, th); goto done; }
     const uint32_t ntx = (W + tw - 1) / tw, nty = (H + th - 1) / th;
     if(to->cnt < ntx * nty) { snprintf(err, errlen, "%s: %u tiles for %ux%u", filename, to->cnt, ntx, nty); goto done; }
-    img->data = calloc((size_t)W * H, 2);
-    if(!img->data) { snprintf(err, errlen, "out of memory"); goto done; }
+    img->tiles = calloc((size_t)ntx * nty, sizeof(vkdt_lj92_tile_t));
+    if(!img->tiles) { snprintf(err, errlen, "out of memory"); goto done; }
     for(uint32_t k = 0; k < ntx * nty; k++)
     {
       const vkdt_lj92_tile_t tile = { (uint64_t)ent_val(&t, to, k), (uint64_t)ent_val(&t, tcnt, k),
                                       (k % ntx) * tw, (k / ntx) * th, tw, th, 0 };
       if(tile.offset + tile.length > t.n) { snprintf(err, errlen, "%s: truncated tile %u", filename, k); goto done; }
-      if(vkdt_lj92_decode_host(buf, &tile, img->data, W, H, W))
+      uint32_t fw, fh, fc;   // the stream's header, checked now: decoding comes later
+      if(vkdt_lj92_info(buf + tile.offset, tile.length, &fw, &fh, &fc, 0))
       { snprintf(err, errlen, "%s: tile %u: %s", filename, k, vkdt_hip_last_error()); goto done; }
+      if((uint64_t)fw * fc * fh < (uint64_t)tw * th)
+      { snprintf(err, errlen, "%s: tile %u: lj92 frame %ux%ux%u smaller than the tile", filename, k, fw, fh, fc); goto done; }
+      img->tiles[k] = tile;
     }
+    img->ntiles = ntx * nty;
+    img->packed = buf;     // kept for the decode
+    img->packed_len = t.n;
+    buf = 0;
     goto decoded;
   }
   const entry_t *so = find(ri, 273), *sc = find(ri, 279);
@@ -314,7 +324,7 @@ decoded:
   img->iso = 0.0f;
   ret = 0;
 done:
-  if(ret) { free(img->data); img->data = 0; }
+  if(ret) dng_free(img);
   free(buf);
   return ret;
 }
@@ -322,5 +332,29 @@ done:
 void dng_free(dng_image_t *img)
 {
   free(img->data);
+  free(img->packed);
+  free(img->tiles);
   img->data = 0;
+  img->packed = 0;
+  img->tiles = 0;
+  img->packed_len = 0;
+  img->ntiles = 0;
+}
+
+int dng_decode_host(dng_image_t *img, char *err, size_t errlen)
+{
+  if(img->data) return 0;
+  if(!img->packed) { snprintf(err, errlen, "no raw data"); return 1; }
+  const uint32_t W = (uint32_t)img->width, H = (uint32_t)img->height;
+  img->data = calloc((size_t)W * H, 2);
+  if(!img->data) { snprintf(err, errlen, "out of memory"); return 1; }
+  for(uint32_t k = 0; k < img->ntiles; k++)
+    if(vkdt_lj92_decode_host(img->packed, img->tiles + k, img->data, W, H, W))
+    {
+      snprintf(err, errlen, "tile %u: %s", k, vkdt_hip_last_error());
+      free(img->data);
+      img->data = 0;
+      return 1;
+    }
+  return 0;
 }

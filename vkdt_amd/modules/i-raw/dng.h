@@ -7,6 +7,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include "vkdt_hip.h"
+
 typedef struct dng_image_t
 {
   char     maker[32], model[32];
@@ -22,10 +24,19 @@ typedef struct dng_image_t
   uint8_t  cfa[36];           // 0 red, 1 green, 2 blue
   uint64_t crop_aabb[4];      // x y X Y
   float    iso;
-  uint16_t *data;             // height x stride samples, owned
+  uint16_t *data;             // height x stride samples, owned (0 for a lossless-JPEG raw until decoded)
+  // lossless-JPEG raws: the file's bytes and one stream per tile or strip,
+  // decoded on the GPU into the module's buffer (read_source_hip) or on the
+  // host on demand (dng_decode_host)
+  uint8_t  *packed;
+  size_t    packed_len;
+  vkdt_lj92_tile_t *tiles;
+  uint32_t  ntiles;
 }
 dng_image_t;
 
 // returns 0 on success; on failure `err` holds a message
 int  dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen);
 void dng_free(dng_image_t *img);
+// fills data from the lossless-JPEG streams (no-op when data is there)
+int  dng_decode_host(dng_image_t *img, char *err, size_t errlen);

@@ -3,7 +3,8 @@
  *
  * Callbacks follow src/pipe/modules/i-raw/main.c: modify_roi_out loads the
  * file and publishes extent and image metadata (main.c:138-259),
- * read_source copies the CFA rows into the mapped staging buffer
+ * read_source copies the CFA rows into the mapped staging buffer (lossless-JPEG
+ * raws are decoded on the GPU by read_source_hip instead, when the runtime has one)
  * (main.c:261-299).  The CFA offset normalisation and block rounding of the
  * raw loader (rawloader-c/lib.rs:200-283) are applied here, so downstream
  * kernels always see RGGB or the canonical X-Trans layout.
@@ -110,7 +111,7 @@ static void normalise_cfa(raw_t *r)
 static int load_raw(dt_module_t *mod, int frame, const char *filename)
 {
   raw_t *r = mod->data;
-  if(!strcmp(r->filename, filename) && r->frame == frame && r->img.data) return 0;
+  if(!strcmp(r->filename, filename) && r->frame == frame && (r->img.data || r->img.packed)) return 0;
   dng_free(&r->img);
   r->filename[0] = 0;
   char err[512];
@@ -245,10 +246,41 @@ int read_source(dt_module_t *mod, void *mapped, dt_read_source_params_t *p)
   char filename[PATH_MAX];
   if(dt_graph_get_resource_filename(mod, fname, id + mod->graph->frame, filename, sizeof(filename))) return 1;
   if(load_raw(mod, id + mod->graph->frame, filename)) return 1;
-  const raw_t *r = mod->data;
+  raw_t *r = mod->data;
+  char err[512];
+  if(dng_decode_host(&r->img, err, sizeof(err)))
+  {
+    snprintf(mod->graph->err, sizeof(mod->graph->err), "%s: %s", filename, err);
+    return 1;
+  }
   uint16_t *buf = (uint16_t *)mapped;
   const uint64_t wd = r->width, ht = r->height, stride = r->img.stride;
   for(uint64_t j = 0; j < ht; j++)
     memcpy(buf + j * wd, r->img.data + (j + r->oy) * stride + r->ox, sizeof(uint16_t) * wd);
+  return 0;
+}
+
+// The raw straight into the output buffer on the device: lossless-JPEG raws
+// are uploaded compressed and decoded on the GPU, cropped to the CFA-aligned
+// window (ox, oy, width x height) by the decode itself.  Returns 1 for raws
+// that take read_source's path (uncompressed: a plain copy and upload), 0
+// when done, -1 on error.  No reference counterpart: the reference decodes
+// every raw on the host inside read_source (main.c:240-259).
+int read_source_hip(dt_module_t *mod, void *dev, void *stream)
+{
+  const int id = dt_module_param_int(mod, 3)[0];
+  const char *fname = dt_module_param_string(mod, 0);
+  char filename[PATH_MAX];
+  if(dt_graph_get_resource_filename(mod, fname, id + mod->graph->frame, filename, sizeof(filename))) return -1;
+  if(load_raw(mod, id + mod->graph->frame, filename)) return -1;
+  const raw_t *r = mod->data;
+  if(!r->img.packed || r->img.data) return 1;
+  const vkdt_lj92_image_t win = { dev, (uint32_t)r->width, (uint32_t)r->height, (uint32_t)r->width,
+                                  (uint32_t)r->ox, (uint32_t)r->oy };
+  if(vkdt_hip_lj92_decode(r->img.packed, r->img.packed_len, r->img.tiles, (int)r->img.ntiles, &win, 1, stream))
+  {
+    snprintf(mod->graph->err, sizeof(mod->graph->err), "%s: %s", filename, vkdt_hip_last_error());
+    return -1;
+  }
   return 0;
 }

@@ -240,6 +240,9 @@ typedef struct dt_module_so_t
   void (*commit_params)(dt_graph_t *, dt_module_t *);
   dt_graph_run_t (*check_params)(dt_module_t *, uint32_t, uint32_t, void *);
   int  (*read_source)(dt_module_t *, void *, dt_read_source_params_t *);
+  // optional, no reference counterpart: fill the source's device buffer on
+  // `stream` directly (GPU unpack); 0 done, 1 use read_source, -1 error
+  int  (*read_source_hip)(dt_module_t *, void *dev, void *stream);
   void (*write_sink)(dt_module_t *, void *, dt_write_sink_params_t *);
   dt_ui_param_t *param[DT_MAX_PARAMS];
   int num_params;

@@ -1049,11 +1049,17 @@ int dt_graph_run(dt_graph_t *g, dt_graph_run_t run)
       dt_connector_t *cc = n->connector + 0;
       dt_read_source_params_t p = { .node = n, .a = 0 };
       uint8_t *mapped = (uint8_t *)g->staging + stoff;
+      stoff += cc->bytes;
+      if(n->module->so->read_source_hip)
+      { // the module fills its device buffer itself (GPU unpack)
+        const int rc = n->module->so->read_source_hip(n->module, cc->dev, g->stream);
+        if(rc < 0) return gerr(g, -1, "read_source_hip failed for %" PRItkn, dt_token_str(n->module->name));
+        if(rc == 0) continue;
+      }
       if(n->module->so->read_source(n->module, mapped, &p))
         return gerr(g, -1, "read_source failed for %" PRItkn, dt_token_str(n->module->name));
       const size_t bytes = dt_connector_bufsize(cc, cc->roi.wd, cc->roi.ht) * MAX(1, cc->array_length);
       if(vkdt_hip_upload(cc->dev, mapped, bytes, g->stream)) return gerr(g, -3, "upload: %s", vkdt_hip_last_error());
-      stoff += cc->bytes;
     }
   }
   // uniforms
