@@ -175,6 +175,14 @@ int vkdt_lj92_decode_host(const uint8_t *data, const vkdt_lj92_tile_t *tile, uin
 int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const vkdt_lj92_tile_t *tiles, int n,
                          const vkdt_lj92_image_t *img, int nimg, void *stream);
 
+// Unpack n samples of `bits` bits (1..16) from a bit-packed frame on the
+// host (`packed`, bytes) into a uint16 device buffer: sample i is bits
+// [i*bits, (i+1)*bits) of the stream of 16-bit little-endian words, each
+// word read from its most significant bit (Magic Lantern raw video frames).
+// Replaces the host loop of i-mlv/video_mlv.c:262-276 (mlv_get_frame); the
+// bytes are uploaded as stored and unpacked on `stream`, which it waits for.
+int vkdt_hip_unpack_msb16(const uint8_t *packed, size_t bytes, void *out, uint32_t n, int bits, void *stream);
+
 // ---- batched develop ------------------------------------------------------
 // The raw-development graph i-raw -> denoise -> demosaic -> colour ->
 // filmcurv -> llap of many stills or frames, of any sizes and CFAs, in one

@@ -1,0 +1,131 @@
+"""Magic Lantern raw video (i-mlv) on the CPU: the clip reader, the metadata
+it publishes and the host unpack/decode of read_source (reference
+i-mlv/main.c, video_mlv.c).
+
+  * the synthetic writer's bit packing is checked against the reference's
+    unpack formula restated (a 32-bit window of two little-endian words
+    rotated right by 16 + (32 - bits) - shift, the count taken mod 32 as
+    x86 does, video_mlv.c:262-276) for 10..16 bits -- this pins the format
+    both the host and the GPU unpack read;
+  * every frame of clips with packed (10, 12, 14 bit) and lossless-JPEG
+    frames reads back bit for bit, in timestamp order when the file holds
+    them out of order, across an unknown block (resynchronised on the next
+    known block type), across a second chunk file (.M00) and past frame
+    padding (frameSpace);
+  * the frame index is held at the header's last frame;
+  * extent, levels, filters, crop, exposure, aperture, ISO, focal length,
+    maker/model, frame count and rate are the clip's.
+"""
+import numpy as np
+import pytest
+
+from vkdt_amd import pipe, synth
+
+
+def ref_unpack(packed, n, bits):
+    w = np.frombuffer(packed + b"\0\0\0\0", "<u2")
+    out = np.zeros(n, np.uint32)
+    for i in range(n):
+        off = i * bits
+        a, s = off // 16, off % 16
+        v = int(w[a]) | (int(w[a + 1]) << 16)
+        r = (16 + ((32 - bits) - s)) % 32
+        v = ((v >> r) | (v << (32 - r))) & 0xffffffff
+        out[i] = v & ((1 << bits) - 1)
+    return out
+
+
+@pytest.mark.parametrize("bits", [10, 11, 12, 14, 16])
+def test_packing_matches_the_reference_unpack(bits):
+    rng = np.random.default_rng(bits)
+    x = rng.integers(0, 1 << bits, 777).astype(np.uint16)
+    assert np.array_equal(ref_unpack(synth.pack_msb16(x, bits), x.size, bits), x)
+
+
+def clip_frames(n=4, W=96, H=64, bits=14, seed=3):
+    rng = np.random.default_rng(seed)
+    return [rng.integers(0, 1 << bits, size=(H, W)).astype(np.uint16) for _ in range(n)]
+
+
+def read_frames(path, n, W, H):
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    out = []
+    for f in range(n):
+        g.set_frame(f)
+        g.plan()
+        out.append(g.read_source("i-mlv", "main", W * H * 2).view(np.uint16).reshape(H, W).copy())
+    return g, out
+
+
+CASES = {
+    "packed14": dict(bits=14),
+    "packed12": dict(bits=12),
+    "packed10": dict(bits=10),
+    "lj92-1": dict(bits=14, lj92=dict(components=1, predictor=1)),
+    "lj92-2": dict(bits=14, lj92=dict(components=2, predictor=1)),
+    "lj92-pred6": dict(bits=14, lj92=dict(components=1, predictor=6)),
+    "shuffled": dict(bits=14, order=[2, 0, 3, 1]),
+    "junk": dict(bits=12, junk_before=(1, 3)),
+    "chunks": dict(bits=14, chunk_split=2),
+    "padding": dict(bits=14, frame_space=20),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_frames_read_back(tmp_path, case):
+    kw = CASES[case]
+    frames = clip_frames(bits=kw["bits"])
+    path = str(tmp_path / "clip.MLV")
+    synth.write_mlv(path, frames, **kw)
+    g, got = read_frames(path, len(frames), 96, 64)
+    for f, (a, b) in enumerate(zip(got, frames)):
+        assert np.array_equal(a, b), f
+    assert g.frame_cnt() == len(frames)
+    g.close()
+
+
+def test_frame_index_held_at_the_last(tmp_path):
+    frames = clip_frames(n=3)
+    path = str(tmp_path / "c.mlv")
+    synth.write_mlv(path, frames, header_frames=2)
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    g.set_frame(7)
+    g.plan()
+    got = g.read_source("i-mlv", "main", 96 * 64 * 2).view(np.uint16).reshape(64, 96)
+    assert np.array_equal(got, frames[1])       # min(frame, header count - 1)
+    assert g.frame_cnt() == 2
+    g.close()
+
+
+def test_metadata(tmp_path):
+    frames = clip_frames(n=2)
+    path = str(tmp_path / "m.mlv")
+    synth.write_mlv(path, frames, black=2047, white=15200, fps=(24000, 1001), camera="Canon EOS 5D Mark III",
+                    iso=1600, shutter_us=10000, aperture=560, focal=35)
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    g.plan()
+    ip = g.img_param("i-mlv", "main")
+    assert ip["full"] == (96, 64)
+    assert ip["black"] == [2047.0] * 4 and ip["white"] == [15200.0] * 4
+    assert ip["filters"] == 0x5d5d5d5d
+    assert ip["crop"] == [0, 0, 96, 64]
+    assert ip["iso"] == 1600.0
+    assert abs(ip["exposure"] - 0.01) < 1e-9 and abs(ip["aperture"] - 5.6) < 1e-6 and ip["focal_length"] == 35.0
+    assert ip["maker"] == "Canon" and ip["model"] == "Canon EOS 5D Mark III"
+    assert ip["noise"] == [1.0, 1.0]
+    assert ip["wb"] == [1.0] * 4
+    assert g.frame_cnt() == 2 and abs(g.frame_rate() - 24000 / 1001) < 1e-4
+    g.close()
+
+
+def test_rejects_bad_clips(tmp_path):
+    bad = str(tmp_path / "x.mlv")
+    open(bad, "wb").write(b"RIFF" + bytes(60))
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(bad))
+    g.export("o-pfm", bad + ".out")
+    with pytest.raises(pipe.PipeError):
+        g.plan()
+    g.close()

@@ -81,6 +81,7 @@ def lib():
         L.vkdt_lj92_info.argtypes = [vp, sz] + [C.POINTER(C.c_uint32)] * 4
         L.vkdt_lj92_decode_host.argtypes = [vp, C.POINTER(Lj92Tile), vp, C.c_uint32, C.c_uint32, C.c_uint32]
         L.vkdt_hip_lj92_decode.argtypes = [vp, sz, C.POINTER(Lj92Tile), C.c_int, C.POINTER(Lj92Image), C.c_int, vp]
+        L.vkdt_hip_unpack_msb16.argtypes = [vp, sz, vp, C.c_uint32, C.c_int, vp]
         L.vkdt_hip_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -216,6 +217,12 @@ def lj92_decode_host(blob, tile, W, H):
     buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
     check(lib().vkdt_lj92_decode_host(buf, C.byref(tile), out.ctypes.data, W, H, W), "lj92 host decode")
     return out
+
+
+def unpack_msb16(packed, out, n, bits, stream=None):
+    """bit-packed samples (bytes, Magic Lantern layout) into a uint16 device buffer"""
+    buf = (C.c_uint8 * max(1, len(packed))).from_buffer_copy(packed or b"\0")
+    check(lib().vkdt_hip_unpack_msb16(buf, len(packed), out, n, bits, stream), "unpack")
 
 
 def lj92_decode(blob, tiles, images, stream=None):

@@ -41,6 +41,10 @@ def lib():
         L.dt_graph_last_error.restype = cp
         L.dt_graph_set_perf.argtypes = [vp, i32]
         L.dt_graph_set_skip_dead.argtypes = [vp, i32]
+        L.dt_graph_set_frame.argtypes = [vp, i32]
+        L.dt_graph_frame_cnt.argtypes = [vp]
+        L.dt_graph_frame_rate.argtypes = [vp]
+        L.dt_graph_frame_rate.restype = C.c_float
         L.dt_graph_num_nodes.argtypes = [vp]
         L.dt_graph_num_exec.argtypes = [vp]
         L.dt_graph_exec_node.argtypes = [vp, i32]
@@ -60,6 +64,7 @@ def lib():
         L.dt_graph_node_buffer.argtypes = [vp, i32, i32]
         L.dt_graph_node_buffer.restype = vp
         f4, u4 = C.c_float * 4, C.c_uint32 * 4
+        L.dt_graph_img_meta_get.argtypes = [vp, i32, f4, cp, cp, cp]
         L.dt_graph_img_param_get.argtypes = [vp, i32, f4, f4, f4, C.c_float * 9, C.POINTER(u32), u4,
                                              C.c_float * 2, C.c_uint32 * 2]
         L.dt_graph_commit.argtypes = [vp]
@@ -131,8 +136,12 @@ class Graph:
         black, white, wb, cam = f4(), f4(), f4(), (C.c_float * 9)()
         filters, crop, noise, roi = C.c_uint32(), (C.c_uint32 * 4)(), (C.c_float * 2)(), (C.c_uint32 * 2)()
         self.L.dt_graph_img_param_get(self.g, m, black, white, wb, cam, C.byref(filters), crop, noise, roi)
+        meta, maker, model, dtime = f4(), C.create_string_buffer(32), C.create_string_buffer(32), C.create_string_buffer(20)
+        self.L.dt_graph_img_meta_get(self.g, m, meta, maker, model, dtime)
         return dict(black=list(black), white=list(white), wb=list(wb), cam=list(cam), filters=filters.value,
-                    crop=list(crop), noise=list(noise), full=(roi[0], roi[1]))
+                    crop=list(crop), noise=list(noise), full=(roi[0], roi[1]), exposure=meta[0], aperture=meta[1],
+                    iso=meta[2], focal_length=meta[3], maker=maker.value.decode(errors="replace"),
+                    model=model.value.decode(errors="replace"), datetime=dtime.value.decode(errors="replace"))
 
     def commit(self):
         self.L.dt_graph_commit(self.g)
@@ -160,6 +169,16 @@ class Graph:
 
     def set_skip_dead(self, on=True):
         self.L.dt_graph_set_skip_dead(self.g, 1 if on else 0)
+
+    def set_frame(self, frame):
+        """the frame the next run delivers (video sources)"""
+        self.L.dt_graph_set_frame(self.g, frame)
+
+    def frame_cnt(self):
+        return self.L.dt_graph_frame_cnt(self.g)
+
+    def frame_rate(self):
+        return self.L.dt_graph_frame_rate(self.g)
 
     def stream(self):
         return self.L.dt_graph_stream(self.g)
@@ -246,4 +265,13 @@ def raw_graph_cfg(dng, strength=0.5, noise_a=0.0, noise_b=0.0, clarity=0.2, numg
         f"param:llap:01:numgamma:{numgamma}",
         f"param:llap:01:levels:{levels}",
     ]
+    return lines + list(extra)
+
+
+def mlv_graph_cfg(mlv, strength=0.5, clarity=0.2, numgamma=10, levels=12, extra=()):
+    """cfg lines of the raw-development graph with a raw video source:
+    i-mlv -> denoise -> demosaic -> colour -> filmcurv -> llap -> display"""
+    lines = [l.replace("i-raw:main", "i-mlv:main") for l in
+             raw_graph_cfg(mlv, strength=strength, clarity=clarity, numgamma=numgamma, levels=levels)
+             if "noise a" not in l and "noise b" not in l]
     return lines + list(extra)

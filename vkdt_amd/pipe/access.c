@@ -24,6 +24,10 @@ void dt_graph_free(dt_graph_t *g)
 
 void dt_graph_set_perf(dt_graph_t *g, int on) { g->perf = on; }
 void dt_graph_set_skip_dead(dt_graph_t *g, int on) { g->skip_dead = on; }
+// the frame a run delivers (graph-export.c:268-271 sets graph->frame per frame)
+void dt_graph_set_frame(dt_graph_t *g, int frame) { g->frame = frame; }
+int  dt_graph_frame_cnt(const dt_graph_t *g) { return g->frame_cnt; }
+float dt_graph_frame_rate(const dt_graph_t *g) { return g->frame_rate; }
 int  dt_graph_num_nodes(const dt_graph_t *g) { return g->num_nodes; }
 int  dt_graph_num_exec(const dt_graph_t *g) { return g->num_exec; }
 int  dt_graph_exec_node(const dt_graph_t *g, int k) { return (k >= 0 && k < g->num_exec) ? g->exec[k] : -1; }
@@ -132,6 +136,21 @@ int dt_graph_img_param_get(const dt_graph_t *g, int m, float black[4], float whi
   noise[1] = p->noise_b;
   roi[0] = g->module[m].connector[0].roi.full_wd;
   roi[1] = g->module[m].connector[0].roi.full_ht;
+  return 0;
+}
+
+// capture metadata: exposure (s), aperture, iso, focal length; maker, model, datetime
+int dt_graph_img_meta_get(const dt_graph_t *g, int m, float meta[4], char maker[32], char model[32], char datetime[20])
+{
+  if(m < 0 || m >= g->num_modules) return 1;
+  const dt_image_params_t *p = &g->module[m].img_param;
+  meta[0] = p->exposure;
+  meta[1] = p->aperture;
+  meta[2] = p->iso;
+  meta[3] = p->focal_length;
+  memcpy(maker, p->maker, 32);
+  memcpy(model, p->model, 32);
+  memcpy(datetime, p->datetime, 20);
   return 0;
 }
 

@@ -303,6 +303,7 @@ struct dt_graph_t
   char basedir[1024];
 
   int frame, frame_cnt;
+  float frame_rate;         // frames per second of a video source (0: stills)
   dt_graph_run_t runflags;
 
   // device state

@@ -56,7 +56,7 @@ int main(int argc, char *argv[])
 {
   const char *cfg = 0, *fmt = "o-pfm", *fname = "output";
   int prim = parse_prim("sRGB"), trc = parse_prim("sRGB");   // cli/main.c:58-59
-  int perf = 0, dev = 0, max_wd = 0, max_ht = 0;
+  int perf = 0, dev = 0, max_wd = 0, max_ht = 0, last_only = 0;
   for(int i = 1; i < argc; i++)
   {
     if(!strcmp(argv[i], "-g") && i + 1 < argc) cfg = argv[++i];
@@ -68,12 +68,13 @@ int main(int argc, char *argv[])
     else if(!strcmp(argv[i], "--device") && i + 1 < argc) dev = atoi(argv[++i]);
     else if(!strcmp(argv[i], "--width") && i + 1 < argc) max_wd = (int)atof(argv[++i]);
     else if(!strcmp(argv[i], "--height") && i + 1 < argc) max_ht = (int)atof(argv[++i]);
+    else if(!strcmp(argv[i], "--last-frame-only")) last_only = 1;
     else if(!strcmp(argv[i], "--version")) { printf("vkdt-cli (raw-development path, HIP gfx950)\n"); return 0; }
   }
   if(!cfg)
   {
     fprintf(stderr, "usage: %s -g <graph.cfg> [--format o-pfm] [--filename <base>] [--width <px>] [--height <px>]"
-                    " [-d perf]\n", argv[0]);
+                    " [--last-frame-only] [-d perf]\n", argv[0]);
     return 1;
   }
   if(strcmp(fmt, "o-pfm")) { fprintf(stderr, "[cli] output format %s not supported (o-pfm only)\n", fmt); return 1; }
@@ -94,8 +95,34 @@ int main(int argc, char *argv[])
   dt_graph_disconnect_display_modules(g);
   dt_module_set_param_string(g->module + m, dt_token("filename"), fname);
   dt_graph_set_export_size(g, m, max_wd, max_ht);
+  // a video source publishes its frame count in the roi pass: plan first, then
+  // name the outputs <base>_%04d per frame unless only the last is wanted
+  // (graph-export.c:213-230,251-316)
+  if(dt_graph_plan(g, s_graph_run_all)) { fprintf(stderr, "[cli] plan failed: %s\n", dt_graph_last_error(g)); goto out; }
+  const int frames = g->frame_cnt > 1 ? g->frame_cnt : 1;
+  char name[2048];
+  if(frames > 1 && !last_only)
+  {
+    snprintf(name, sizeof(name), "%s_%04d", fname, 0);
+    dt_module_set_param_string(g->module + m, dt_token("filename"), name);
+  }
   const double t0 = now_ms();
-  if(dt_graph_run(g, s_graph_run_all)) { fprintf(stderr, "[cli] run failed: %s\n", dt_graph_last_error(g)); goto out; }
+  g->frame = 0;
+  if(dt_graph_run(g, s_graph_run_all & ~(frames > 1 && last_only ? s_graph_run_download_sink : 0)))
+  { fprintf(stderr, "[cli] run failed: %s\n", dt_graph_last_error(g)); goto out; }
+  for(int f = 1; f < frames; f++)
+  {
+    g->frame = f;
+    if(!last_only) snprintf(name, sizeof(name), "%s_%04d", fname, f);
+    else snprintf(name, sizeof(name), "%s", fname);
+    dt_module_set_param_string(g->module + m, dt_token("filename"), name);
+    const int dl = !last_only || f == frames - 1;
+    if(dt_graph_run(g, s_graph_run_record_cmd_buf | s_graph_run_upload_source | s_graph_run_wait_done |
+                         (dl ? s_graph_run_download_sink : 0)))
+    { fprintf(stderr, "[cli] frame %d failed: %s\n", f, dt_graph_last_error(g)); goto out; }
+    fprintf(stderr, "\r[cli] processing frame %d/%d", f, frames - 1);
+  }
+  if(frames > 1) fprintf(stderr, "\n");
   const double t1 = now_ms();
   if(perf)
   {

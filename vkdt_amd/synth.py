@@ -251,3 +251,68 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
         else:
             f.write(np.ascontiguousarray(cfa, dtype="<u2").tobytes())
     return path
+
+
+def pack_msb16(samples, bits):
+    """samples (uint16) as the bit stream of Magic Lantern raw frames: each
+    sample's low `bits` bits most significant first, cut into 16-bit words
+    stored little endian"""
+    s = np.ascontiguousarray(samples, np.uint16).ravel()
+    b = np.unpackbits(s.astype(">u2").view(np.uint8).reshape(-1, 2), axis=1)[:, 16 - bits:].ravel()
+    b = np.concatenate([b, np.zeros((-b.size) % 16, np.uint8)])
+    words = np.packbits(b.reshape(-1, 16), axis=1)             # [hi, lo] per word
+    return words[:, ::-1].tobytes()                           # little endian
+
+
+def write_mlv(path, frames, bits=14, lj92=None, black=2048, white=15000, fps=(25000, 1000),
+              camera="Canon EOS 5D Mark III", iso=800, shutter_us=20000, aperture=280, focal=50,
+              order=None, junk_before=(), chunk_split=None, frame_space=0, header_frames=None):
+    """Write a Magic Lantern raw video clip of uint16 CFA frames (H x W, rggb).
+
+    ``lj92`` None packs ``bits`` per sample; a dict (components, predictor)
+    stores each frame as one lossless-JPEG stream (class flag 0x20).
+    ``order`` permutes the frames' positions in the file (timestamps keep
+    the frame index, so a reader must sort); ``junk_before`` puts a block of
+    unknown type before those frame indices; ``chunk_split`` k writes frames
+    from index k on into ``<name>.M00``; ``header_frames`` overrides the
+    frame count in the file header."""
+    import struct
+    from vkdt_amd import lj92 as L
+    H, W = frames[0].shape
+    n = len(frames)
+    vclass = 0x01 | (0x20 if lj92 else 0)
+
+    def mlvi():
+        cnt = n if header_frames is None else header_frames
+        return struct.pack("<4sI8sQHHIHHIIII", b"MLVI", 52, b"v2.0\0\0\0\0", 0x1234, 0, 1, 0, vclass, 0,
+                           cnt, 0, fps[0], fps[1])
+
+    raw_info = struct.pack("<II7i4i4i2i2i18ii", 1, 0, H, W, W * bits // 8, W * H * bits // 8, bits, black, white,
+                           0, 0, W, H, 0, 0, H, W, 0, 0, 0x02010100, 0, *([0] * 18), 1100)
+    meta = [struct.pack("<4sIQHH", b"RAWI", 180, 0, W, H) + raw_info,
+            struct.pack("<4sIQ32sI32s", b"IDNT", 84, 0, camera.encode(), 0x80000285, b"123"),
+            struct.pack("<4sIQIIIIQ", b"EXPO", 40, 0, 0, iso, iso, 0, shutter_us),
+            struct.pack("<4sIQHHHBBII32s32s", b"LENS", 96, 0, focal, 65535, aperture, 0, 1, 0, 0, b"EF50mm f/1.8", b""),
+            struct.pack("<4sIQ10H8s", b"RTCI", 44, 0, 12, 34, 10, 2, 11, 126, 3, 0, 0, 0, b"UTC")]
+
+    def vidf(k):
+        f = np.ascontiguousarray(frames[k], np.uint16)
+        if lj92:
+            data = L.encode(f, 16 if bits > 14 else 14, lj92.get("predictor", 1), lj92.get("components", 1))
+        else:
+            data = pack_msb16(f, bits)
+        pad = b"\0" * frame_space
+        body = struct.pack("<IHHHHI", k, 0, 0, 0, 0, frame_space) + pad + data
+        return struct.pack("<4sIQ", b"VIDF", 16 + len(body), 1000 + 40000 * k) + body
+
+    pos = list(order) if order is not None else list(range(n))
+    split = chunk_split if chunk_split is not None else n + 1
+    chunks = [bytearray(mlvi() + b"".join(meta)), bytearray(mlvi())]
+    for i, k in enumerate(pos):
+        c = chunks[0] if i < split else chunks[1]
+        if k in junk_before:
+            c += struct.pack("<4sIQ", b"ZZZZ", 40, 0) + b"\x5a" * 24
+        c += vidf(k)
+    open(path, "wb").write(bytes(chunks[0]))
+    if chunk_split is not None:
+        open(path[:-2] + "00", "wb").write(bytes(chunks[1]))
