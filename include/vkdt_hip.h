@@ -37,13 +37,15 @@ enum
 // (graph-run-nodes-allocate.h:39-96): ui16 = R16_UNORM, f16 = R16F/RGBA16F,
 // f32 = R32F/RGBA32F.  atom: the accumulators that kernels add into with
 // atomics (the reference's R32F float atomics, :53), held here as FP64 --
-// 8 bytes per channel -- so per-frame sums do not lose precision.
+// 8 bytes per channel -- so per-frame sums do not lose precision.  ui32:
+// R32_UINT counters (histograms the reference adds into with imageAtomicAdd).
 enum
 {
   VKDT_FMT_UI16 = 1,
   VKDT_FMT_F16  = 2,
   VKDT_FMT_F32  = 3,
   VKDT_FMT_ATOM = 4,
+  VKDT_FMT_UI32 = 5,
 };
 
 // one connector's device buffer.  Replaces the descriptor-set image

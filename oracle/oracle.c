@@ -1685,6 +1685,129 @@ static int rh_map(const float *par, const or_buf_t *c)
   return 0;
 }
 
+/* ------------------------------------------------------------------------
+ * hist (src/pipe/modules/hist) and autoexp (src/pipe/modules/autoexp):
+ * counters are doubles holding the u32 values of the reference's
+ * R32_UINT images and uint buffers (exact below 2^53; wrapped to 32 bits
+ * where the shaders compute in uint).
+ * --------------------------------------------------------------------- */
+
+// float -> row as uint() + clamp(0, n-1); below 0 (and NaN) row 0
+static inline int hist_row(real v, int n)
+{
+  if(!(v > R(0))) return 0;
+  return v >= (real)(n - 1) ? n - 1 : (int)v;
+}
+
+// hist/collect.comp:13-25
+static int hist_collect(const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  double *h = (double *)out->data;
+  const real scale = (real)out->wd / (real)in->wd;
+  for(int y = 0; y < in->ht; y++) for(int x = 0; x < in->wd; x++)
+  {
+    const v4 rgb = texel(in, x, y, 0);
+    int op = (int)(scale * (real)x);
+    op = op < 0 ? 0 : (op > out->wd - 1 ? out->wd - 1 : op);
+    const real fh = (real)out->ht;
+    h[(size_t)hist_row((R(1.0) - rgb.x) * fh + R(0.5), out->ht) * out->wd + op] += 1.0;
+    h[(size_t)hist_row((R(1.0) - rgb.y) * fh + R(0.5), out->ht) * out->wd + op] += 1024.0;
+    h[(size_t)hist_row((R(1.0) - rgb.z) * fh + R(0.5), out->ht) * out->wd + op] += 1048576.0;
+  }
+  return 0;
+}
+
+// hist/map.comp:13-24
+static int hist_map(const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  const double *h = (const double *)in->data;
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const uint32_t cnt = (uint32_t)(uint64_t)h[(size_t)y * in->wd + x];
+    v4 o = { POW(R(1.0) / R(255.0) * (real)(cnt & 1023u), R(0.3)),
+             POW(R(1.0) / R(255.0) * (real)((cnt >> 10) & 1023u), R(0.3)),
+             POW(R(1.0) / R(255.0) * (real)((cnt >> 20) & 1023u), R(0.3)), R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
+// autoexp/hist.comp:18-25: luminance not >= 2^-15 (NaN too) is bin 0, infinite bin 255
+static inline int ae_bin(real r, real g, real b)
+{
+  const real l = R(2.62700212e-01) * r + R(6.77998072e-01) * g + R(5.93017165e-02) * b;
+  if(!(l >= R(3.0517578125e-05))) return 0;
+  const real normalized = (LOG2(l) - R(-15.0)) / R(20.0);
+  const real nb = normalized * R(254.0) + R(1.0);   // saturating uint(), infinite l included
+  return nb >= R(255.0) ? 255 : (int)(uint32_t)nb;
+}
+
+// autoexp/hist.comp:37-42: the radial thinning
+static inline int ae_mask(int x, int y, int wd, int ht)
+{
+  const real dx = (real)(wd / 2) - (real)x, dy = (real)(ht / 2) - (real)y;
+  const real d = SQRT(dx * dx + dy * dy);
+  const real s = smoothstepr(R(0.1) * (real)wd / R(2.0), (real)(wd / 2), d);
+  uint32_t mx = (uint32_t)(s * R(13.0)), my = (uint32_t)(s * R(7.0));
+  mx = mx > 1u ? mx : 1u;
+  my = my > 1u ? my : 1u;
+  return ((uint32_t)x % mx) == 0u && ((uint32_t)y % my) == 0u;
+}
+
+// autoexp/hist.comp:27-60 over the whole image: 256 bins and the count
+static int ae_hist(const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 1;
+  double *h = (double *)out->data;
+  for(int y = 0; y < in->ht; y++) for(int x = 0; x < in->wd; x++)
+  {
+    if(!ae_mask(x, y, in->wd, in->ht)) continue;
+    const v4 v = texel(in, x, y, 0);
+    const int bin = ae_bin(v.x, v.y, v.z);
+    if(bin > 0) { h[bin] += 1.0; h[256] += 1.0; }
+  }
+  return 0;
+}
+
+// autoexp/lum.comp:27-56; params vup, vdown, timediff (float), reset, frame (int)
+static int ae_lum(const float *par, const or_buf_t *c)
+{
+  const double *h = (const double *)c[1].data;
+  double *lum = (double *)c[2].data;
+  const int32_t reset = ((const int32_t *)par)[3];
+  const uint32_t frame = ((const uint32_t *)par)[4];
+  uint32_t sum = 0;   // the shader's uint tree sum of count * bin
+  for(int i = 0; i < 256; i++) sum += (uint32_t)(uint64_t)h[i] * (uint32_t)i;
+  const uint32_t count = (uint32_t)(uint64_t)h[256], bin0 = (uint32_t)(uint64_t)h[0];
+  const real num_bright = FMAX((real)(float)count - (real)(float)bin0, R(1.0));
+  const real average_bin = ((real)(float)sum / num_bright) - R(1.0);
+  const real average_l = EXP2(((average_bin / R(254.0)) * R(20.0)) + R(-15.0));
+  real ml = average_l;
+  real last = (reset == 1 || frame == 0u) ? ml : (real)lum[0];
+  if(isnan(last) || isinf(last)) last = ml;
+  const real tau = -R(par[2]) * (ml > last ? R(par[0]) : R(par[1]));
+  ml = last + (ml - last) * (R(1.0) - EXP(tau));
+  lum[0] = (double)ml;
+  return 0;
+}
+
+// autoexp/exp.comp:22-37
+static int ae_exp(const or_buf_t *c)
+{
+  const or_buf_t *in = c + 0, *out = c + 3;
+  const double *lum = (const double *)c[2].data;
+  const real e = FMIN(FMAX(R(0.18) / (real)lum[0], R(1e-3)), R(1e3));
+  for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
+  {
+    const v4 v = fetch(in, x, y, 0);
+    const v4 o = { v.x * e, v.y * e, v.z * e, R(1.0) };
+    store(out, x, y, 0, o);
+  }
+  return 0;
+}
+
 // nprof/main.c:84-140: screen bins by pairs, then a count-weighted line
 int or_nprof_fit(const double *h, int wd, float *a_out, float *b_out)
 {
@@ -1774,6 +1897,11 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   if(IS("shared", "resample"))  return nconn >= 2 ? sh_resample(conn) : -1;
   if(IS("rawhist", "collect"))  return nconn >= 2 ? rh_collect(par, conn, wd, ht) : -1;
   if(IS("rawhist", "map"))      return nconn >= 2 ? rh_map(par, conn) : -1;
+  if(IS("hist", "collect"))     return nconn >= 2 ? hist_collect(conn) : -1;
+  if(IS("hist", "map"))         return nconn >= 2 ? hist_map(conn) : -1;
+  if(IS("autoexp", "hist"))     return nconn >= 2 ? ae_hist(conn) : -1;
+  if(IS("autoexp", "lum"))      return nconn >= 3 ? ae_lum(par, conn) : -1;
+  if(IS("autoexp", "exp"))      return nconn >= 4 ? ae_exp(conn) : -1;
 #undef IS
   return -1;
 }

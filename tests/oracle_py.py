@@ -75,7 +75,9 @@ def orbuf(arr, fmt, chan, planes=1):
         ht, wd = arr.shape[1], arr.shape[2]
     else:
         ht, wd = arr.shape[0], arr.shape[1]
-    return OrBuf(arr.ctypes.data, wd, ht, chan, fmt, planes)
+    b = OrBuf(arr.ctypes.data, wd, ht, chan, fmt, planes)
+    b._keep = arr   # the pointer is only valid while the array lives (callers pass temporaries)
+    return b
 
 
 def node(bits, module, kernel, pc, params, bufs, wd, ht, dp=1, storage_f16=True):

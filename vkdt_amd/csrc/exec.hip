@@ -97,6 +97,11 @@ static const reg_entry g_registry[] = {
   { tkn("hilite"),   tkn("doub"),     launch_hilite_doub },
   { tkn("rawhist"),  tkn("collect"),  launch_rawhist_collect },
   { tkn("rawhist"),  tkn("map"),      launch_rawhist_map },
+  { tkn("hist"),     tkn("collect"),  launch_hist_collect },
+  { tkn("hist"),     tkn("map"),      launch_hist_map },
+  { tkn("autoexp"),  tkn("hist"),     launch_autoexp_hist },
+  { tkn("autoexp"),  tkn("lum"),      launch_autoexp_lum },
+  { tkn("autoexp"),  tkn("exp"),      launch_autoexp_exp },
   { tkn("demosaic"), tkn("halfsize"), launch_demosaic_halfsize },
   { tkn("shared"),   tkn("resample"), launch_shared_resample },
   { tkn("colenc"),   tkn("main"),     launch_colenc_main },

@@ -28,7 +28,7 @@ int vkdt_set_error(int code, const char *fmt, ...);
 // bytes per channel of a buffer format
 static inline size_t vkdt_fmt_bytes(uint32_t fmt)
 {
-  return fmt == VKDT_FMT_ATOM ? 8 : (fmt == VKDT_FMT_F32 ? 4 : 2);
+  return fmt == VKDT_FMT_ATOM ? 8 : ((fmt == VKDT_FMT_F32 || fmt == VKDT_FMT_UI32) ? 4 : 2);
 }
 
 // element-pitch device view of a connector
@@ -89,6 +89,11 @@ int launch_hilite_reduce(const node_call &);
 int launch_hilite_assemble(const node_call &);
 int launch_hilite_doub(const node_call &);
 int launch_rawhist_collect(const node_call &);
+int launch_hist_collect(const node_call &);
+int launch_hist_map(const node_call &);
+int launch_autoexp_hist(const node_call &);
+int launch_autoexp_lum(const node_call &);
+int launch_autoexp_exp(const node_call &);
 int launch_rawhist_map(const node_call &);
 int launch_demosaic_halfsize(const node_call &);
 int launch_shared_resample(const node_call &);

@@ -12,7 +12,7 @@ import numpy as np
 from . import lib_path
 
 OK, INCOMPLETE, OOM, DEVICE = 0, -1, -2, -3
-FMT_UI16, FMT_F16, FMT_F32, FMT_ATOM = 1, 2, 3, 4   # atom: FP64 accumulators
+FMT_UI16, FMT_F16, FMT_F32, FMT_ATOM, FMT_UI32 = 1, 2, 3, 4, 5   # atom: FP64 accumulators; ui32: counters
 
 
 class Buf(C.Structure):
@@ -103,7 +103,7 @@ class DeviceBuffer:
 
     def __init__(self, wd, ht, chan, fmt, planes=1):
         self.wd, self.ht, self.chan, self.fmt, self.planes = wd, ht, chan, fmt, planes
-        self.esize = {FMT_ATOM: 8, FMT_F32: 4}.get(fmt, 2) * chan
+        self.esize = {FMT_ATOM: 8, FMT_F32: 4, FMT_UI32: 4}.get(fmt, 2) * chan
         self.plane_bytes = wd * ht * self.esize
         self.nbytes = self.plane_bytes * planes
         p = C.c_void_p()
@@ -114,7 +114,8 @@ class DeviceBuffer:
         return Buf(self.ptr, self.wd, self.ht, self.wd, self.chan, self.fmt, self.planes, self.plane_bytes)
 
     def dtype(self):
-        return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32, FMT_ATOM: np.float64}[self.fmt]
+        return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32, FMT_ATOM: np.float64,
+                FMT_UI32: np.uint32}[self.fmt]
 
     def shape(self):
         s = (self.planes, self.ht, self.wd) if self.planes > 1 else (self.ht, self.wd)

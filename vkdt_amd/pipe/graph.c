@@ -65,6 +65,7 @@ uint32_t dt_connector_hip_format(const dt_connector_t *c)
   if(c->format == dt_token("f16")) return VKDT_FMT_F16;
   if(c->format == dt_token("f32")) return VKDT_FMT_F32;
   if(c->format == dt_token("atom")) return VKDT_FMT_ATOM;
+  if(c->format == dt_token("ui32") || c->format == dt_token("u32")) return VKDT_FMT_UI32;
   return 0;
 }
 
@@ -852,9 +853,11 @@ static int allocate(dt_graph_t *g)
       slot[ns].node = ni;
       slot[ns].conn = c;
       slot[ns].size = (dt_connector_bufsize(cc, cc->roi.wd, cc->roi.ht) * planes + 255) & ~(uint64_t)255;
-      // sources keep their upload for repeated runs: never aliased
-      slot[ns].beg = node_is_source(n) ? -1 : k;
-      slot[ns].end = node_is_source(n) ? INF : last;
+      // sources keep their upload for repeated runs, protected buffers their
+      // contents from frame to frame: never aliased
+      const int keep = node_is_source(n) || (cc->flags & s_conn_protected);
+      slot[ns].beg = keep ? -1 : k;
+      slot[ns].end = keep ? INF : last;
       ns++;
     }
   }
