@@ -39,6 +39,7 @@ enum
 // atomics (the reference's R32F float atomics, :53), held here as FP64 --
 // 8 bytes per channel -- so per-frame sums do not lose precision.  ui32:
 // R32_UINT counters (histograms the reference adds into with imageAtomicAdd).
+// ui8: R8G8B8A8_UNORM, the 8-bit export images.
 enum
 {
   VKDT_FMT_UI16 = 1,
@@ -46,6 +47,7 @@ enum
   VKDT_FMT_F32  = 3,
   VKDT_FMT_ATOM = 4,
   VKDT_FMT_UI32 = 5,
+  VKDT_FMT_UI8  = 6,
 };
 
 // one connector's device buffer.  Replaces the descriptor-set image
@@ -184,6 +186,20 @@ int vkdt_hip_lj92_decode(const uint8_t *data, size_t data_len, const vkdt_lj92_t
 // Replaces the host loop of i-mlv/video_mlv.c:262-276 (mlv_get_frame); the
 // bytes are uploaded as stored and unpacked on `stream`, which it waits for.
 int vkdt_hip_unpack_msb16(const uint8_t *packed, size_t bytes, void *out, uint32_t n, int bits, void *stream);
+
+// Baseline JPEG of an 8-bit rgba image on the device (`pitch` pixels a row):
+// the o-jpg sink's encoder.  Replaces the libjpeg calls of
+// o-jpg/main.c:101-160 (jpeg_set_quality, sampling, optimised Huffman
+// codes, 300 dpi JFIF) with the same published steps on the GPU: the file
+// holds the tables and entropy-coded data libjpeg writes for the same image
+// and settings.  hs x vs luma sampling: 1x1 (4:4:4), 2x1 (4:2:2) or 2x2
+// (4:2:0).  *out (malloc'd, free with vkdt_jpg_free) gets the whole file.
+int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, uint32_t pitch, int quality, int hs, int vs,
+                        uint8_t **out, size_t *out_len, void *stream);
+void vkdt_jpg_free(uint8_t *p);
+// IJG quality scaling of the T.81 Annex K tables (natural order), as
+// jpeg_set_quality(quality, force_baseline) computes them
+void vkdt_jpg_quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]);
 
 // ---- batched develop ------------------------------------------------------
 // The raw-development graph i-raw -> denoise -> demosaic -> colour ->

@@ -28,7 +28,7 @@ int vkdt_set_error(int code, const char *fmt, ...);
 // bytes per channel of a buffer format
 static inline size_t vkdt_fmt_bytes(uint32_t fmt)
 {
-  return fmt == VKDT_FMT_ATOM ? 8 : ((fmt == VKDT_FMT_F32 || fmt == VKDT_FMT_UI32) ? 4 : 2);
+  return fmt == VKDT_FMT_ATOM ? 8 : ((fmt == VKDT_FMT_F32 || fmt == VKDT_FMT_UI32) ? 4 : (fmt == VKDT_FMT_UI8 ? 1 : 2));
 }
 
 // element-pitch device view of a connector

@@ -12,7 +12,7 @@ import numpy as np
 from . import lib_path
 
 OK, INCOMPLETE, OOM, DEVICE = 0, -1, -2, -3
-FMT_UI16, FMT_F16, FMT_F32, FMT_ATOM, FMT_UI32 = 1, 2, 3, 4, 5   # atom: FP64 accumulators; ui32: counters
+FMT_UI16, FMT_F16, FMT_F32, FMT_ATOM, FMT_UI32, FMT_UI8 = 1, 2, 3, 4, 5, 6   # atom: FP64 accumulators; ui32: counters
 
 
 class Buf(C.Structure):
@@ -82,6 +82,9 @@ def lib():
         L.vkdt_lj92_decode_host.argtypes = [vp, C.POINTER(Lj92Tile), vp, C.c_uint32, C.c_uint32, C.c_uint32]
         L.vkdt_hip_lj92_decode.argtypes = [vp, sz, C.POINTER(Lj92Tile), C.c_int, C.POINTER(Lj92Image), C.c_int, vp]
         L.vkdt_hip_unpack_msb16.argtypes = [vp, sz, vp, C.c_uint32, C.c_int, vp]
+        L.vkdt_hip_jpg_encode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_void_p), C.POINTER(sz), vp]
+        L.vkdt_jpg_free.argtypes = [vp]
         L.vkdt_hip_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -103,7 +106,7 @@ class DeviceBuffer:
 
     def __init__(self, wd, ht, chan, fmt, planes=1):
         self.wd, self.ht, self.chan, self.fmt, self.planes = wd, ht, chan, fmt, planes
-        self.esize = {FMT_ATOM: 8, FMT_F32: 4, FMT_UI32: 4}.get(fmt, 2) * chan
+        self.esize = {FMT_ATOM: 8, FMT_F32: 4, FMT_UI32: 4, FMT_UI8: 1}.get(fmt, 2) * chan
         self.plane_bytes = wd * ht * self.esize
         self.nbytes = self.plane_bytes * planes
         p = C.c_void_p()
@@ -115,7 +118,7 @@ class DeviceBuffer:
 
     def dtype(self):
         return {FMT_UI16: np.uint16, FMT_F16: np.float16, FMT_F32: np.float32, FMT_ATOM: np.float64,
-                FMT_UI32: np.uint32}[self.fmt]
+                FMT_UI32: np.uint32, FMT_UI8: np.uint8}[self.fmt]
 
     def shape(self):
         s = (self.planes, self.ht, self.wd) if self.planes > 1 else (self.ht, self.wd)
@@ -224,6 +227,16 @@ def unpack_msb16(packed, out, n, bits, stream=None):
     """bit-packed samples (bytes, Magic Lantern layout) into a uint16 device buffer"""
     buf = (C.c_uint8 * max(1, len(packed))).from_buffer_copy(packed or b"\0")
     check(lib().vkdt_hip_unpack_msb16(buf, len(packed), out, n, bits, stream), "unpack")
+
+
+def jpg_encode(rgba_dev, wd, ht, pitch, quality=95, hs=1, vs=1, stream=None):
+    """baseline JPEG (bytes) of an 8-bit rgba device image"""
+    p, n = C.c_void_p(), C.c_size_t()
+    check(lib().vkdt_hip_jpg_encode(rgba_dev, wd, ht, pitch, quality, hs, vs, C.byref(p), C.byref(n), stream),
+          "jpg encode")
+    out = C.string_at(p, n.value)
+    lib().vkdt_jpg_free(p)
+    return out
 
 
 def lj92_decode(blob, tiles, images, stream=None):

@@ -66,6 +66,7 @@ uint32_t dt_connector_hip_format(const dt_connector_t *c)
   if(c->format == dt_token("f32")) return VKDT_FMT_F32;
   if(c->format == dt_token("atom")) return VKDT_FMT_ATOM;
   if(c->format == dt_token("ui32") || c->format == dt_token("u32")) return VKDT_FMT_UI32;
+  if(c->format == dt_token("ui8") || c->format == dt_token("u8")) return VKDT_FMT_UI8;
   return 0;
 }
 
