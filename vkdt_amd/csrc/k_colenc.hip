@@ -84,6 +84,17 @@ __global__ void __launch_bounds__(256) k_colenc(dimg in, dimg out, enc_args a)
   }
   const float4 o = make_float4(enc_trc(r, a.trc), enc_trc(g, a.trc), enc_trc(b, a.trc), 1.0f);
   if(out.fmt == DIMG_F32) st_f4(out, x, y, o);
+  else if(out.fmt == VKDT_FMT_UI8)
+  { // R8G8B8A8_UNORM: round(clamp(v, 0, 1) * 255), ties to even
+    const float v[4] = { o.x, o.y, o.z, o.w };
+    uint32_t u = 0;
+    for(int k = 0; k < 4; k++)
+    {
+      const float c = v[k] > 0.0f ? (v[k] < 1.0f ? v[k] : 1.0f) : 0.0f;   // NaN -> 0
+      u |= (uint32_t)__builtin_rintf(c * 255.0f) << (8 * k);
+    }
+    ((uint32_t *)out.p)[(long)y * out.pitch + x] = u;
+  }
   else st_h4(out, x, y, o);
 }
 
@@ -91,7 +102,9 @@ int launch_colenc_main(const node_call &c)
 {
   if(int e = check_blob(c, 0, 2 * 4, "colenc/main")) return e;
   if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_F32, 4, 1, "colenc/main input")) return e;
-  if(int e = check_conn2(c, 1, VKDT_FMT_F16, VKDT_FMT_F32, 4, 1, "colenc/main output")) return e;
+  if(c.nconn > 1 && c.conn[1].format == VKDT_FMT_UI8)
+  { if(int e = check_conn(c, 1, VKDT_FMT_UI8, 4, 1, "colenc/main output")) return e; }
+  else if(int e = check_conn2(c, 1, VKDT_FMT_F16, VKDT_FMT_F32, 4, 1, "colenc/main output")) return e;
   const int32_t *p = (const int32_t *)c.params;
   enc_args a;
   // primaries the reference encodes (colenc/main.comp:26-34); others stay rec2020

@@ -243,6 +243,9 @@ typedef struct dt_module_so_t
   // optional, no reference counterpart: fill the source's device buffer on
   // `stream` directly (GPU unpack); 0 done, 1 use read_source, -1 error
   int  (*read_source_hip)(dt_module_t *, void *dev, void *stream);
+  // optional, no reference counterpart: consume the sink's device buffer on
+  // `stream` directly (GPU encode); 0 done, 1 use download + write_sink, -1 error
+  int  (*write_sink_hip)(dt_module_t *, const void *dev, void *stream);
   void (*write_sink)(dt_module_t *, void *, dt_write_sink_params_t *);
   dt_ui_param_t *param[DT_MAX_PARAMS];
   int num_params;

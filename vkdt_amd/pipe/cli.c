@@ -57,6 +57,7 @@ int main(int argc, char *argv[])
   const char *cfg = 0, *fmt = "o-pfm", *fname = "output";
   int prim = parse_prim("sRGB"), trc = parse_prim("sRGB");   // cli/main.c:58-59
   int perf = 0, dev = 0, max_wd = 0, max_ht = 0, last_only = 0;
+  float quality = 0.0f;   // > 0: the output's quality parameter
   for(int i = 1; i < argc; i++)
   {
     if(!strcmp(argv[i], "-g") && i + 1 < argc) cfg = argv[++i];
@@ -69,15 +70,17 @@ int main(int argc, char *argv[])
     else if(!strcmp(argv[i], "--width") && i + 1 < argc) max_wd = (int)atof(argv[++i]);
     else if(!strcmp(argv[i], "--height") && i + 1 < argc) max_ht = (int)atof(argv[++i]);
     else if(!strcmp(argv[i], "--last-frame-only")) last_only = 1;
+    else if(!strcmp(argv[i], "--quality") && i + 1 < argc) quality = (float)atof(argv[++i]);
     else if(!strcmp(argv[i], "--version")) { printf("vkdt-cli (raw-development path, HIP gfx950)\n"); return 0; }
   }
   if(!cfg)
   {
     fprintf(stderr, "usage: %s -g <graph.cfg> [--format o-pfm] [--filename <base>] [--width <px>] [--height <px>]"
-                    " [--last-frame-only] [-d perf]\n", argv[0]);
+                    " [--quality <q>] [--last-frame-only] [-d perf]\n", argv[0]);
     return 1;
   }
-  if(strcmp(fmt, "o-pfm")) { fprintf(stderr, "[cli] output format %s not supported (o-pfm only)\n", fmt); return 1; }
+  if(strcmp(fmt, "o-pfm") && strcmp(fmt, "o-jpg"))
+  { fprintf(stderr, "[cli] output format %s not supported (o-pfm, o-jpg)\n", fmt); return 1; }
 
   char exe[4096] = {0}, base[4096];
   if(readlink("/proc/self/exe", exe, sizeof(exe) - 1) <= 0) snprintf(exe, sizeof(exe), "%s", argv[0]);
@@ -90,10 +93,15 @@ int main(int argc, char *argv[])
   g->perf = perf;
   int ret = 1;
   if(dt_graph_read_config_ascii(g, cfg)) goto out;
-  const int m = dt_graph_replace_display_colour(g, dt_token("main"), dt_token("o-pfm"), prim, trc);
-  if(m < 0) { fprintf(stderr, "[cli] graph has no connected display:main\n"); goto out; }
+  const int m = dt_graph_replace_display_colour(g, dt_token("main"), dt_token(fmt), prim, trc);
+  if(m < 0) { fprintf(stderr, "[cli] graph has no connected display:main: %s\n", dt_graph_last_error(g)); goto out; }
   dt_graph_disconnect_display_modules(g);
   dt_module_set_param_string(g->module + m, dt_token("filename"), fname);
+  if(quality > 0.0f)
+  { // graph-export.c:240-241
+    const int qp = dt_module_get_param(g->module[m].so, dt_token("quality"));
+    if(qp >= 0) *(float *)dt_module_param_float(g->module + m, qp) = quality;
+  }
   dt_graph_set_export_size(g, m, max_wd, max_ht);
   // a video source publishes its frame count in the roi pass: plan first, then
   // name the outputs <base>_%04d per frame unless only the last is wanted

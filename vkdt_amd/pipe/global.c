@@ -134,6 +134,7 @@ static int load_so(dt_module_so_t *so, const char *dirname)
     so->check_params   = dlsym(so->dlhandle, "check_params");
     so->read_source    = dlsym(so->dlhandle, "read_source");
     so->read_source_hip = dlsym(so->dlhandle, "read_source_hip");
+    so->write_sink_hip = dlsym(so->dlhandle, "write_sink_hip");
     so->write_sink     = dlsym(so->dlhandle, "write_sink");
   }
   return 0;

@@ -1104,8 +1104,14 @@ int dt_graph_run(dt_graph_t *g, dt_graph_run_t run)
     for(int k = 0; k < g->num_exec; k++)
     {
       dt_node_t *n = g->node + g->exec[k];
-      if(!node_is_sink(n) || !n->module->so->write_sink) continue;
+      if(!node_is_sink(n) || !(n->module->so->write_sink || n->module->so->write_sink_hip)) continue;
       dt_connector_t *cc = n->connector + 0;
+      if(n->module->so->write_sink_hip)
+      { // the module consumes its device buffer itself (GPU encode)
+        const int rc = n->module->so->write_sink_hip(n->module, cc->dev, g->stream);
+        if(rc < 0) return gerr(g, -1, "write_sink_hip failed for %" PRItkn, dt_token_str(n->module->name));
+        if(rc == 0 || !n->module->so->write_sink) continue;
+      }
       const dt_connector_t *own = g->node[cc->connected.i].connector + cc->connected.c;
       const size_t bytes = dt_connector_bufsize(own, own->roi.wd, own->roi.ht) * MAX(1, own->array_length);
       uint8_t *mapped = (uint8_t *)g->staging + stoff;
