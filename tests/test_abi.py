@@ -41,7 +41,8 @@ def test_abi_version_and_registry():
                  ("llap", "curve"), ("llap", "reduce"), ("llap", "assemble"), ("llap", "colour"),
                  ("hilite", "half"), ("hilite", "reduce"), ("hilite", "assemble"), ("hilite", "doub"),
                  ("rawhist", "collect"), ("rawhist", "map"), ("demosaic", "halfsize"), ("shared", "resample"),
-                 ("colenc", "main"), ("demosaic", "rcd_conv"), ("demosaic", "rcd_fill")]:
+                 ("colenc", "main"), ("demosaic", "rcd_conv"), ("demosaic", "rcd_fill"), ("hist", "collect"),
+                 ("hist", "map"), ("autoexp", "hist"), ("autoexp", "lum"), ("autoexp", "exp")]:
         assert lib.vkdt_hip_node_supported(tok(m), tok(k)) == 1, (m, k)
     # kernels of modules outside this path are refused by name
     assert lib.vkdt_hip_node_supported(tok("demosaic"), tok("coldumb")) == 0
@@ -64,7 +65,11 @@ def test_launch_rejects_bad_nodes_without_touching_the_device():
 
 
 MODULE_CALLBACKS = {
-    "i-raw": ["modify_roi_out", "read_source", "init", "cleanup", "check_params"],
+    "i-raw": ["modify_roi_out", "read_source", "read_source_hip", "init", "cleanup", "check_params"],
+    "i-mlv": ["modify_roi_out", "read_source", "read_source_hip", "init", "cleanup"],
+    "hist": ["create_nodes", "modify_roi_in", "modify_roi_out"],
+    "autoexp": ["create_nodes", "commit_params", "init"],
+    "o-jpg": ["write_sink", "write_sink_hip"],
     "denoise": ["create_nodes", "modify_roi_in", "modify_roi_out", "check_params"],
     "demosaic": ["create_nodes", "modify_roi_in", "modify_roi_out", "check_params"],
     "colour": ["create_nodes", "modify_roi_in", "modify_roi_out", "commit_params", "init", "check_params"],
@@ -81,7 +86,7 @@ def test_module_dso_callbacks(mod):
     so = C.CDLL(os.path.join(PKG_DIR, "modules", mod, f"lib{mod}.so"))
     for cb in MODULE_CALLBACKS[mod]:
         assert hasattr(so, cb), (mod, cb)
-    for f in ("connectors", "params") if mod != "display" else ("connectors",):
+    for f in ("connectors", "params") if mod not in ("display", "hist") else ("connectors",):   # hist has no params
         assert os.path.exists(os.path.join(PKG_DIR, "modules", mod, f)), (mod, f)
 
 
