@@ -46,12 +46,17 @@ __global__ void __launch_bounds__(256) k_ll_curve(dimg in, dimg out, ll_args a, 
   *(uint2 *)(po + a.num_gamma * out.pstride) = w.u;
 }
 
-// all planes of four adjacent output texels per thread
-__global__ void __launch_bounds__(256) k_ll_reduce(dimg in, dimg out, int planes, int vec)
+// four adjacent output texels of one plane (blockIdx.z) per thread: the
+// planes' loads are independent, so they go to separate waves rather than
+// one thread's serial loop
+__global__ void __launch_bounds__(256) k_ll_reduce(dimg in, dimg out, int vec)
 {
   const int x = 4 * tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  ll_reduce_px4(in, out, planes, x, y, vec != 0);
+  dimg i1 = in, o1 = out;
+  i1.p = (__half *)in.p + (long)blockIdx.z * in.pstride;
+  o1.p = (__half *)out.p + (long)blockIdx.z * out.pstride;
+  ll_reduce_px4(i1, o1, 1, x, y, vec != 0);
 }
 
 #define AS_R 4   // fine rows per thread: 64 x 16 fine texels per work group
@@ -126,8 +131,7 @@ int launch_llap_reduce(const node_call &c)
   // 16-byte input rows and 8-byte output groups at every row and plane start
   const int vec = in.pitch % 8 == 0 && in.pstride % 8 == 0 && (uintptr_t)in.p % 16 == 0 &&
                   out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
-  hipLaunchKernelGGL(k_ll_reduce, grid2d((c.wd + 3) / 4, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     in, out, (int)c.dp, vec);
+  hipLaunchKernelGGL(k_ll_reduce, grid2d((c.wd + 3) / 4, c.ht, c.dp), dim3(BX, BY), 0, c.stream, in, out, vec);
   return launch_post("llap/reduce");
 }
 
