@@ -6,7 +6,8 @@
 //   k_jpg_coef    one lane per 8x8 block in MCU order: fixed-point YCbCr,
 //                 2:1 chroma boxes with the alternating bias, samples past
 //                 the edges replicated, the islow integer DCT and the
-//                 rounded quantisation, all in registers
+//                 rounded quantisation, all in registers; stored zigzag
+//                 ordered, so the passes below load a block in 8 vectors
 //   k_jpg_dummy   blocks of the last MCUs past a component's own extent:
 //                 zero, with the DC of the block to the left / above
 //   k_jpg_stats   symbol counts of the four tables (LDS, then global)
@@ -24,10 +25,6 @@
 
 namespace {
 
-__device__ static const uint8_t jz[64] = {   // zigzag position -> natural index
-  0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21,
-  28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54,
-  47, 55, 62, 63 };
 static const uint8_t jz_host[64] = {
   0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21,
   28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54,
@@ -136,12 +133,37 @@ __global__ void __launch_bounds__(64) k_jpg_coef(jpg_geom g, const int32_t *q8, 
 #pragma unroll
   for(int k = 0; k < 8; k++) jpg_fdct8<1, 8>(d + k);
   const int32_t *q = q8 + (c ? 64 : 0);
+  // stored in zigzag order, the order the entropy coder walks them
+  constexpr uint8_t zz[64] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21,
+    28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54,
+    47, 55, 62, 63 };
 #pragma unroll
   for(int k = 0; k < 64; k += 2)
   {
-    const int a = jpg_quant(d[k], q[k]), e = jpg_quant(d[k + 1], q[k + 1]);
+    const int a = jpg_quant(d[zz[k]], q[zz[k]]), e = jpg_quant(d[zz[k + 1]], q[zz[k + 1]]);
     *(uint32_t *)(o + k) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)e << 16);
   }
+}
+
+// a block's 64 coefficients (zigzag order) in registers: eight 16-byte loads
+struct jpg_blk
+{
+  uint4 w[8];
+  __device__ __forceinline__ int at(int k) const
+  {
+    const uint4 &v = w[k >> 3];
+    const int j = (k >> 1) & 3;
+    const uint32_t u = j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+    return (int)(int16_t)(uint16_t)(u >> (16 * (k & 1)));
+  }
+};
+
+DEV void jpg_load(jpg_blk &z, const int16_t *coef, int b)
+{
+  const uint4 *p = (const uint4 *)(coef + (long)b * 64);
+#pragma unroll
+  for(int i = 0; i < 8; i++) z.w[i] = p[i];
 }
 
 // the dummy blocks' DC: pass 1 the right edge (block to the left), pass 2 the
@@ -181,15 +203,17 @@ __global__ void __launch_bounds__(256) k_jpg_stats(jpg_geom g, const int16_t *co
     int c, bx, by, i;
     jpg_block(g, b, c, bx, by, i);
     const int t = c ? 1 : 0;
-    const int16_t *z = coef + (long)b * 64;
+    jpg_blk z;
+    jpg_load(z, coef, b);
     const int p = jpg_prev(g, b);
     const int last = p < 0 ? 0 : coef[(long)p * 64];
-    atomicAdd(h + (2 * t) * 257 + jpg_nbits(z[0] - last), 1u);
+    atomicAdd(h + (2 * t) * 257 + jpg_nbits(z.at(0) - last), 1u);
     uint32_t *ac = h + (2 * t + 1) * 257;
     int run = 0;
+#pragma unroll
     for(int k = 1; k < 64; k++)
     {
-      const int v = z[jz[k]];
+      const int v = z.at(k);
       if(!v) { run++; continue; }
       while(run > 15) { atomicAdd(ac + 0xf0, 1u); run -= 16; }
       atomicAdd(ac + ((run << 4) | jpg_nbits(v)), 1u);
@@ -216,7 +240,8 @@ __global__ void __launch_bounds__(256) k_jpg_code(jpg_geom g, const int16_t *coe
   const int t = c ? 1 : 0;
   const uint16_t *dcc = hc->code[2 * t], *acc_c = hc->code[2 * t + 1];
   const uint8_t *dcl = hc->len[2 * t], *acl = hc->len[2 * t + 1];
-  const int16_t *z = coef + (long)b * 64;
+  jpg_blk z;
+  jpg_load(z, coef, b);
   const int p = jpg_prev(g, b);
   const int last = p < 0 ? 0 : coef[(long)p * 64];
   // writer: acc holds the bits from word w on, `used` of them taken
@@ -237,13 +262,14 @@ __global__ void __launch_bounds__(256) k_jpg_code(jpg_geom g, const int16_t *coe
       w++;
     }
   };
-  const int diff = z[0] - last, n = jpg_nbits(diff);
+  const int diff = z.at(0) - last, n = jpg_nbits(diff);
   put(dcc[n], dcl[n]);
   put((uint32_t)(diff < 0 ? diff - 1 : diff) & ((1u << n) - 1), n);
   int run = 0;
+#pragma unroll
   for(int k = 1; k < 64; k++)
   {
-    const int v = z[jz[k]];
+    const int v = z.at(k);
     if(!v) { run++; continue; }
     while(run > 15) { put(acc_c[0xf0], acl[0xf0]); run -= 16; }
     const int s = jpg_nbits(v), sym = (run << 4) | s;
