@@ -223,10 +223,18 @@ def stage_bytes(d):
     L = [lw[l] * lh[l] for l in range(nl)]
     pyr = sum((ng + 1) * 2 * (L[l - 1] + L[l]) for l in range(2, nl))            # reduce 2..
     pyr += sum(4 * 2 * L[l] + 3 * 2 * L[l + 1] for l in range(1, nl - 1))        # assemble nl-2..1
+    hl = 0
+    if d.hl_on:   # half, reduce 1..L, assemble L-1..0, doub (rgba f16 levels of the block grid)
+        hq = [Q]
+        for _ in range(d.hl_levels):
+            hq.append(((W // b - 1) // 2 ** len(hq) + 1) * ((H // b - 1) // 2 ** len(hq) + 1))
+        hl = 2 * W * H + 8 * Q + sum(8 * (hq[l - 1] + hq[l]) for l in range(1, len(hq)))
+        hl += sum(8 * (2 * hq[l] + hq[l + 1]) for l in range(len(hq) - 1)) + 4 * W * H + 8 * Q
     return [2 * W * H + 8 * Q,                  # half: mosaic -> Q rgba
             4 * 16 * Q,                         # downcov + 3 down
             6 * 8 * Q,                          # assemble: 5 levels -> Q
             4 * W * H + 16 * Q,                 # doub: mosaic + 2 Q -> mosaic f16
+            hl,                                 # hilite: mosaic -> mosaic f16 over its wavelet levels
             2 * W * H + 8 * Q,                  # gauss: mosaic f16 -> block covariance
             10 * W * H + 8 * Q,                 # develop: mosaic + cov -> rgba f16
             8 * W * H + (ng + 1) * 2 * L[1],    # pyr1: rgba f16 -> level 1 planes
@@ -256,7 +264,7 @@ def run_batch(args):
     plain_ms = {}
     for (W, H, k) in ([] if args.no_plain else combos):
         na, nb = synth.noise_params(5e-4, 1e-6)
-        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp)
+        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp, hilite=args.hilite)
         g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)
         g.run(pipe.RUN_RECORD)
         ts = []
@@ -274,7 +282,7 @@ def run_batch(args):
     descs, keep = [], []
     for i, (W, H, k, a) in enumerate(work):
         na, nb = synth.noise_params(a, 1e-6)
-        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp)
+        g = plain_graph(dngs[(W, H, k)], k, na, nb, tmp, hilite=args.hilite)
         g.plan()
         d = batch.describe(g)
         g.close()
@@ -334,7 +342,8 @@ def run_batch(args):
         "config": {"workload": "batched: 64 mixed stills in one hipGraph (16x 4000x3000, 24x 6000x4000, "
                                "16x 8256x5504, 8x 9504x6336; 16 X-Trans by seeded shuffle; noise a~U[1e-4,1e-3])",
                    "images": len(descs), "cfa_pixels": P, "storage": "f16 intermediates, rgba f32 out",
-                   "graph": "i-raw>denoise(strength .5)>demosaic>colour>filmcurv>llap(clarity .2)",
+                   "graph": "i-raw>denoise(strength .5)>" + ("hilite>" if args.hilite else "")
+                            + "demosaic>colour>filmcurv>llap(clarity .2)",
                    "tune": dict(zip(names, tune))},
         "gpu_ms_per_step": round(gpu_ms, 3),
         "plain": None if not plain_ms else {
@@ -398,14 +407,14 @@ def run_serve(args, tune):
 
     def plan(path, a, black, white):
         na, nb = synth.noise_params(a, 1e-6, black, white)
-        g = plain_graph(path, "bayer", na, nb, tmp)
+        g = plain_graph(path, "bayer", na, nb, tmp, hilite=args.hilite)
         g.plan()
         d = batch.describe(g)
         g.close()
         return d, na, nb, a
 
     def time_plain(path, na, nb):
-        g = plain_graph(path, "bayer", na, nb, tmp)
+        g = plain_graph(path, "bayer", na, nb, tmp, hilite=args.hilite)
         g.run(pipe.RUN_ALL & ~pipe.RUN_DOWNLOAD)
         ts = []
         for _ in range(4):
@@ -462,7 +471,8 @@ def run_serve(args, tune):
     sizes = [b[0] for b in srv.batches]
     return {
         "workload": f"Poisson {len(work) / args.serve_seconds:.0f} req/s for {args.serve_seconds:g} s: "
-                    f"{STREAMS} streams of {FRAME_W}x{FRAME_H} RGGB 12-bit frames + 24-61 MP 14-bit stills",
+                    f"{STREAMS} streams of {FRAME_W}x{FRAME_H} RGGB 12-bit frames + 24-61 MP 14-bit stills"
+                    + (", hilite between denoise and demosaic" if args.hilite else ""),
         "requests": len(reqs), "stills": nst, "frames": len(reqs) - nst,
         "offered_mpix_per_s": round(px / args.serve_seconds / 1e6, 1),
         "served": s, "batches": len(sizes), "mean_batch": round(float(np.mean(sizes)), 2),
@@ -483,7 +493,7 @@ def main():
                          "plain: the per-image path on one still")
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
     ap.add_argument("--hilite", action="store_true",
-                    help="plain mode: insert hilite between denoise and demosaic")
+                    help="insert hilite between denoise and demosaic (the stock darkroom chain)")
     ap.add_argument("--noise-profile", action="store_true",
                     help="plain mode: time the noise-profile graph i-raw > rawhist > nprof instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -202,8 +202,8 @@ void vkdt_jpg_free(uint8_t *p);
 void vkdt_jpg_quant_tables(int quality, uint8_t luma[64], uint8_t chroma[64]);
 
 // ---- batched develop ------------------------------------------------------
-// The raw-development graph i-raw -> denoise -> demosaic -> colour ->
-// filmcurv -> llap of many stills or frames, of any sizes and CFAs, in one
+// The raw-development graph i-raw -> denoise -> [hilite ->] demosaic ->
+// colour -> filmcurv -> llap of many stills or frames, of any sizes and CFAs, in one
 // launch per stage.  No reference counterpart: in the reference each image
 // is its own graph run (graph.c:719 dt_graph_run per export), which is what
 // vkdt_hip_node_launch replaces one dispatch at a time.  The batch runs the
@@ -233,6 +233,10 @@ typedef struct vkdt_develop_image_t
   float    ll_params[4];      // llap/params: sigma, shadows, hilights, clarity
   int32_t  ll_numgamma;       // curve planes G-1 (llap/curve push constant)
   int32_t  ll_levels;         // pyramid levels nl (reduce nodes + 1)
+  int32_t  hl_on;             // 1: hilite runs between denoise and demosaic
+  float    hl_wb[4];          // hilite push constants: white balance
+  float    hl_params[3];      // hilite/params: white, desat, soft
+  int32_t  hl_levels;         // hilite reduce nodes (coarse levels of the block grid)
 }
 vkdt_develop_image_t;
 
@@ -249,7 +253,7 @@ int vkdt_hip_batch_scratch_bytes(const vkdt_develop_image_t *img, int n, size_t 
 // entries as written by vkdt_hip_batch_autotune, each
 //   persistent work groups per CU (0: one tile per work group)
 //   | rows per thread of the per-pixel stages (1..8) << 8.
-#define VKDT_BATCH_STAGES 9
+#define VKDT_BATCH_STAGES 10
 int vkdt_hip_batch_create(vkdt_batch_t **batch, const vkdt_develop_image_t *img, int n,
                           void *scratch, size_t scratch_bytes, const int32_t *tune);
 

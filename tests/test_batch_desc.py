@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from vkdt_amd import batch, pipe, synth
+from vkdt_amd import batch, hip, pipe, synth
 
 
 def planned(tmp_path, kind, W, H, extra=(), **kw):
@@ -91,21 +91,37 @@ def test_refuses_graphs_the_batch_does_not_run(tmp_path):
     g.close()
 
 
-def test_graph_with_hilite_is_left_to_the_per_node_path(tmp_path):
-    """hilite between denoise and demosaic is outside the batch's fused chain"""
-    cfa, _ = synth.mosaic(128, 96, 7, "bayer")
-    dng = str(tmp_path / "h.dng")
-    synth.write_dng(dng, cfa, "bayer")
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, hilite=True))
-    g.export("o-pfm", str(tmp_path / "out3"))
-    g.plan()
-    with pytest.raises(pipe.PipeError, match="chain"):
-        batch.describe(g)
+@pytest.mark.parametrize("kind,W,H", [("bayer", 128, 96), ("xtrans", 198, 132)])
+def test_hilite_between_denoise_and_demosaic(tmp_path, kind, W, H):
+    """hilite's push constants, params and coarse levels are read off its
+    nodes (hilite/main.c); the scratch grows by its buffers"""
+    g, _ = planned(tmp_path, kind, W, H, hilite=True, extra=["param:hilite:01:desat:0.5"])
+    d = batch.describe(g)
+    half = node(g, "hilite", "half")[0]
+    pf = np.frombuffer(half["pc"], np.float32)
+    assert d.hl_on == 1
+    np.testing.assert_array_equal(np.array(d.hl_wb, np.float32), pf[:4])
+    np.testing.assert_array_equal(np.array(d.hl_params, np.float32), np.float32([0.985, 0.5, 0.6]))
+    assert d.hl_levels == len(node(g, "hilite", "reduce")) == len(node(g, "hilite", "assemble"))
+    b = 3 if kind == "xtrans" else 2
+    assert (half["wd"], half["ht"]) == (W // b, H // b)
+    a = lambda v: (v + 255) // 256 * 256
+    hw, hh = [W // b], [H // b]
+    for _ in range(d.hl_levels):
+        hw.append((hw[-1] - 1) // 2 + 1)
+        hh.append((hh[-1] - 1) // 2 + 1)
+    extra = a((W + 7) // 8 * 8 * H * 2) + sum(a(hw[l] * hh[l] * 8) for l in range(d.hl_levels + 1))
+    extra += sum(a(hw[l] * hh[l] * 8) for l in range(d.hl_levels))
+    with_hl = batch.scratch_bytes([d])
+    d.hl_on = 0
+    assert batch.scratch_bytes([d]) + extra == with_hl
+    d.hl_on, d.hl_levels = 1, d.hl_levels + 1
+    with pytest.raises(hip.HipError, match="hilite levels"):
+        batch.scratch_bytes([d])
     g.close()
 
 
 def test_create_rejects_bad_input_without_touching_the_device(tmp_path):
-    from vkdt_amd import hip
     g, _ = planned(tmp_path, "bayer", 128, 96)
     d = batch.describe(g)
     with pytest.raises(hip.HipError, match="no mosaic or output"):

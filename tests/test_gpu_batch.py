@@ -21,6 +21,10 @@ IMAGES = [
     ("bayer", 258, 194, 43, dict(strength=0.8, clarity=0.5, numgamma=6, levels=5)),
     ("xtrans", 198, 132, 44, dict(extra=["param:demosaic:01:colour:1", "param:llap:01:shadows:0.7"])),
     ("bayer", 64, 48, 45, dict(extra=["param:colour:01:exposure:0.5", "param:filmcurv:01:colour:1"])),
+    # hilite between denoise and demosaic (the stock darkroom chain)
+    ("bayer", 290, 210, 46, dict(hilite=True)),
+    ("xtrans", 246, 180, 47, dict(hilite=True, extra=["param:hilite:01:soft:0.3", "param:hilite:01:desat:0.6"])),
+    ("bayer", 64, 48, 48, dict(hilite=True, levels=4)),
 ]
 
 

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import hip, pipe
 
-STAGES = 9
+STAGES = 10
 
 
 class DevelopImage(C.Structure):
@@ -25,6 +25,7 @@ class DevelopImage(C.Structure):
         ("noise_a", C.c_float), ("noise_b", C.c_float), ("dn_params", C.c_float * 8), ("dm_fixup", C.c_int32),
         ("co_params", C.c_float * 242), ("fc_params", C.c_float * 4), ("ll_params", C.c_float * 4),
         ("ll_numgamma", C.c_int32), ("ll_levels", C.c_int32),
+        ("hl_on", C.c_int32), ("hl_wb", C.c_float * 4), ("hl_params", C.c_float * 3), ("hl_levels", C.c_int32),
     ]
 
 

@@ -11,14 +11,17 @@
 //   group 1  wavelet   denoise/downcov, down x3       Q -> Q (four launches)
 //   group 2  dnasm     denoise/assemble               5x Q -> Q
 //   group 3  doub      denoise/doub                   CFA + 2x Q -> mosaic f16
-//   group 4  gauss     demosaic/gauss                 mosaic -> block covariance
-//   group 5  develop   demosaic/splat + demosaic/fix + colour + filmcurv,
+//   group 4  hilite    hilite/half, reduce 1..L, assemble L-1..0, doub
+//                      (images whose graph has hilite between denoise and
+//                      demosaic)                      mosaic -> mosaic f16
+//   group 5  gauss     demosaic/gauss                 mosaic -> block covariance
+//   group 6  develop   demosaic/splat + demosaic/fix + colour + filmcurv,
 //                      fused: the green of a tile and its halo lives in LDS,
 //                      fix/colour/filmcurv run in registers -> rgba f16
-//   group 6  pyr1      llap/curve + llap/reduce to level 1, fused: the
+//   group 7  pyr1      llap/curve + llap/reduce to level 1, fused: the
 //                      full-resolution curve planes exist only in LDS
-//   group 7  pyramid   llap/reduce levels 2.., llap/assemble levels nl-2..1
-//   group 8  final     llap/assemble level 0 + llap/colour, fused: level 0
+//   group 8  pyramid   llap/reduce levels 2.., llap/assemble levels nl-2..1
+//   group 9  final     llap/assemble level 0 + llap/colour, fused: level 0
 //                      planes are re-evaluated from the filmcurv output
 //
 // Wherever the per-node graph stores an intermediate as f16 and the next
@@ -30,8 +33,10 @@
 #include <algorithm>
 #include "launch.h"
 #include "ops.h"
+#include "ops_hilite.h"
 
 #define BT_LEVELS 12
+#define HL_LEVELS 15  // hilite: the block grid and at most 14 coarser levels (hilite/main.c)
 #define COV_R 2     // downcov: 64 x 8 texels per tile
 #define DOWN_R 4    // down: 64 x 16 texels per tile
 #define DEV_R 4     // develop: 64 x 16 texels per tile
@@ -41,7 +46,12 @@ struct bt_img
 {
   dimg cfa, out;
   dimg qh, qd[4], qa;      // denoise: half, downcov + down 1..3, assemble (Q rgba f16)
-  dimg mos, cov, film;     // doub output (f16), gauss output (rgba f16), filmcurv output (rgba f16)
+  dimg mos, cov, film;     // demosaic input (f16), gauss output (rgba f16), filmcurv output (rgba f16)
+  dimg dmos;               // denoise/doub output: mos itself unless hilite runs in between
+  dimg hq[HL_LEVELS];      // hilite: half output (level 0) and reduce outputs (levels 1..L)
+  dimg ha[HL_LEVELS];      // hilite: assemble outputs (levels 0..L-1); ha[L] aliases hq[L]
+  hl_args hl;
+  int  hl_nx, hl_ny;       // hilite/doub: blocks run (8-aligned node extent, bounded by the mosaic)
   dimg lp[BT_LEVELS];      // llap planes of level l >= 1, num_gamma + 1 planes
   dimg la[BT_LEVELS];      // llap reconstruction of level l, 1 <= l <= nl-2
   int  nl, xt, dm_r;
@@ -253,7 +263,7 @@ __global__ void __launch_bounds__(256) kb_doub(bt_launch L)
   bt_tiles(L, [&](const bt_job &jb) {
     const bt_img &im = L.img[jb.img];
     lds_copy(a, im.dn_doub);
-    cfa = rfl(im.cfa); crs0 = rfl(im.qa); crs1 = rfl(im.qh); o = rfl(im.mos);
+    cfa = rfl(im.cfa); crs0 = rfl(im.qa); crs1 = rfl(im.qh); o = rfl(im.dmos);
   }, [&](const bt_job &jb, int lt) {
     PIX_TILE(L, jb, lt, X0, Y0);
     const int x = X0 + threadIdx.x;
@@ -262,6 +272,90 @@ __global__ void __launch_bounds__(256) kb_doub(bt_launch L)
     {
       const int y = Y0 + threadIdx.y + BY * k;
       if(x < o.wd && y < o.ht) st_h1(o, x, y, dn_doub_px(cfa, crs0, crs1, o.wd, o.ht, a, x, y));
+    }
+  });
+}
+
+// ---- hilite -----------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) kb_hl_half(bt_launch L)
+{
+  __shared__ hl_args a;
+  dimg in, o;
+  int xt = 0;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.hl);
+    in = rfl(im.dmos); o = rfl(im.hq[0]); xt = rfl(im.xt);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, xt ? hl_half_at<1>(in, x, y, a.white) : hl_half_at<0>(in, x, y, a.white));
+    }
+  });
+}
+
+// reduce to level jb.level: tiles of BX x BY coarse texels
+__global__ void __launch_bounds__(256) kb_hl_reduce(bt_launch L)
+{
+  __shared__ hl_lds t;
+  __shared__ hl_args a;
+  dimg in, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.hl);
+    in = rfl(im.hq[jb.level - 1]); o = rfl(im.hq[jb.level]);
+  }, [&](const bt_job &jb, int lt) {
+    hl_reduce_tile(in, o, a, (lt % jb.tiles_x) * BX, (lt / jb.tiles_x) * BY, t);
+  });
+}
+
+// assemble of level jb.level from its fine level and the next coarser one
+__global__ void __launch_bounds__(256) kb_hl_asm(bt_launch L)
+{
+  __shared__ hl_args a;
+  dimg fine, crs, o;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.hl);
+    fine = rfl(im.hq[jb.level]); crs = rfl(im.ha[jb.level + 1]); o = rfl(im.ha[jb.level]);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, hl_assemble_at(fine, crs, a, x, y));
+    }
+  });
+}
+
+// back onto the mosaic: one thread per block
+__global__ void __launch_bounds__(256) kb_hl_doub(bt_launch L)
+{
+  __shared__ hl_args a;
+  dimg in, crs, o;
+  int xt = 0, nx = 0, ny = 0;
+  bt_tiles(L, [&](const bt_job &jb) {
+    const bt_img &im = L.img[jb.img];
+    lds_copy(a, im.hl);
+    in = rfl(im.dmos); crs = rfl(im.ha[0]); o = rfl(im.mos);
+    xt = rfl(im.xt); nx = rfl(im.hl_nx); ny = rfl(im.hl_ny);
+  }, [&](const bt_job &jb, int lt) {
+    PIX_TILE(L, jb, lt, X0, Y0);
+    const int x = X0 + threadIdx.x;
+#pragma unroll 1
+    for(int k = 0; k < L.rows; k++)
+    {
+      const int y = Y0 + threadIdx.y + BY * k;
+      if(x >= nx || y >= ny) continue;
+      if(xt) hl_doub_at<1>(in, crs, o, x, y, a.white);
+      else   hl_doub_at<0>(in, crs, o, x, y, a.white);
     }
   });
 }
@@ -517,11 +611,11 @@ __global__ void __launch_bounds__(256) kb_final(bt_launch L)
 
 namespace {
 
-enum kind_t { K_HALF, K_DOWNCOV, K_DOWN, K_DNASM, K_DOUB, K_GAUSS, K_DEVELOP, K_DEVELOP_XT, K_PYR1, K_REDUCE, K_ASM,
-              K_FINAL };
-const int kind_group[] = { 0, 1, 1, 2, 3, 4, 5, 5, 6, 7, 7, 8 };
-const char *group_name[VKDT_BATCH_STAGES] = { "half", "wavelet", "dnasm", "doub", "gauss", "develop", "pyr1",
-                                              "pyramid", "final" };
+enum kind_t { K_HALF, K_DOWNCOV, K_DOWN, K_DNASM, K_DOUB, K_HL_HALF, K_HL_REDUCE, K_HL_ASM, K_HL_DOUB, K_GAUSS,
+              K_DEVELOP, K_DEVELOP_XT, K_PYR1, K_REDUCE, K_ASM, K_FINAL };
+const int kind_group[] = { 0, 1, 1, 2, 3, 4, 4, 4, 4, 5, 6, 6, 7, 8, 8, 9 };
+const char *group_name[VKDT_BATCH_STAGES] = { "half", "wavelet", "dnasm", "doub", "hilite", "gauss", "develop",
+                                              "pyr1", "pyramid", "final" };
 
 struct launch_t
 {
@@ -539,6 +633,8 @@ struct dims_t
 {
   int W, H, qw, qh, cw, ch, nl, ng, block;
   int lw[BT_LEVELS], lh[BT_LEVELS];
+  int hl, hn;                          // hilite on, its coarse levels L
+  int hw[HL_LEVELS], hh[HL_LEVELS];    // hilite level extents (0: the block grid)
 };
 
 int image_dims(const vkdt_develop_image_t &d, dims_t &m, int idx)
@@ -564,6 +660,26 @@ int image_dims(const vkdt_develop_image_t &d, dims_t &m, int idx)
   m.ng = d.ll_numgamma;
   m.lw[0] = m.W; m.lh[0] = m.H;
   for(int l = 1; l < m.nl; l++) { m.lw[l] = (m.lw[l - 1] - 1) / 2 + 1; m.lh[l] = (m.lh[l - 1] - 1) / 2 + 1; }
+  m.hl = d.hl_on != 0;
+  m.hn = 0;
+  if(m.hl)
+  { // hilite/main.c create_nodes: a reduce per level until the next coarser
+    // extent reaches 1 or 14 levels exist
+    if(m.W % m.block || m.H % m.block)
+      return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: hilite over %dx%d, not whole %dx%d blocks", idx,
+                            m.W, m.H, m.block, m.block);
+    m.hw[0] = m.qw; m.hh[0] = m.qh;
+    for(int l = 1; l < HL_LEVELS; l++)
+    {
+      m.hw[l] = (m.hw[l - 1] - 1) / 2 + 1;
+      m.hh[l] = (m.hh[l - 1] - 1) / 2 + 1;
+      m.hn = l;
+      if((m.hw[l] - 1) / 2 + 1 <= 1 || (m.hh[l] - 1) / 2 + 1 <= 1) break;
+    }
+    if(m.hn != d.hl_levels)
+      return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: %d hilite levels, the block grid %dx%d has %d", idx,
+                            d.hl_levels, m.qw, m.qh, m.hn);
+  }
   return VKDT_HIP_OK;
 }
 
@@ -579,6 +695,12 @@ size_t image_scratch(const dims_t &m)
     s += align256((size_t)(m.ng + 1) * pitch8(m.lw[l]) * m.lh[l] * 2);
   for(int l = 1; l <= m.nl - 2; l++)
     s += align256((size_t)pitch8(m.lw[l]) * m.lh[l] * 2);
+  if(m.hl)
+  {
+    s += align256((size_t)pitch8(m.W) * m.H * 2);               // dmos
+    for(int l = 0; l <= m.hn; l++) s += align256((size_t)m.hw[l] * m.hh[l] * 8);
+    for(int l = 0; l < m.hn; l++) s += align256((size_t)m.hw[l] * m.hh[l] * 8);
+  }
   return s;
 }
 
@@ -593,7 +715,7 @@ dimg mk(void *p, int wd, int ht, int pitch, int chan, long pstride, int fmt = DI
 
 struct vkdt_batch_t
 {
-  int n = 0, ncu = 256, max_nl = 0, max_ng = 0;
+  int n = 0, ncu = 256, max_nl = 0, max_ng = 0, max_hn = 0;
   std::vector<dims_t> dm;
   std::vector<bt_img> img;         // host copies of the device tables
   std::vector<bt_job> job;
@@ -614,7 +736,7 @@ struct vkdt_batch_t
 
 // the device tables at the end of the scratch: image records, job lists
 // (at most one job per image and launch) and the tile counters
-#define BT_MAX_LAUNCH (10 + 2 * BT_LEVELS)
+#define BT_MAX_LAUNCH (10 + 2 * BT_LEVELS + 2 * HL_LEVELS)
 static size_t table_bytes(int n)
 {
   return align256(sizeof(bt_img) * n) + align256(sizeof(bt_job) * (size_t)BT_MAX_LAUNCH * n)
@@ -674,6 +796,26 @@ static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m
     b.la[l] = mk(sp, m.lw[l], m.lh[l], pitch8(m.lw[l]), 1, 0);
     sp += align256((size_t)pitch8(m.lw[l]) * m.lh[l] * 2);
   }
+  b.dmos = b.mos;
+  if(m.hl)
+  {
+    b.dmos = mk(sp, m.W, m.H, pitch8(m.W), 1, 0); sp += align256((size_t)pitch8(m.W) * m.H * 2);
+    for(int l = 0; l <= m.hn; l++)
+    {
+      b.hq[l] = mk(sp, m.hw[l], m.hh[l], m.hw[l], 4, 0);
+      sp += align256((size_t)m.hw[l] * m.hh[l] * 8);
+    }
+    for(int l = 0; l < m.hn; l++)
+    {
+      b.ha[l] = mk(sp, m.hw[l], m.hh[l], m.hw[l], 4, 0);
+      sp += align256((size_t)m.hw[l] * m.hh[l] * 8);
+    }
+    b.ha[m.hn] = b.hq[m.hn];
+    hl_args_init(b.hl, d.hl_wb, d.hl_params);
+    // hilite/doub: ceil(wd/8) x ceil(ht/8) groups of 8x8 blocks, bounded by the mosaic's extent
+    b.hl_nx = std::min((m.qw + 7) / 8 * 8, m.W);
+    b.hl_ny = std::min((m.qh + 7) / 8 * 8, m.H);
+  }
   b.nl = m.nl;
   b.xt = d.filters == 9u;
   b.dm_r = b.xt ? std::min(3, std::max(2, d.dm_fixup + 2)) : std::min(2, std::max(1, d.dm_fixup + 1));
@@ -699,7 +841,11 @@ static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m
   return VKDT_HIP_OK;
 }
 
-static int is_pixel_stage(int kind) { return kind == K_HALF || kind == K_DNASM || kind == K_DOUB || kind == K_GAUSS; }
+static int is_pixel_stage(int kind)
+{
+  return kind == K_HALF || kind == K_DNASM || kind == K_DOUB || kind == K_GAUSS || kind == K_HL_HALF ||
+         kind == K_HL_ASM || kind == K_HL_DOUB;
+}
 
 // launch shape of a stage group: tune = persist | rows << 8 (rows: per-pixel
 // stages' rows per thread, 1..8, default 4)
@@ -715,6 +861,10 @@ static int tiles_of(int kind, const dims_t &m, int level, int rows, int &tx)
     case K_DOWNCOV: w = m.qw; h = m.qh; th = BY * COV_R; break;
     case K_DOWN: w = m.qw; h = m.qh; th = BY * DOWN_R; break;
     case K_DOUB: w = m.W; h = m.H; th = BY * rows; break;
+    case K_HL_HALF: w = m.qw; h = m.qh; th = BY * rows; break;
+    case K_HL_REDUCE: w = m.hw[level]; h = m.hh[level]; break;
+    case K_HL_ASM: w = m.hw[level]; h = m.hh[level]; th = BY * rows; break;
+    case K_HL_DOUB: w = std::min((m.qw + 7) / 8 * 8, m.W); h = std::min((m.qh + 7) / 8 * 8, m.H); th = BY * rows; break;
     case K_GAUSS: w = m.cw; h = m.ch; th = BY * rows; break;
     case K_DEVELOP: case K_DEVELOP_XT: w = m.W; h = m.H; th = BY * DEV_R; break;
     case K_PYR1: w = m.lw[1]; h = m.lh[1]; break;
@@ -742,6 +892,9 @@ static void build_launches(vkdt_batch_t *b)
       if(kind == K_ASM && level > m.nl - 2) continue;
       if(kind == K_DEVELOP && m.block == 3) continue;      // Bayer and X-Trans develop
       if(kind == K_DEVELOP_XT && m.block != 3) continue;   // run as two launches
+      if(kind >= K_HL_HALF && kind <= K_HL_DOUB && !m.hl) continue;
+      if(kind == K_HL_REDUCE && level > m.hn) continue;
+      if(kind == K_HL_ASM && level >= m.hn) continue;
       int tx;
       const int t = tiles_of(kind, m, level, rows, tx);
       if(t <= 0) continue;
@@ -757,6 +910,10 @@ static void build_launches(vkdt_batch_t *b)
   for(int l = 1; l < 4; l++) add(K_DOWN, l, 0);
   add(K_DNASM, 0, 0);
   add(K_DOUB, 0, 0);
+  add(K_HL_HALF, 0, 0);
+  for(int l = 1; l <= b->max_hn; l++) add(K_HL_REDUCE, l, 0);
+  for(int l = b->max_hn - 1; l >= 0; l--) add(K_HL_ASM, l, 0);
+  add(K_HL_DOUB, 0, 0);
   add(K_GAUSS, 0, 0);
   add(K_DEVELOP, 0, 0);
   add(K_DEVELOP_XT, 0, 0);
@@ -774,7 +931,7 @@ int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, i
   *out = nullptr;
   std::vector<dims_t> dm(n);
   size_t need = 0;
-  int max_nl = 0, max_ng = 0;
+  int max_nl = 0, max_ng = 0, max_hn = 0;
   for(int i = 0; i < n; i++)
   {
     if(int e = image_dims(img[i], dm[i], i)) return e;
@@ -782,6 +939,7 @@ int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, i
     need += image_scratch(dm[i]);
     max_nl = std::max(max_nl, dm[i].nl);
     max_ng = std::max(max_ng, dm[i].ng);
+    max_hn = std::max(max_hn, dm[i].hn);
   }
   const size_t tables = need;
   need += table_bytes(n);
@@ -799,6 +957,7 @@ int vkdt_hip_batch_create(vkdt_batch_t **out, const vkdt_develop_image_t *img, i
   b->img = h;
   b->max_nl = max_nl;
   b->max_ng = max_ng;
+  b->max_hn = max_hn;
   for(int s = 0; s < VKDT_BATCH_STAGES; s++) b->tune[s] = tune ? tune[s] : (4 << 8);
   asm_mats_init(b->mats);
   int dev = 0;
@@ -852,6 +1011,10 @@ static int enqueue(vkdt_batch_t *b, hipStream_t s, int timed)
       case K_DOWN:    hipLaunchKernelGGL(kb_down, grid, blk, 0, s, a); break;
       case K_DNASM:   hipLaunchKernelGGL(kb_dnasm, grid, blk, 0, s, a, b->mats); break;
       case K_DOUB:    hipLaunchKernelGGL(kb_doub, grid, blk, 0, s, a); break;
+      case K_HL_HALF: hipLaunchKernelGGL(kb_hl_half, grid, blk, 0, s, a); break;
+      case K_HL_REDUCE: hipLaunchKernelGGL(kb_hl_reduce, grid, blk, 0, s, a); break;
+      case K_HL_ASM:  hipLaunchKernelGGL(kb_hl_asm, grid, blk, 0, s, a); break;
+      case K_HL_DOUB: hipLaunchKernelGGL(kb_hl_doub, grid, blk, 0, s, a); break;
       case K_GAUSS:   hipLaunchKernelGGL(kb_gauss, grid, blk, 0, s, a); break;
       case K_DEVELOP: hipLaunchKernelGGL(kb_develop<0>, grid, blk, 0, s, a); break;
       case K_DEVELOP_XT: hipLaunchKernelGGL(kb_develop<1>, grid, blk, 0, s, a); break;
@@ -954,7 +1117,7 @@ int vkdt_hip_batch_autotune(vkdt_batch_t *b, void *stream, int32_t *tune)
     for(int k = 0; k < VKDT_BATCH_STAGES; k++)
     {
       const int pix = k == kind_group[K_HALF] || k == kind_group[K_DNASM] || k == kind_group[K_DOUB] ||
-                      k == kind_group[K_GAUSS];
+                      k == kind_group[K_GAUSS] || k == kind_group[K_HL_HALF];
       cand[k] = pix ? (pers[(c % 3) * 2] | rows[c / 3] << 8) : (pers[c % 5] | 4 << 8);
       b->tune[k] = cand[k];
     }

@@ -19,6 +19,19 @@ struct hl_args
   float soft;
 };
 
+// the arguments of every hilite node from its push constants' white balance
+// and its params white, desat, soft (reduce.comp:23-28, assemble.comp)
+inline void hl_args_init(hl_args &a, const float *wb4, const float *par)
+{
+  float wb[3] = { wb4[0], wb4[1], wb4[2] };
+  if(!(wb[0] * wb[0] + wb[1] * wb[1] + wb[2] * wb[2] > 1e-3f)) wb[0] = wb[1] = wb[2] = 1.0f;
+  for(int k = 0; k < 3; k++) a.wb[k] = wb[k];
+  a.white = par[0];
+  a.white_pos = par[0] > 0.0f ? par[0] : 1.0f;
+  a.ds = par[1] * par[1];
+  a.soft = par[2];
+}
+
 // binomial taps 1 4 6 4 1 / 16 and the edge taps 1 2 0 -2 -1
 DEV float hl_w(int i) { return i == 2 ? 0.375f : ((i & 1) ? 0.25f : 0.0625f); }
 DEV float hl_sw(int i) { return i == 0 ? 1.0f : (i == 1 ? 2.0f : (i == 2 ? 0.0f : (i == 3 ? -2.0f : -1.0f))); }
@@ -233,5 +246,90 @@ DEV void hl_doub_xtrans(float c[9], float3 up, float white)
     c[4] = mixf(c[4], up.y * scale, t);
     c[6] = mixf(c[6], up.y * scale, t);
     c[8] = mixf(c[8], up.y * scale, t);
+  }
+}
+
+// ---- one texel / tile of each node (per-node kernels and the batch) -------
+
+template<int XT>
+DEV float4 hl_half_at(const dimg &in, int x, int y, float white)
+{
+  if(XT) return hl_half_px<1>([&](int px, int py) { return fetch_m(in, px, py); }, x, y, white);
+  return hl_half_px<0>([&](int px, int py) { return centre_m(in, px, py); }, x, y, white);
+}
+
+// reduce over BX x BY coarse texels from (X0,Y0): their fine footprint
+// (2BX+3) x (2BY+3) is loaded once (mirrored addresses) and each fine texel
+// prepared once in LDS -- the reference evaluates the desaturation of every
+// fine texel for each of the up to 9 coarse texels whose 5x5 window covers
+// it.  Every thread of the BX x BY work group calls this (it synchronises).
+#define HLR_W (2 * BX + 3)
+#define HLR_H (2 * BY + 3)
+struct hl_lds
+{
+  float r[HLR_H * HLR_W], g[HLR_H * HLR_W], b[HLR_H * HLR_W], l[HLR_H * HLR_W], v[HLR_H * HLR_W];
+};
+
+DEV void hl_reduce_tile(const dimg &in, const dimg &out, const hl_args &a, int X0, int Y0, hl_lds &s)
+{
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for(int i = tid; i < HLR_W * HLR_H; i += BX * BY)
+  {
+    const int lx = i % HLR_W, ly = i / HLR_W;
+    const hl_tap t = hl_prep(centre_h4(in, 2 * X0 - 2 + lx, 2 * Y0 - 2 + ly), a);
+    s.r[i] = t.r; s.g[i] = t.g; s.b[i] = t.b; s.l[i] = t.l; s.v[i] = t.valid;
+  }
+  __syncthreads();
+  const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
+  if(x >= out.wd || y >= out.ht) return;
+  const int base = 2 * threadIdx.y * HLR_W + 2 * threadIdx.x;
+  st_h4(out, x, y, hl_reduce_sum([&](int ii, int jj) {
+    const int k = base + jj * HLR_W + ii;
+    return hl_tap{ s.r[k], s.g[k], s.b[k], s.l[k], s.v[k] };
+  }));
+}
+
+DEV float4 hl_assemble_at(const dimg &fine, const dimg &coarse, const hl_args &a, int x, int y)
+{
+  const float3 up = hl_expand([&](int px, int py) { return centre_h4(coarse, px, py); }, x, y);
+  return hl_assemble_px(up, fetch_h4(fine, x, y), a);
+}
+
+// one mosaic site of the output: f16 or R16_UNORM store, dropped outside
+DEV void hl_store(const dimg &o, int x, int y, float v)
+{
+  if(x >= o.wd || y >= o.ht) return;
+  if(o.fmt == DIMG_UI16) st_u16(o, x, y, v);
+  else st_h1(o, x, y, v);
+}
+
+// doub of mosaic block (x,y).  doub.comp:24-25 bounds the block index by
+// the output's extent, not the node's: the threads of the last 8x8 work
+// groups past the node extent run too, reading zero coarse texels
+// (texelFetch) and writing the sites that fall inside the output
+template<int XT>
+DEV void hl_doub_at(const dimg &in, const dimg &coarse, const dimg &out, int x, int y, float white)
+{
+  const float4 u = fetch_h4(coarse, x, y);
+  if(XT)
+  {
+    const bool swap = ((x + y) & 1) == 0;
+    const float3 up = swap ? make_float3(u.z, u.y, u.x) : make_float3(u.x, u.y, u.z);
+    float c[9];
+    hl_block3([&](int px, int py) { return fetch_m(in, px, py); }, x, y, c);
+    hl_doub_xtrans(c, up, white);
+#pragma unroll
+    for(int i = 0; i < 3; i++)
+#pragma unroll
+      for(int j = 0; j < 3; j++) hl_store(out, 3 * x + i, 3 * y + j, c[3 * i + j]);
+  }
+  else
+  {
+    const float4 g = hl_doub_bayer(hl_gather2([&](int px, int py) { return centre_m(in, px, py); }, x, y),
+                                   make_float3(u.x, u.y, u.z), white);
+    hl_store(out, 2 * x, 2 * y, g.w);
+    hl_store(out, 2 * x + 1, 2 * y, g.z);
+    hl_store(out, 2 * x, 2 * y + 1, g.x);
+    hl_store(out, 2 * x + 1, 2 * y + 1, g.y);
   }
 }

@@ -1,7 +1,8 @@
 // develop.c -- one image of the batched develop path, read off a planned
 // graph.  The batch (vkdt_hip_batch_*) runs the raw-development chain
-// i-raw -> denoise -> demosaic -> colour -> filmcurv -> llap with the values
-// this graph's nodes would receive, so that its result equals the graph's.
+// i-raw -> denoise -> [hilite ->] demosaic -> colour -> filmcurv -> llap with
+// the values this graph's nodes would receive, so that its result equals
+// the graph's.
 #include "api.h"
 
 static const dt_node_t *find_node(const dt_graph_t *g, const char *name, const char *kernel, int *count)
@@ -57,32 +58,34 @@ int dt_graph_develop_image(dt_graph_t *g, vkdt_develop_image_t *d)
   if(!doub || !half) return fail(g, "denoise is not the mosaic wavelet path (strength > 0 on a raw)");
   if(!gauss || !fix) return fail(g, "demosaic is not the gaussian splat path");
   if(!col || !fc || !curve || !llc) return fail(g, "colour, filmcurv or llap missing");
-  // the chain must be exactly i-raw -> denoise -> demosaic -> colour -> filmcurv -> llap
-  const dt_module_t *chain[6] = { llc->module, fc->module, col->module, fix->module, doub->module, 0 };
-  for(int k = 0; k < 5; k++)
+  // the chain must be exactly i-raw -> denoise -> [hilite ->] demosaic ->
+  // colour -> filmcurv -> llap
+  static const char *chain_msg =
+      "the modules are not the chain i-raw > denoise > [hilite >] demosaic > colour > filmcurv > llap";
+  const dt_module_t *chain[4] = { llc->module, fc->module, col->module, fix->module };
+  for(int k = 0; k < 3; k++)
+    if(upstream(g, chain[k]) != chain[k + 1]) return fail(g, chain_msg);
+  const dt_module_t *hl = 0, *up = upstream(g, fix->module);
+  if(up && up->name == dt_token("hilite"))
   {
-    const dt_module_t *up = upstream(g, chain[k]);
-    if(!up || (k < 4 && up != chain[k + 1]))
-      return fail(g, "the modules are not the chain i-raw > denoise > demosaic > colour > filmcurv > llap");
-    if(k == 4)
-    {
-      if(up->name != dt_token("i-raw")) return fail(g, "denoise does not read i-raw");
-      chain[5] = up;
-    }
+    hl = up;
+    up = upstream(g, hl);
   }
+  if(up != doub->module) return fail(g, chain_msg);
+  const dt_module_t *raw = upstream(g, doub->module);
+  if(!raw || raw->name != dt_token("i-raw")) return fail(g, "denoise does not read i-raw");
   for(int n = 0; n < g->num_nodes; n++)
   {
     const dt_node_t *nd = g->node + n;
     if(nd->culled) continue;
     const dt_token_t mn = nd->name;
     if(mn != dt_token("denoise") && mn != dt_token("demosaic") && mn != dt_token("colour") &&
-       mn != dt_token("filmcurv") && mn != dt_token("llap") && nd->module != chain[5] &&
+       mn != dt_token("filmcurv") && mn != dt_token("llap") && !(hl && nd->module == hl) && nd->module != raw &&
        nd->module->connector[0].type != dt_token("sink"))
       return fail(g, "the graph runs nodes outside the raw-development chain");
   }
 
   const uint32_t *pc = doub->push_constant;
-  const dt_module_t *raw = chain[5];
   d->cfa_wd = raw->connector[0].roi.wd;
   d->cfa_ht = raw->connector[0].roi.ht;
   d->cfa_pitch = d->cfa_wd;
@@ -119,5 +122,27 @@ int dt_graph_develop_image(dt_graph_t *g, vkdt_develop_image_t *d)
   d->ll_levels = nreduce + 1;
   if(llc->wd != d->wd || llc->ht != d->ht || fix->wd != d->wd || fix->ht != d->ht)
     return fail(g, "demosaic or llap extent differs from the denoise output");
+  if(hl)
+  { // hilite/main.c: half carries wb[4] + filters, a reduce per coarse level
+    const dt_node_t *hh = 0;
+    int nhr = 0;
+    for(int n = 0; n < g->num_nodes; n++)
+    {
+      const dt_node_t *nd = g->node + n;
+      if(nd->module != hl) continue;
+      if(nd->kernel == dt_token("half")) hh = nd;
+      if(nd->kernel == dt_token("reduce")) nhr++;
+    }
+    if(!hh) return fail(g, "hilite passes its input through (not a mosaic)");
+    const uint32_t block = d->filters == 9u ? 3 : 2;
+    if(hh->wd != d->wd / block || hh->ht != d->ht / block || hh->push_constant[4] != d->filters)
+      return fail(g, "hilite extent or filters differ from the denoise output");
+    memcpy(d->hl_wb, hh->push_constant, 16);
+    p = node_params(hh, &sz);
+    if(sz < 12) return fail(g, "hilite params");
+    memcpy(d->hl_params, p, 12);
+    d->hl_levels = nhr;
+    d->hl_on = 1;
+  }
   return 0;
 }
