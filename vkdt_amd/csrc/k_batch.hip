@@ -557,10 +557,13 @@ __global__ void __launch_bounds__(256) kb_asm(bt_launch L)
     for(int k = 0; k < ASM_R; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      v[k] = (x < fine.wd && y < fine.ht) ? ld_h1(fine, x, y, ng) : 0.0f;
+      v[k] = ld_h1(fine, min(x, fine.wd - 1), min(y, fine.ht - 1), ng);   // clamped: no result outside
     }
+    const int xc = min(x, fine.wd - 1);
     ll_assemble_tile<ASM_R>(coarse, l1, fine.wd, fine.ht, a, first, X0, Y0, tile, lims, v,
-                            [&](int k, int p) { return ld_h1(fine, x, Y0 + threadIdx.y + BY * k, p); }, res);
+                            [&](int k, int p) {
+                              return ld_h1(fine, xc, min(Y0 + (int)threadIdx.y + BY * k, fine.ht - 1), p);
+                            }, res);
 #pragma unroll
     for(int k = 0; k < ASM_R; k++)
     {
@@ -592,7 +595,7 @@ __global__ void __launch_bounds__(256) kb_final(bt_launch L)
     for(int k = 0; k < ASM_R; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      t[k] = (x < f.wd && y < f.ht) ? ld_h4(f, x, y) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      t[k] = ld_h4(f, min(x, f.wd - 1), min(y, f.ht - 1));   // clamped: no result outside
       yl[k] = ll_lum(t[k]);
       v[k] = r16(yl[k]);
     }

@@ -73,10 +73,12 @@ __global__ void __launch_bounds__(256) k_ll_assemble(dimg coarse, dimg l0, dimg 
   for(int k = 0; k < AS_R; k++)
   {
     const int y = Y0 + threadIdx.y + 4 * k;
-    v[k] = (x < out.wd && y < out.ht) ? ld_h1(l0, x, y, a.num_gamma) : 0.0f;
+    v[k] = ld_h1(l0, min(x, out.wd - 1), min(y, out.ht - 1), a.num_gamma);   // clamped: no result outside
   }
+  const int xc = min(x, out.wd - 1);
   ll_assemble_tile<AS_R>(coarse, l1, out.wd, out.ht, a, a.first, X0, Y0, tile, lims, v,
-                         [&](int k, int p) { return ld_h1(l0, x, Y0 + threadIdx.y + 4 * k, p); }, res);
+                         [&](int k, int p) { return ld_h1(l0, xc, min(Y0 + (int)threadIdx.y + 4 * k, out.ht - 1), p); },
+                         res);
 #pragma unroll
   for(int k = 0; k < AS_R; k++)
   {

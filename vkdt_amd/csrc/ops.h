@@ -904,8 +904,11 @@ DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int 
 //   first        the coarsest level: expand the luminance plane of l1
 //                instead of a coarse reconstruction
 //   v[k]         the fine luminance plane at the thread's texel k
-//   plane(k, p)  the fine curve plane p there
-// Texels outside (owd, oht) get no result.
+//   plane(k, p)  the fine curve plane p there; called for every k, also for
+//                texels outside (owd, oht), so it must clamp its address
+// Texels outside (owd, oht) get no result.  Loads are issued unconditionally
+// (clamped addresses) so that the rows' loads and the staging's are in
+// flight together instead of one memory latency each.
 #define AS_TX 36                       // coarse tile width for a 64 wide fine tile (+-2 halo)
 #define AS_TY(R) (2 * (R) + 4)         // coarse tile height for a 4R high fine tile
 #define AS_TN(R) (AS_TX * AS_TY(R))
@@ -929,10 +932,11 @@ DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, 
   {
     const int y = Y0 + threadIdx.y + 4 * k;
     hi[k] = gamma_hi_from_v(v[k], a);
+    // issued before the staging so their latency overlaps it
+    pv0[k] = plane(k, hi[k] - 1);
+    pv1[k] = plane(k, hi[k]);
     if(x < owd && y < oht)
-    { // issued before the staging so their latency overlaps it
-      pv0[k] = plane(k, hi[k] - 1);
-      pv1[k] = plane(k, hi[k]);
+    {
       lo_t = min(lo_t, hi[k] - 1);
       hi_t = max(hi_t, hi[k]);
     }
@@ -945,16 +949,29 @@ DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, 
   const int p0 = lims[0], np = lims[1] - p0 + 1;
   const int cx0 = X0 / 2 - 2, cy0 = Y0 / 2 - 2;
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for(int idx = tid; idx < (np + 1) * CN; idx += BX * BY)
+  // slots 0..np-1: planes p0.. of l1; slot np: the coarse reconstruction
+  // (l1's plane ng on the first assemble).  Four loads per thread and pass.
+  const int n = (np + 1) * CN;
+  constexpr int S = BX * BY, U = 4;
+  for(int i0 = tid; i0 < n; i0 += U * S)
   {
-    const int sl = idx / CN, r = idx - sl * CN;
-    const int ty = r / AS_TX, tx = r - ty * AS_TX;
-    const int gx = mirror(cx0 + tx, l1.wd), gy = mirror(cy0 + ty, l1.ht);
-    float c;
-    if(sl < np) c = ld_h1(l1, gx, gy, p0 + sl);
-    else if(first > 0) c = ld_h1(l1, gx, gy, ng);
-    else c = ld_h1(coarse, gx, gy, 0);
-    tile[idx] = c;
+    float c[U];
+#pragma unroll
+    for(int u = 0; u < U; u++)
+    {
+      const int idx = min(i0 + u * S, n - 1);
+      const int sl = idx / CN, r = idx - sl * CN;
+      const int ty = r / AS_TX, tx = r - ty * AS_TX;
+      const int gx = mirror(cx0 + tx, l1.wd), gy = mirror(cy0 + ty, l1.ht);
+      const bool fc = sl >= np && first <= 0;
+      const __half *src = fc ? (const __half *)coarse.p : (const __half *)l1.p;
+      const long off = fc ? (long)gy * coarse.pitch + gx
+                          : (long)(sl < np ? p0 + sl : ng) * l1.pstride + (long)gy * l1.pitch + gx;
+      c[u] = h2f(src[off]);
+    }
+#pragma unroll
+    for(int u = 0; u < U; u++)
+      if(i0 + u * S < n) tile[i0 + u * S] = c[u];
   }
   __syncthreads();
   const float we[5] = { 0.5f, 0.5f, 1.0f, 0.5f, 0.5f }, wo[5] = { 0.0f, 1.0f, 0.5f, 0.5f, 1.0f };
