@@ -125,6 +125,17 @@ DEV float fetch_u16(const dimg &b, int x, int y)
   return ld_u16(b, x, y);
 }
 
+// mirror() without branches for coordinates within one extent of the image
+// (-n <= i < 2n, where it equals mirror()); beyond that it clamps into the
+// image, so that a load stays in bounds wherever its value is discarded.
+// Branch-free addressing lets a kernel keep several loads in flight.
+DEV int mirror_near(int i, int n)
+{
+  int r = i < 0 ? -i - 1 : i;
+  r = r >= n ? 2 * n - 1 - r : r;
+  return min(max(r, 0), n - 1);
+}
+
 // texture() at a texel centre
 DEV float4 centre_h4(const dimg &b, int x, int y) { return ld_h4(b, mirror(x, b.wd), mirror(y, b.ht)); }
 DEV float  centre_h1(const dimg &b, int x, int y, int pl = 0) { return ld_h1(b, mirror(x, b.wd), mirror(y, b.ht), pl); }

@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) kb_hl_half(bt_launch L)
     for(int k = 0; k < L.rows; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      if(x < o.wd && y < o.ht) st_h4(o, x, y, xt ? hl_half_at<1>(in, x, y, a.white) : hl_half_at<0>(in, x, y, a.white));
+      if(x < o.wd && y < o.ht) st_h4(o, x, y, xt ? hl_half_at<1, 0>(in, x, y, a.white) : hl_half_at<0, 0>(in, x, y, a.white));
     }
   });
 }
@@ -354,8 +354,8 @@ __global__ void __launch_bounds__(256) kb_hl_doub(bt_launch L)
     {
       const int y = Y0 + threadIdx.y + BY * k;
       if(x >= nx || y >= ny) continue;
-      if(xt) hl_doub_at<1>(in, crs, o, x, y, a.white);
-      else   hl_doub_at<0>(in, crs, o, x, y, a.white);
+      if(xt) hl_doub_at<1, 0>(in, crs, o, x, y, a.white);   // the denoised mosaic is f16
+      else   hl_doub_at<0, 0>(in, crs, o, x, y, a.white);
     }
   });
 }

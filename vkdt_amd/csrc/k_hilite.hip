@@ -8,12 +8,12 @@
 #include "launch.h"
 #include "ops_hilite.h"
 
-template<int XT>
+template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_hl_half(dimg in, dimg out, float white)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, hl_half_at<XT>(in, x, y, white));
+  st_h4(out, x, y, hl_half_at<XT, U16>(in, x, y, white));
 }
 
 // reduce: a work group covers BX x BY coarse texels (hl_reduce_tile)
@@ -31,12 +31,12 @@ __global__ void __launch_bounds__(256) k_hl_assemble(dimg fine, dimg coarse, dim
   st_h4(out, x, y, hl_assemble_at(fine, coarse, a, x, y));
 }
 
-template<int XT>
+template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_hl_doub(dimg in, dimg coarse, dimg out, int nx, int ny, float white)
 { // one thread per block of the mosaic (hl_doub_at)
   const int x = tid_x(), y = tid_y();
   if(x >= nx || y >= ny || x >= out.wd || y >= out.ht) return;
-  hl_doub_at<XT>(in, coarse, out, x, y, white);
+  hl_doub_at<XT, U16>(in, coarse, out, x, y, white);
 }
 
 // ---- host launchers --------------------------------------------------------
@@ -58,10 +58,9 @@ int launch_hilite_half(const node_call &c)
   if(int e = check_conn2(c, 0, VKDT_FMT_F16, VKDT_FMT_UI16, 1, 1, "hilite/half input")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "hilite/half output")) return e;
   const dimg in = to_dimg(c.conn[0]), out = to_dimg(c.conn[1]);
-  if(filters == 9)
-    hipLaunchKernelGGL(k_hl_half<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream, in, out, a.white);
-  else
-    hipLaunchKernelGGL(k_hl_half<0>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream, in, out, a.white);
+  const int u16 = in.fmt == DIMG_UI16;
+  auto k = filters == 9 ? (u16 ? k_hl_half<1, 1> : k_hl_half<1, 0>) : (u16 ? k_hl_half<0, 1> : k_hl_half<0, 0>);
+  hipLaunchKernelGGL(k, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream, in, out, a.white);
   return launch_post("hilite/half");
 }
 
@@ -101,9 +100,8 @@ int launch_hilite_doub(const node_call &c)
   // the reference dispatches ceil(wd/8) x ceil(ht/8) groups of 8x8
   const int nx = (int)((c.wd + 7) / 8 * 8), ny = (int)((c.ht + 7) / 8 * 8);
   const dimg in = to_dimg(c.conn[0]), crs = to_dimg(c.conn[1]), out = to_dimg(c.conn[2]);
-  if(filters == 9)
-    hipLaunchKernelGGL(k_hl_doub<1>, grid2d(nx, ny, 1), dim3(BX, BY), 0, c.stream, in, crs, out, nx, ny, a.white);
-  else
-    hipLaunchKernelGGL(k_hl_doub<0>, grid2d(nx, ny, 1), dim3(BX, BY), 0, c.stream, in, crs, out, nx, ny, a.white);
+  const int u16 = in.fmt == DIMG_UI16;
+  auto k = filters == 9 ? (u16 ? k_hl_doub<1, 1> : k_hl_doub<1, 0>) : (u16 ? k_hl_doub<0, 1> : k_hl_doub<0, 0>);
+  hipLaunchKernelGGL(k, grid2d(nx, ny, 1), dim3(BX, BY), 0, c.stream, in, crs, out, nx, ny, a.white);
   return launch_post("hilite/doub");
 }

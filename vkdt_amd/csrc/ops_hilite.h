@@ -250,12 +250,32 @@ DEV void hl_doub_xtrans(float c[9], float3 up, float white)
 }
 
 // ---- one texel / tile of each node (per-node kernels and the batch) -------
+// The mosaic format is a template parameter (U16: R16_UNORM, else f16) and
+// addresses are formed without branches (mirror_near, clamped loads with a
+// select for texelFetch's zero outside): every coordinate read here lies
+// within two texels of the image, where mirror_near equals mirror, and the
+// loads of a texel are in flight together.
 
-template<int XT>
+template<int U16>
+DEV float hl_mos_centre(const dimg &in, int px, int py)
+{
+  return ld_mt<U16>(in, mirror_near(px, in.wd), mirror_near(py, in.ht));
+}
+
+template<int U16>
+DEV float hl_mos_fetch(const dimg &in, int px, int py)
+{
+  const float v = ld_mt<U16>(in, min(max(px, 0), in.wd - 1), min(max(py, 0), in.ht - 1));
+  return (px >= 0 && py >= 0 && px < in.wd && py < in.ht) ? v : 0.0f;
+}
+
+DEV float4 hl_centre_h4(const dimg &b, int x, int y) { return ld_h4(b, mirror_near(x, b.wd), mirror_near(y, b.ht)); }
+
+template<int XT, int U16>
 DEV float4 hl_half_at(const dimg &in, int x, int y, float white)
 {
-  if(XT) return hl_half_px<1>([&](int px, int py) { return fetch_m(in, px, py); }, x, y, white);
-  return hl_half_px<0>([&](int px, int py) { return centre_m(in, px, py); }, x, y, white);
+  if(XT) return hl_half_px<1>([&](int px, int py) { return hl_mos_fetch<U16>(in, px, py); }, x, y, white);
+  return hl_half_px<0>([&](int px, int py) { return hl_mos_centre<U16>(in, px, py); }, x, y, white);
 }
 
 // reduce over BX x BY coarse texels from (X0,Y0): their fine footprint
@@ -273,11 +293,25 @@ struct hl_lds
 DEV void hl_reduce_tile(const dimg &in, const dimg &out, const hl_args &a, int X0, int Y0, hl_lds &s)
 {
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for(int i = tid; i < HLR_W * HLR_H; i += BX * BY)
+  constexpr int N = HLR_W * HLR_H, S = BX * BY, U = 4;   // four loads in flight per pass
+  for(int i0 = tid; i0 < N; i0 += U * S)
   {
-    const int lx = i % HLR_W, ly = i / HLR_W;
-    const hl_tap t = hl_prep(centre_h4(in, 2 * X0 - 2 + lx, 2 * Y0 - 2 + ly), a);
-    s.r[i] = t.r; s.g[i] = t.g; s.b[i] = t.b; s.l[i] = t.l; s.v[i] = t.valid;
+    float4 t[U];
+#pragma unroll
+    for(int u = 0; u < U; u++)
+    {
+      const int i = min(i0 + u * S, N - 1);
+      const int lx = i % HLR_W, ly = i / HLR_W;
+      t[u] = hl_centre_h4(in, 2 * X0 - 2 + lx, 2 * Y0 - 2 + ly);
+    }
+#pragma unroll
+    for(int u = 0; u < U; u++)
+    {
+      const int i = i0 + u * S;
+      if(i >= N) break;
+      const hl_tap p = hl_prep(t[u], a);
+      s.r[i] = p.r; s.g[i] = p.g; s.b[i] = p.b; s.l[i] = p.l; s.v[i] = p.valid;
+    }
   }
   __syncthreads();
   const int x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
@@ -289,10 +323,11 @@ DEV void hl_reduce_tile(const dimg &in, const dimg &out, const hl_args &a, int X
   }));
 }
 
+// (x,y) inside the fine level
 DEV float4 hl_assemble_at(const dimg &fine, const dimg &coarse, const hl_args &a, int x, int y)
 {
-  const float3 up = hl_expand([&](int px, int py) { return centre_h4(coarse, px, py); }, x, y);
-  return hl_assemble_px(up, fetch_h4(fine, x, y), a);
+  const float3 up = hl_expand([&](int px, int py) { return hl_centre_h4(coarse, px, py); }, x, y);
+  return hl_assemble_px(up, ld_h4(fine, x, y), a);
 }
 
 // one mosaic site of the output: f16 or R16_UNORM store, dropped outside
@@ -307,16 +342,17 @@ DEV void hl_store(const dimg &o, int x, int y, float v)
 // the output's extent, not the node's: the threads of the last 8x8 work
 // groups past the node extent run too, reading zero coarse texels
 // (texelFetch) and writing the sites that fall inside the output
-template<int XT>
+template<int XT, int U16>
 DEV void hl_doub_at(const dimg &in, const dimg &coarse, const dimg &out, int x, int y, float white)
 {
-  const float4 u = fetch_h4(coarse, x, y);
+  float4 u = ld_h4(coarse, min(x, coarse.wd - 1), min(y, coarse.ht - 1));
+  if(x >= coarse.wd || y >= coarse.ht) u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if(XT)
   {
     const bool swap = ((x + y) & 1) == 0;
     const float3 up = swap ? make_float3(u.z, u.y, u.x) : make_float3(u.x, u.y, u.z);
     float c[9];
-    hl_block3([&](int px, int py) { return fetch_m(in, px, py); }, x, y, c);
+    hl_block3([&](int px, int py) { return hl_mos_fetch<U16>(in, px, py); }, x, y, c);
     hl_doub_xtrans(c, up, white);
 #pragma unroll
     for(int i = 0; i < 3; i++)
@@ -325,7 +361,7 @@ DEV void hl_doub_at(const dimg &in, const dimg &coarse, const dimg &out, int x, 
   }
   else
   {
-    const float4 g = hl_doub_bayer(hl_gather2([&](int px, int py) { return centre_m(in, px, py); }, x, y),
+    const float4 g = hl_doub_bayer(hl_gather2([&](int px, int py) { return hl_mos_centre<U16>(in, px, py); }, x, y),
                                    make_float3(u.x, u.y, u.z), white);
     hl_store(out, 2 * x, 2 * y, g.w);
     hl_store(out, 2 * x + 1, 2 * y, g.z);
