@@ -133,14 +133,14 @@ DEV float4 hl_reduce_sum(Tap tap)
       const hl_tap t = tap(ii, jj);
       ex += (hl_w(jj) * hl_sw(ii)) * t.l;
       ey += (hl_w(ii) * hl_sw(jj)) * t.l;
-      if(t.valid != 0.0f)
-      {
-        const float u = hl_w(ii) * hl_w(jj);
-        cr += t.r * u;
-        cg += t.g * u;
-        cb += t.b * u;
-        wgt += u;
-      }
+      // only valid taps count: a clipped tap has colour +0 and valid 0, so
+      // adding its terms leaves every sum unchanged (they start at +0 and
+      // can never become -0), which spares the branch per tap
+      const float u = hl_w(ii) * hl_w(jj);
+      cr += t.r * u;
+      cg += t.g * u;
+      cb += t.b * u;
+      wgt += t.valid * u;
     }
   float4 o;
   if(wgt == 0.0f) o.x = o.y = o.z = 1.0f;
@@ -156,7 +156,7 @@ DEV float3 hl_expand(Tap tap, int ox, int oy)
 {
   const int ix = ox >> 1, iy = oy >> 1;
   const int dx = ox & 1, dy = oy & 1;
-  float cx = 0.0f, cy = 0.0f, cz = 0.0f, wgt = 0.0f;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
 #pragma unroll
   for(int ii = -1; ii <= 1; ii++)
   {
@@ -171,10 +171,12 @@ DEV float3 hl_expand(Tap tap, int ox, int oy)
       cx += t.x * wy * wx;
       cy += t.y * wy * wx;
       cz += t.z * wy * wx;
-      wgt += wy * wx;
     }
   }
-  return make_float3(cx / wgt, cy / wgt, cz / wgt);   // wgt >= 0.25
+  // the taps of either parity sum to 0.5 per axis, so the weight sum the
+  // reference divides by is exactly 0.25 (dyadic products and sums) and
+  // the division an exact scaling by 4
+  return make_float3(cx * 4.0f, cy * 4.0f, cz * 4.0f);
 }
 
 // colour-ratio scale of the reconstruction: the channels weighted by how
