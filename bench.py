@@ -386,9 +386,13 @@ def serve_workload(seconds, rate=600.0, p_still=0.2, seed=31):
             nxt[s] += 1
 
 
-def run_serve(args, tune):
+def run_serve(args, tune=None):
     """the serving loop on the mixed request stream, beside a FIFO of the
-    plain per-image path over the same arrivals (measured per-image times)"""
+    plain per-image path over the same arrivals (measured per-image times).
+    tune None: the built-in launch shape, one tile per work group, so that
+    the small batches in flight on the three streams overlap on the device
+    (persistent work groups, which the 64-still batch's autotuner may pick,
+    fill every CU for one large batch and serialise concurrent ones)"""
     import numpy as np
     import torch
     from vkdt_amd import batch, hip, pipe, serve, synth, synth_gpu
@@ -477,6 +481,7 @@ def run_serve(args, tune):
         "offered_mpix_per_s": round(px / args.serve_seconds / 1e6, 1),
         "served": s, "batches": len(sizes), "mean_batch": round(float(np.mean(sizes)), 2),
         "budget_gib": 200, "peak_gib": round(srv.arena.peak / 2**30, 2), "slots": 3,
+        "launch_shape": "built-in: one tile per work group" if tune is None else dict(zip(batch.stage_names(), tune)),
         "plain_fifo": {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                        "capacity_mpix_per_s": round(px / busy / 1e6, 1),
                        "method": "FIFO over the same arrivals with the per-node graph's measured time per shape"},
@@ -524,7 +529,7 @@ def main():
     else:
         line = run_batch(args)
         if args.serve_seconds > 0:
-            line["serving"] = run_serve(args, [line["config"]["tune"][k] for k in line["config"]["tune"]])
+            line["serving"] = run_serve(args)
     print(json.dumps(line), flush=True)
 
 
