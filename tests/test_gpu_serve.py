@@ -1,5 +1,5 @@
-"""The serving loop on the GPU: requests from two raw-video streams and two
-stills arrive as a Poisson process; the server batches whatever has arrived
+"""The serving loop on the GPU: requests from two raw-video streams and three
+stills (one of them with hilite between denoise and demosaic) arrive as a Poisson process; the server batches whatever has arrived
 whenever a slot frees, keeps several batches in flight, and admits within
 its memory budget.  Every request's result must equal -- bit for bit -- the
 same request served alone (a batch of one), which is itself the per-node
@@ -28,12 +28,12 @@ class Pinned:
         H.lib().vkdt_hip_host_free(self.ptr)
 
 
-def profile(tmp, name, kind, W, Hh, a):
+def profile(tmp, name, kind, W, Hh, a, hilite=False):
     cfa, _ = synth.mosaic(W, Hh, 3, kind)
     dng = str(tmp / f"{name}.dng")
     synth.write_dng(dng, cfa, kind)
     na, nb = synth.noise_params(a, 1e-6)
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb))
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb, hilite=hilite))
     g.export("o-pfm", str(tmp / name))
     g.plan()
     d = batch.describe(g)
@@ -47,7 +47,8 @@ def world(tmp_path_factory, hip):
     profs = {"s0": profile(tmp, "s0", "bayer", 320, 240, 2e-4),
              "s1": profile(tmp, "s1", "bayer", 320, 240, 6e-4),
              "x": profile(tmp, "x", "xtrans", 330, 246, 4e-4),
-             "b": profile(tmp, "b", "bayer", 400, 300, 3e-4)}
+             "b": profile(tmp, "b", "bayer", 400, 300, 3e-4),
+             "h": profile(tmp, "h", "bayer", 360, 270, 3e-4, hilite=True)}
     mos = {}
     for s, seed in (("s0", 11), ("s1", 12)):   # four frames per stream, drifting 2 px per frame
         sc = None
@@ -56,12 +57,13 @@ def world(tmp_path_factory, hip):
             mos[(s, f)] = Pinned(cfa)
     mos[("x", 0)] = Pinned(synth.mosaic(330, 246, 13, "xtrans")[0])
     mos[("b", 0)] = Pinned(synth.mosaic(400, 300, 14, "bayer")[0])
+    mos[("h", 0)] = Pinned(synth.mosaic(360, 270, 15, "bayer")[0])
     rng = np.random.default_rng(7)
     t, reqs = 0.0, []
     for i in range(48):
         t += rng.exponential(1.0 / 3000.0)      # Poisson arrivals, 3000 / s
         if rng.random() < 0.2:
-            key = ("x", 0) if rng.random() < 0.5 else ("b", 0)
+            key = (("x", "b", "h")[int(rng.integers(3))], 0)
         else:
             s = "s0" if rng.random() < 0.5 else "s1"
             key = (s, i % 4)

@@ -127,6 +127,11 @@ class _Slot:
 class Server:
     """Continuous batching over `slots` streams within `budget` bytes of HBM.
 
+    tune None keeps the batches' built-in launch shape, one tile per work
+    group, under which the small batches in flight on the slots' streams
+    overlap on the device; persistent work groups (a large batch's autotuned
+    shape) fill every CU per batch and serialise the streams.
+
     on_done(request) is called when a request's batch has completed, with
     request.out the device address of its rgba f32 result (valid during the
     call only)."""
