@@ -456,8 +456,10 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
   }
 }
 
+// at most 80 VGPRs, 6 waves per SIMD (X-Trans needed 96, 5 waves): the
+// staging and splat phases' memory latency is hidden by the extra waves
 template<int XT>
-__global__ void __launch_bounds__(256) kb_develop(bt_launch L)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) kb_develop(bt_launch L)
 {
   __shared__ float gt[(BX + 6) * (BY * DEV_R + 6)];                 // green, halo r <= 3
   __shared__ float mt[(BX + 2 * DEV_M) * (BY * DEV_R + 2 * DEV_M)];  // mosaic, halo r + rs
@@ -573,8 +575,10 @@ __global__ void __launch_bounds__(256) kb_asm(bt_launch L)
   });
 }
 
-// assemble level 0 + llap/colour: level 0 planes from the filmcurv output
-__global__ void __launch_bounds__(256) kb_final(bt_launch L)
+// assemble level 0 + llap/colour: level 0 planes from the filmcurv output.
+// At most 80 VGPRs (6 waves per SIMD instead of 5): its dependent phases
+// (rows, plane range, staging) need the extra waves to hide memory latency
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) kb_final(bt_launch L)
 {
   extern __shared__ float tile[];
   __shared__ int lims[2];
