@@ -48,6 +48,14 @@ typedef double real;
 #endif
 #define R(x) ((real)(x))
 
+// per-pixel nodes write only their own output texel, so their row loops
+// run on all host threads (OpenMP); the accumulating nodes stay serial
+#ifdef _OPENMP
+#define PAR_ROWS _Pragma("omp parallel for schedule(dynamic, 4)")
+#else
+#define PAR_ROWS
+#endif
+
 static int g_storage_f16 = 1;
 void or_set_storage_f16(int on) { g_storage_f16 = on; }
 int or_real_bytes(void) { return (int)sizeof(real) * 2; }
@@ -269,6 +277,7 @@ static int dn_noop(const int32_t *pci, const float *pcf, const void *params,
 {
   (void)params;
   const or_buf_t *in = c + 0, *out = c + 1;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 col = fetch(in, x + pci[0], y + pci[1], 0);
@@ -289,6 +298,7 @@ static int dn_half(const int32_t *pci, const float *pcf, const or_buf_t *c)
   const real white = R(pcf[9]);
   const int cx = pci[12], cy = pci[13];
   const uint32_t filters = (uint32_t)pci[16];
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 rgba;
@@ -402,6 +412,7 @@ static int dn_downcov(const int32_t *pci, const float *pcf, const or_buf_t *c, i
 {
   (void)pcf;
   const or_buf_t *in = c + 0, *out = c + 1, *cv = nconn > 2 ? c + 2 : 0;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real rgb[3], cov[4];
@@ -430,6 +441,7 @@ static int dn_down(const int32_t *pci, const float *pcf, const float *par, const
   const uint32_t blk = (uint32_t)pci[19];
   const real block = (blk == 3) ? R(2.23607) : (blk == 2 ? R(1.414213) : R(1.0));
   static const real off[4][2] = { {1.2, 0.4}, {-1.2, -0.4}, {0.4, -1.2}, {-0.4, 1.2} };
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 c0 = centre(in, x + cx, y + cy, 0);
@@ -500,6 +512,7 @@ static int dn_assemble(const int32_t *pci, const float *pcf, const float *par, c
   const real black[3] = { R(pcf[4]), R(pcf[5]), R(pcf[6]) };
   const real white[3] = { R(pcf[8]), R(pcf[9]), R(pcf[10]) };
   const int szx = s1->wd + 1, szy = s1->ht + 1;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 o0 = fetch(s0, x + pci[12], y + pci[13], 0);
@@ -564,6 +577,7 @@ static int dn_doub(const int32_t *pci, const float *pcf, const float *par, const
   const or_buf_t *in = c + 0, *crs0 = c + 1, *crs1b = c + 2, *out = c + 3;
   const uint32_t filters = (uint32_t)pci[16];
   const real strength = R(par[0]), luma = R(par[1]), detail = R(par[2]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const real sx0 = (R(x) + R(0.5)) / R(out->wd), sy0 = (R(y) + R(0.5)) / R(out->ht);
@@ -623,6 +637,7 @@ static int dm_down(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   const uint32_t filters = (uint32_t)pci[4];
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real lum = 0;
@@ -650,6 +665,7 @@ static int dm_gauss(const int32_t *pci, const or_buf_t *c)
   const uint32_t filters = (uint32_t)pci[4];
   const int xt = filters == 9;
   const int n0 = xt ? 0 : -1, n1 = xt ? 3 : 3, bs = xt ? 3 : 2;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real mw[2] = {0, 0}, mb[2] = {0, 0}, smw = 0, smb = 0;
@@ -725,6 +741,7 @@ static int dm_splat(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *ga = c + 1, *out = c + 2;
   const uint32_t filters = (uint32_t)pci[4];
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
@@ -776,6 +793,7 @@ static int dm_fix(const int32_t *pci, const int32_t *par, const or_buf_t *c)
   const or_buf_t *in = c + 0, *gr = c + 1, *cv = c + 2, *out = c + 3;
   const uint32_t filters = (uint32_t)pci[4];
   const int fixup = par ? par[0] : 0;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
@@ -868,6 +886,7 @@ static int co_main(const int32_t *pci, const float *par, const or_buf_t *c)
   mv3(cl_src, toc, one);
   mv3(cl_dst, toc, mul);
   const real expo = R(par[3]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 t = centre(in, x, y, 0);
@@ -912,6 +931,7 @@ static int fc_main(const float *par, const or_buf_t *c)
   const int mode = ((const int32_t *)par)[3];
   if(mode != 1 && mode != 3) return -2;
   const real il = FMAX(R(5e-3), R(par[0])), k = FMAX(R(1e-4), R(par[1])), bias = R(par[2]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 t = fetch(in, x, y, 0);
@@ -961,6 +981,7 @@ static int ll_curve_node(const int32_t *pci, const float *par, const or_buf_t *c
   const or_buf_t *in = c + 0, *out = c + 1;
   const int ng = pci[0];
   if(out->array_length < ng + 1) return -3;
+  PAR_ROWS
   for(int y = 0; y < in->ht; y++) for(int x = 0; x < in->wd; x++)
   {
     v4 t = fetch(in, x, y, 0);
@@ -983,6 +1004,7 @@ static int ll_reduce(const or_buf_t *c, int dp)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   for(int z = 0; z < dp && z < out->array_length; z++)
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const real sc = R(2 * x) + R(0.5), tc = R(2 * y) + R(0.5);
@@ -1010,6 +1032,7 @@ static int ll_assemble(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *coarse = c + 0, *l0 = c + 1, *l1 = c + 2, *out = c + 3;
   const int ng = pci[0], first = pci[1];
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const real res = first > 0 ? expand(l1, ng, x, y) : expand(coarse, 0, x, y);
@@ -1031,6 +1054,7 @@ static int ll_assemble(const int32_t *pci, const or_buf_t *c)
 static int ll_colour(const or_buf_t *c)
 {
   const or_buf_t *lum = c + 0, *org = c + 1, *out = c + 2;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real yv = fetch(lum, x, y, 0).x;
@@ -1065,6 +1089,7 @@ static int hl_half(const int32_t *pci, const float *par, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   const real white = R(par[0]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 rgba;
@@ -1107,6 +1132,7 @@ static int hl_reduce(const float *pcf, const float *par, const or_buf_t *c)
   real wb[3] = { R(pcf[0]), R(pcf[1]), R(pcf[2]) };
   if(!(wb[0] * wb[0] + wb[1] * wb[1] + wb[2] * wb[2] > R(1e-3))) wb[0] = wb[1] = wb[2] = R(1.0);
   const real ds = R(par[1]) * R(par[1]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real ex = 0, ey = 0, col[3] = { 0, 0, 0 }, wgt = 0;
@@ -1175,6 +1201,7 @@ static int hl_assemble(const float *par, const or_buf_t *c)
 {
   const or_buf_t *fine = c + 0, *coarse = c + 1, *out = c + 2;
   const real white = R(par[0]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real up[3];
@@ -1208,6 +1235,7 @@ static int hl_doub(const int32_t *pci, const float *par, const or_buf_t *c, int 
   // every invocation of the ceil(wd/8) x ceil(ht/8) groups of 8x8 runs
   // unless its block index lies outside the output's extent (doub.comp:24-25)
   const int nx = (wd + 7) / 8 * 8, ny = (ht + 7) / 8 * 8;
+  PAR_ROWS
   for(int y = 0; y < ny; y++) for(int x = 0; x < nx; x++)
   {
     if(x >= out->wd || y >= out->ht) continue;
@@ -1514,6 +1542,7 @@ static int co_colenc(const int32_t *par, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   const double *M = enc_matrix(par[0]);
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const v4 t = fetch(in, x, y, 0);
@@ -1539,6 +1568,7 @@ static int co_colenc(const int32_t *par, const or_buf_t *c)
 static int dm_halfsize(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     v4 o;
@@ -1566,6 +1596,7 @@ static int sh_resample(const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   const real W = (real)in->wd, H = (real)in->ht;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const real u = ((real)x + R(0.5)) / (real)out->wd, v = ((real)y + R(0.5)) / (real)out->ht;
@@ -1662,6 +1693,7 @@ static int rh_map(const float *par, const or_buf_t *c)
   const or_buf_t *hist = c + 0, *out = c + 1;
   const double *h = (const double *)hist->data;
   const int bins = hist->wd;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const real cnt = x < bins ? (real)(float)h[x] : 0, mean = x < bins ? (real)(float)h[bins + x] : 0;
@@ -1723,6 +1755,7 @@ static int hist_map(const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *out = c + 1;
   const double *h = (const double *)in->data;
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const uint32_t cnt = (uint32_t)(uint64_t)h[(size_t)y * in->wd + x];
@@ -1799,6 +1832,7 @@ static int ae_exp(const or_buf_t *c)
   const or_buf_t *in = c + 0, *out = c + 3;
   const double *lum = (const double *)c[2].data;
   const real e = FMIN(FMAX(R(0.18) / (real)lum[0], R(1e-3)), R(1e3));
+  PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     const v4 v = fetch(in, x, y, 0);

@@ -43,22 +43,27 @@ def lib(bits=64):
         path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
         if not os.path.exists(path):
             raise ImportError(f"oracle not built: {path}")
-        L = C.CDLL(path)
-        L.or_node.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                              C.POINTER(OrBuf), C.c_int, C.c_int, C.c_int, C.c_int]
-        L.or_develop.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(OrDevelop), C.c_void_p,
-                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_f16_round.argtypes = [C.c_double]
-        L.or_f16_round.restype = C.c_double
-        L.or_mirror.argtypes = [C.c_int, C.c_int]
-        L.or_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]
-        L.or_bilinear.restype = C.c_double
-        L.or_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
-        L.or_evd2x2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_set_storage_f16.argtypes = [C.c_int]
-        L.or_nprof_fit.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        _libs[bits] = L
+        _libs[bits] = load(path)
     return _libs[bits]
+
+
+def load(path):
+    """an oracle build at `path` with its entry points typed"""
+    L = C.CDLL(path)
+    L.or_node.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                          C.POINTER(OrBuf), C.c_int, C.c_int, C.c_int, C.c_int]
+    L.or_develop.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(OrDevelop), C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_f16_round.argtypes = [C.c_double]
+    L.or_f16_round.restype = C.c_double
+    L.or_mirror.argtypes = [C.c_int, C.c_int]
+    L.or_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.or_bilinear.restype = C.c_double
+    L.or_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+    L.or_evd2x2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_set_storage_f16.argtypes = [C.c_int]
+    L.or_nprof_fit.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    return L
 
 
 def as_double(a):

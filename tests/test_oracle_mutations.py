@@ -1,0 +1,96 @@
+"""The stencil pins must fail on plausible restatement mistakes.
+
+Each case plants one mistake in a copy of oracle/oracle.c (text
+substitution in a temporary directory; the oracle itself is untouched),
+builds it with the oracle's own flags, and runs every check of
+tests/test_oracle_pins_stencils.py against the build.  A mutant that passes
+all of them would be a mistake the pins cannot see, so the test requires at
+least one failure per mutant.  The first four are the denoise/demosaic
+mistakes that the closed forms of test_oracle_pins.py alone did not catch.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+
+import pytest
+
+import oracle_py as op
+import test_oracle_pins_stencils as pins
+
+ORACLE = os.path.join(op.ROOT, "oracle")
+
+# (id, original text, planted text, reference line the original restates)
+MUTANTS = [
+    ("flower_taps_mirrored", "{ {1.2, 0.4}, {-1.2, -0.4}, {0.4, -1.2}, {-0.4, 1.2} }",
+     "{ {1.2, -0.4}, {-1.2, 0.4}, {0.4, 1.2}, {-0.4, -1.2} }", "down.comp:96-99"),
+    ("band_scale_b0_0.5", "R(0.7000) / block", "R(0.5000) / block", "assemble.comp:90"),
+    ("level_decay_0.9", "POW(R(0.7), R(level))", "POW(R(0.9), R(level))", "down.comp:85"),
+    ("bayer_eval_scale_0.5", "eval[0] *= R(0.2); eval[1] *= R(0.2);", "eval[0] *= R(0.5); eval[1] *= R(0.5);",
+     "gauss.comp:68"),
+    ("xtrans_eval_scale_0.2", "eval[0] *= R(0.4); eval[1] *= R(0.4);", "eval[0] *= R(0.2); eval[1] *= R(0.2);",
+     "gauss.comp:120"),
+    ("downcov_minor_scale_0.1", "eval[1] *= R(0.05);", "eval[1] *= R(0.1);", "cov.glsl:118"),
+    ("hot_pixel_factor_3", "if(t.x > R(2.0) * mean_b) continue;", "if(t.x > R(3.0) * mean_b) continue;",
+     "cov.glsl:130"),
+    ("down_centre_weight_0.3", "const real t = R(0.2);", "const real t = R(0.3);", "down.comp:73"),
+    ("hard_threshold_5", "FABS(d[l][0]) > R(10.0)", "FABS(d[l][0]) > R(5.0)", "assemble.comp:140"),
+    ("walk_size_n", "const int szx = s1->wd + 1, szy = s1->ht + 1;", "const int szx = s1->wd, szy = s1->ht;",
+     "assemble.comp:55"),
+    ("splat_rotation_transposed",
+     "const real ofx = cov.z * R(i) + cov.w * R(j);\n  const real ofy = -cov.w * R(i) + cov.z * R(j);",
+     "const real ofx = cov.z * R(i) - cov.w * R(j);\n  const real ofy = cov.w * R(i) + cov.z * R(j);",
+     "splat.comp:33-35"),
+    ("splat_centre_weight_66", "w = R(666.0);", "w = R(66.0);", "splat.comp:38"),
+    ("fix_cov_scale_1", "fix_eval(R(2) * cov.x, R(2) * cov.y", "fix_eval(R(1) * cov.x, R(1) * cov.y", "fix.comp:74,86"),
+    ("doub_threshold_scale_1", "real T = R(0.5) * strength * upsm.w", "real T = R(1.0) * strength * upsm.w",
+     "doub.comp:44"),
+    ("doub_protection_pow_2", "POW(FMIN(R(1.0), R(1.0) * upsm.w), R(4.0))", "POW(FMIN(R(1.0), R(1.0) * upsm.w), R(2.0))",
+     "doub.comp:96"),
+    ("doub_sigma_red_green", "(val - crs1) / FMAX(sig[0] + sig[2], R(1e-8))", "(val - crs1) / FMAX(sig[0] + sig[1], R(1e-8))",
+     "doub.comp:89"),
+]
+
+
+def build_mutant(tmp, mid, old, new):
+    src = open(os.path.join(ORACLE, "oracle.c")).read()
+    assert old in src, f"{mid}: the oracle no longer contains {old!r}"
+    d = os.path.join(tmp, mid)
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "oracle.c"), "w") as f:
+        f.write(src.replace(old, new))
+    so = os.path.join(d, "liboracle64.so")
+    # the oracle's own flags (oracle/Makefile), double build; jpeg.c is not needed by the checks
+    r = subprocess.run(["gcc", "-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fopenmp", "-shared",
+                        "-I" + ORACLE, "-o", so, os.path.join(d, "oracle.c"), os.path.join(ORACLE, "jpeg.c"), "-lm"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return so
+
+
+@pytest.fixture(scope="module")
+def mutants(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("mutants"))
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        futs = {m[0]: ex.submit(build_mutant, tmp, m[0], m[1], m[2]) for m in MUTANTS}
+        return {k: f.result() for k, f in futs.items()}
+
+
+@pytest.mark.parametrize("mid", [m[0] for m in MUTANTS])
+def test_mutant_fails_a_pin(mutants, mid):
+    lib = op.load(mutants[mid])
+    failed = []
+    saved = op._libs.get(64)
+    op._libs[64] = lib
+    try:
+        for check in pins.CHECKS:
+            try:
+                check()
+            except AssertionError:
+                failed.append(check.__name__)
+    finally:
+        if saved is not None:
+            op._libs[64] = saved
+        else:
+            op._libs.pop(64, None)
+    print(f"[mutant] {mid}: fails {failed}")
+    assert failed, f"mutant {mid} passes every stencil pin"
