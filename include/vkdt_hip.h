@@ -137,6 +137,19 @@ int vkdt_hip_event_elapsed_ms(float *ms, void *ev_begin, void *ev_end);
 // serving loop polls it to retire batches while it admits new requests.
 int vkdt_hip_event_query(void *event);
 
+// Stream capture: the launches recorded on `stream` between begin and end
+// become one replayable graph (hipGraph), launched again with
+// vkdt_hip_graph_launch.  Takes over the reuse of a recorded Vulkan command
+// buffer when nothing changed between runs (graph.c:719-800, the
+// s_graph_run_record_cmd_buf flag): the host runtime records a run's node
+// launches once and replays them while the buffers and the push-constant
+// and uniform blobs stay the same.  Capture records kernels only (buffer
+// clears are fill kernels).
+int vkdt_hip_capture_begin(void *stream);
+int vkdt_hip_capture_end(void *stream, void **graph_exec);
+int vkdt_hip_graph_launch(void *graph_exec, void *stream);
+int vkdt_hip_graph_destroy(void *graph_exec);
+
 // Human readable text of the last error on this thread.
 const char *vkdt_hip_last_error(void);
 

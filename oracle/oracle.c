@@ -23,7 +23,42 @@
 #include <stdlib.h>
 #include <string.h>
 
-#ifdef ORACLE_F32
+#if defined(ORACLE_F32) && defined(ORACLE_PERTURB)
+// a second float implementation for the error-bound tests: every
+// transcendental and square root is moved by one ulp up or down (or kept),
+// by a hash of its argument, as another math library or the GPU's
+// hardware approximations round differently in the last place
+typedef float real;
+static uint32_t g_pert_seed = 0;
+void or_set_perturb_seed(uint32_t seed) { g_pert_seed = seed; }
+static inline float pert_ulp(float r, float a)
+{
+  uint32_t h;
+  memcpy(&h, &a, 4);
+  h ^= g_pert_seed * 0x9e3779b9u;
+  h *= 2654435761u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  const uint32_t s = h % 3u;
+  return s == 0 ? r : nextafterf(r, s == 1 ? INFINITY : -INFINITY);
+}
+static inline float p_exp(float x) { return pert_ulp(expf(x), x); }
+static inline float p_exp2(float x) { return pert_ulp(exp2f(x), x); }
+static inline float p_log2(float x) { return pert_ulp(log2f(x), x); }
+static inline float p_log(float x) { return pert_ulp(logf(x), x); }
+static inline float p_pow(float x, float y) { return pert_ulp(powf(x, y), x * 3.0f + y); }
+static inline float p_sqrt(float x) { return pert_ulp(sqrtf(x), x); }
+#define EXP   p_exp
+#define EXP2  p_exp2
+#define LOG2  p_log2
+#define LOG   p_log
+#define POW   p_pow
+#define SQRT  p_sqrt
+#define FABS  fabsf
+#define FMIN  fminf
+#define FMAX  fmaxf
+#elif defined(ORACLE_F32)
 typedef float real;
 #define EXP   expf
 #define EXP2  exp2f
@@ -58,6 +93,30 @@ typedef double real;
 
 static int g_storage_f16 = 1;
 void or_set_storage_f16(int on) { g_storage_f16 = on; }
+
+// near-tie flags.  A node with a data-dependent branch (a discontinuity of
+// its output in its inputs) marks each output texel whose decision margin
+// |lhs - rhs| is at most g_tie_eps times the magnitude of the decision's
+// operands: a last-bit difference in the arithmetic, or in the inputs,
+// can take the other branch there.  Off unless a mask is set.
+static uint8_t *g_tie = 0;
+static int g_tie_wd = 0, g_tie_ht = 0;
+static double g_tie_eps = 0.0, g_tie_effect = 0.0;
+void or_set_ties(uint8_t *mask, int wd, int ht, double eps)
+{
+  g_tie = mask; g_tie_wd = wd; g_tie_ht = ht; g_tie_eps = eps;
+}
+// a near tie counts where its other branch moves the node's result by more
+// than `effect` (absolute), or by more than one f16 ulp with effect 0
+void or_set_tie_effect(double effect) { g_tie_effect = effect; }
+static inline void tie_mark(int x, int y)
+{ // concurrent rows only ever store 1: benign
+  if(g_tie && x >= 0 && y >= 0 && x < g_tie_wd && y < g_tie_ht) g_tie[(size_t)y * g_tie_wd + x] = 1;
+}
+static inline int near_tie(real a, real b, real scale)
+{
+  return g_tie && FABS(a - b) <= R(g_tie_eps) * scale;
+}
 int or_real_bytes(void) { return (int)sizeof(real) * 2; }
 
 /* ------------------------------------------------------------------------
@@ -338,8 +397,62 @@ static int dn_half(const int32_t *pci, const float *pcf, const or_buf_t *c)
   return 0;
 }
 
-// cov.glsl:21-138 response(); p is the unsorted texel position
-static void dn_response(const or_buf_t *img, int px_, int py_, real rgb[3], real cov[4])
+// one binary16 ulp at the magnitude of x (2^-24 below the normal range)
+static inline double f16_ulp(double x)
+{
+  x = fabs(x);
+  if(x < 6.103515625e-05) return 5.9604644775390625e-08;
+  int e;
+  frexp(x, &e);
+  return ldexp(1.0, e - 11);
+}
+
+// whether a branch taken the other way changes a result by more than the
+// effect threshold (or one f16 ulp): only such near ties count
+static inline int f16_differs(const real a[3], const real b[3])
+{
+  for(int k = 0; k < 3; k++)
+  {
+    const double d = fabs((double)a[k] - (double)b[k]);
+    if(g_tie_effect > 0.0 ? d > g_tie_effect : d > f16_ulp(fmax(fabs((double)a[k]), fabs((double)b[k])))) return 1;
+  }
+  return 0;
+}
+
+// cov.glsl:109-138: the anisotropic gaussian of tensor S around (px_, py_),
+// hot pixels (red above twice the mean luminance) left out; the decision of
+// tap `flip` (0..24, -1 none) is inverted
+static void dn_aniso(const or_buf_t *img, int px_, int py_, const real S[4], real mean_b, int flip,
+                     real rgb[3], real cov[4])
+{
+  real eval[2], e0[2], e1[2];
+  evd2x2(eval, e0, e1, S);
+  eval[0] *= R(1.0); eval[1] *= R(0.05);
+  eval[0] = clampr(eval[0], R(0.01), R(25.0));
+  eval[1] = clampr(eval[1], R(0.01), R(25.0));
+  cov[0] = eval[0]; cov[1] = eval[1]; cov[2] = e0[0]; cov[3] = e0[1];
+
+  real res[3] = {0, 0, 0}, wt = 0;
+  for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+  {
+    v4 t = centre(img, px_ + i, py_ + j, 0);
+    int hot = t.x > R(2.0) * mean_b;   // hot pixel
+    if((j + 2) * 5 + i + 2 == flip) hot = !hot;
+    if(hot) continue;
+    const real x0 = R(i) * e0[0] + R(j) * e0[1];
+    const real x1 = R(i) * e1[0] + R(j) * e1[1];
+    const real w = FMAX(R(1e-9), EXP(R(-0.5) * ((x0 / eval[0]) * x0 + (x1 / eval[1]) * x1)));
+    res[0] += w * t.x; res[1] += w * t.y; res[2] += w * t.z;
+    wt += w;
+  }
+  const real d = FMAX(wt, R(1e-8));
+  rgb[0] = res[0] / d; rgb[1] = res[1] / d; rgb[2] = res[2] / d;
+}
+
+// cov.glsl:21-138 response(); p is the unsorted texel position.  *tie is
+// set where the determinant choice or a hot-pixel test is a near tie whose
+// other branch changes the result beyond one f16 ulp
+static void dn_response(const or_buf_t *img, int px_, int py_, real rgb[3], real cov[4], int *tie)
 {
   real mw[2] = {0, 0}, mb[2] = {0, 0}, smw = 0, smb = 0, mean_b = 0;
   real lum[25];
@@ -378,27 +491,23 @@ static void dn_response(const or_buf_t *img, int px_, int py_, real rgb[3], real
     sb += p2;
   }
   for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
-  const real *S = det2(Sw) < det2(Sb) ? Sw : Sb;
-  real eval[2], e0[2], e1[2];
-  evd2x2(eval, e0, e1, S);
-  eval[0] *= R(1.0); eval[1] *= R(0.05);
-  eval[0] = clampr(eval[0], R(0.01), R(25.0));
-  eval[1] = clampr(eval[1], R(0.01), R(25.0));
-  cov[0] = eval[0]; cov[1] = eval[1]; cov[2] = e0[0]; cov[3] = e0[1];
-
-  real res[3] = {0, 0, 0}, wt = 0;
+  const int usew = det2(Sw) < det2(Sb);
+  dn_aniso(img, px_, py_, usew ? Sw : Sb, mean_b, -1, rgb, cov);
+  if(!g_tie) return;
+  real alt[3], acov[4];
+  // the determinants cancel the products of their entries
+  if(near_tie(det2(Sw), det2(Sb), FABS(Sw[0] * Sw[3]) + FABS(Sw[1] * Sw[2]) + FABS(Sb[0] * Sb[3]) + FABS(Sb[1] * Sb[2])))
+  {
+    dn_aniso(img, px_, py_, usew ? Sb : Sw, mean_b, -1, alt, acov);
+    if(f16_differs(rgb, alt)) *tie = 1;
+  }
   for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
   {
-    v4 t = centre(img, px_ + i, py_ + j, 0);
-    if(t.x > R(2.0) * mean_b) continue;   // hot pixel
-    const real x0 = R(i) * e0[0] + R(j) * e0[1];
-    const real x1 = R(i) * e1[0] + R(j) * e1[1];
-    const real w = FMAX(R(1e-9), EXP(R(-0.5) * ((x0 / eval[0]) * x0 + (x1 / eval[1]) * x1)));
-    res[0] += w * t.x; res[1] += w * t.y; res[2] += w * t.z;
-    wt += w;
+    const v4 t = centre(img, px_ + i, py_ + j, 0);
+    if(!near_tie(t.x, R(2.0) * mean_b, R(2.0) * mean_b)) continue;
+    dn_aniso(img, px_, py_, usew ? Sw : Sb, mean_b, (j + 2) * 5 + i + 2, alt, acov);
+    if(f16_differs(rgb, alt)) *tie = 1;
   }
-  const real d = FMAX(wt, R(1e-8));
-  rgb[0] = res[0] / d; rgb[1] = res[1] / d; rgb[2] = res[2] / d;
 }
 
 static inline void sorted_pos(int x, int y, int wd, int ht, int *sx, int *sy)
@@ -416,7 +525,8 @@ static int dn_downcov(const int32_t *pci, const float *pcf, const or_buf_t *c, i
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
     real rgb[3], cov[4];
-    dn_response(in, x + pci[12], y + pci[13], rgb, cov);
+    int tie = 0;
+    dn_response(in, x + pci[12], y + pci[13], rgb, cov, &tie);
     if(cv && cv->data) { v4 cc = { cov[0], cov[1], cov[2], cov[3] }; store(cv, x + pci[12], y + pci[13], 0, cc); }
     int sx, sy;
     sorted_pos(x, y, out->wd, out->ht, &sx, &sy);
@@ -425,6 +535,7 @@ static int dn_downcov(const int32_t *pci, const float *pcf, const or_buf_t *c, i
     edge = clampr(R(0.02) + edge, R(0), R(1));
     v4 o = { rgb[0], rgb[1], rgb[2], edge };
     store(out, sx, sy, 0, o);
+    if(tie) tie_mark(sx, sy);
   }
   return 0;
 }
@@ -545,6 +656,18 @@ static int dn_assemble(const int32_t *pci, const float *pcf, const float *par, c
     real d4v[3] = { down[4][0], down[4][1], down[4][2] };
     for(int l = 3; l >= 0; l--)
     {
+      // the band carries the rounding of its two scales over sigma b_l; the
+      // other branch swaps the threshold between strength^4 and 10000
+      if(near_tie(FABS(d[l][0]), R(10.0), (FABS(down[l][0]) + FABS(down[l + 1][0])) / (sig[0] * bl[l])))
+      {
+        real c2[2][3];
+        for(int q = 0; q < 2; q++) for(int k = 0; k < 3; k++)
+        {
+          const real th = q ? R(10000.0) : thrs0, ad = FABS(d[l][k]);
+          c2[q][k] = d4v[k] + sig[k] * bl[l] * signr(d[l][k]) * mixr(FMAX(ad - th, R(0.0)), ad, FMIN(R(1.0), ad / (R(2) * th)));
+        }
+        if(f16_differs(c2[0], c2[1])) tie_mark(x, y);
+      }
       const real thrs = FABS(d[l][0]) > R(10.0) ? R(10000.0) : thrs0;
       for(int k = 0; k < 3; k++)
       {
@@ -658,7 +781,46 @@ static int dm_down(const int32_t *pci, const or_buf_t *c)
   return 0;
 }
 
-// gauss.comp:19-126
+static real splat_px(const or_buf_t *in, const v4 cov, int x, int y, int xt);
+
+// gauss.comp:65-77 (Bayer) and 114-121 (X-Trans): the stored eigenvalues
+// and axis of tensor S.  *how gets the snap taken (0 h, 1 v, 2 none);
+// force >= 0 takes that one instead
+static v4 dm_cov_of(const real S[4], int xt, int force, int *how)
+{
+  real eval[2], e0[2], e1[2];
+  evd2x2(eval, e0, e1, S);
+  int h;
+  if(!xt) h = FABS(e0[0]) > FABS(e0[1]) ? 0 : 1;
+  else h = FABS(e0[0]) > R(2.) * FABS(e0[1]) ? 0 : (FABS(e0[1]) > R(2.) * FABS(e0[0]) ? 1 : 2);
+  if(force >= 0) h = force;
+  if(how) *how = h;
+  const real sc = xt ? R(0.4) : R(0.2);
+  eval[0] *= sc; eval[1] *= sc;
+  if(h == 0) { e0[0] = 1; e0[1] = 0; }
+  else if(h == 1) { e0[0] = 0; e0[1] = 1; }
+  else { eval[0] = R(.1); eval[1] = R(.1); }
+  v4 o = { eval[0], eval[1], e0[0], e0[1] };
+  return o;
+}
+
+// whether two stored covariances of block (bx, by) change the green that
+// splat interpolates (splat.comp:98-140) at a pixel that reads the block
+// by more than one f16 ulp
+static int dm_cov_differs(const or_buf_t *mos, int xt, int bx, int by, v4 a, v4 b)
+{
+  for(int j = xt ? 0 : -1; j < (xt ? 3 : 1); j++) for(int i = xt ? 0 : -1; i < (xt ? 3 : 1); i++)
+  {
+    const int x = (xt ? 3 : 2) * bx + i, y = (xt ? 3 : 2) * by + j;
+    if(x < 0 || y < 0 || x >= mos->wd || y >= mos->ht) continue;
+    const real ga[3] = { splat_px(mos, a, x, y, xt), 0, 0 }, gb[3] = { splat_px(mos, b, x, y, xt), 0, 0 };
+    if(f16_differs(ga, gb)) return 1;
+  }
+  return 0;
+}
+
+// gauss.comp:19-126.  A near tie of the determinant choice or of an axis
+// snap counts where its other branch changes splat's green.
 static int dm_gauss(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *orig = c + 1, *out = c + 2;
@@ -695,24 +857,31 @@ static int dm_gauss(const int32_t *pci, const or_buf_t *c)
       sb += p2;
     }
     for(int k = 0; k < 4; k++) { Sw[k] /= sw; Sb[k] /= sb; }
-    const real *S = det2(Sw) < det2(Sb) ? Sw : Sb;
+    const int usew = det2(Sw) < det2(Sb);
+    const real *S = usew ? Sw : Sb;
+    int how;
+    const v4 o = dm_cov_of(S, xt, -1, &how);
+    store(out, x, y, 0, o);
+    if(!g_tie) continue;
+    int tie = 0;
+    if(near_tie(det2(Sw), det2(Sb),
+                FABS(Sw[0] * Sw[3]) + FABS(Sw[1] * Sw[2]) + FABS(Sb[0] * Sb[3]) + FABS(Sb[1] * Sb[2])))
+      tie |= dm_cov_differs(orig, xt, x, y, o, dm_cov_of(usew ? Sb : Sw, xt, -1, 0));
+    // a perturbation dS of the tensor turns its axes by about |dS| / (eval0 - eval1)
     real eval[2], e0[2], e1[2];
     evd2x2(eval, e0, e1, S);
+    const real ascale = (FABS(S[0]) + FABS(S[2]) + FABS(S[3])) / FMAX(eval[0] - eval[1], R(1e-30));
+    const real ax = FABS(e0[0]), ay = FABS(e0[1]);
     if(!xt)
     {
-      eval[0] *= R(0.2); eval[1] *= R(0.2);
-      if(FABS(e0[0]) > FABS(e0[1])) { e0[0] = 1; e0[1] = 0; }
-      else                          { e0[0] = 0; e0[1] = 1; }
+      if(near_tie(ax, ay, ascale)) tie |= dm_cov_differs(orig, xt, x, y, o, dm_cov_of(S, xt, 1 - how, 0));
     }
     else
     {
-      eval[0] *= R(0.4); eval[1] *= R(0.4);
-      if     (FABS(e0[0]) > R(2.) * FABS(e0[1])) { e0[0] = 1; e0[1] = 0; }
-      else if(FABS(e0[1]) > R(2.) * FABS(e0[0])) { e0[0] = 0; e0[1] = 1; }
-      else { eval[0] = R(.1); eval[1] = R(.1); }
+      if(near_tie(ax, R(2.) * ay, R(3.) * ascale)) tie |= dm_cov_differs(orig, xt, x, y, o, dm_cov_of(S, xt, how == 0 ? 2 : 0, 0));
+      if(near_tie(ay, R(2.) * ax, R(3.) * ascale)) tie |= dm_cov_differs(orig, xt, x, y, o, dm_cov_of(S, xt, how == 1 ? 2 : 1, 0));
     }
-    v4 o = { eval[0], eval[1], e0[0], e0[1] };
-    store(out, x, y, 0, o);
+    if(tie) tie_mark(x, y);
   }
   return 0;
 }
@@ -736,43 +905,47 @@ static inline void splat_eval(const or_buf_t *in, const v4 cov, int ox, int oy, 
   *weight = w;
 }
 
-// splat.comp:98-140
+// splat.comp:98-140: the green of pixel (x, y) under covariance cov
+static real splat_px(const or_buf_t *in, const v4 cov, int x, int y, int xt)
+{
+  real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+  if(xt)
+  {
+    for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
+    {
+      real col, wt;
+      splat_eval(in, cov, x, y, i, j, &col, &wt);
+      const int cl = xtrans_colour(x + i + 6, y + j + 6);
+      const int k = cl == 1 ? 1 : (cl == 2 ? 2 : 0);
+      rgb[k] += col * wt;
+      w[k] += wt;
+    }
+  }
+  else
+  {
+    for(int j = -1; j <= 1; j++) for(int i = -1; i <= 1; i++)
+    {
+      real col, wt;
+      splat_eval(in, cov, x, y, i, j, &col, &wt);
+      const int px = x + i, py = y + j;
+      col *= wt;
+      if(((px & 1) == 0 && (py & 1) == 1) || ((px & 1) == 1 && (py & 1) == 0)) { rgb[1] += col; w[1] += wt; }
+      else if((px & 1) == 0 && (py & 1) == 0) { rgb[0] += col; w[0] += wt; }
+      else { rgb[2] += col; w[2] += wt; }
+    }
+  }
+  return rgb[1] / FMAX(R(1e-8), w[1]);
+}
+
 static int dm_splat(const int32_t *pci, const or_buf_t *c)
 {
   const or_buf_t *in = c + 0, *ga = c + 1, *out = c + 2;
-  const uint32_t filters = (uint32_t)pci[4];
+  const int xt = (uint32_t)pci[4] == 9;
   PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
-    real rgb[3] = {0, 0, 0}, w[3] = {0, 0, 0};
-    if(filters == 9)
-    {
-      const v4 cov = fetch(ga, x / 3, y / 3, 0);
-      for(int j = -2; j <= 2; j++) for(int i = -2; i <= 2; i++)
-      {
-        real col, wt;
-        splat_eval(in, cov, x, y, i, j, &col, &wt);
-        const int cl = xtrans_colour(x + i + 6, y + j + 6);
-        const int k = cl == 1 ? 1 : (cl == 2 ? 2 : 0);
-        rgb[k] += col * wt;
-        w[k] += wt;
-      }
-    }
-    else
-    {
-      const v4 cov = fetch(ga, (x + 1) / 2, (y + 1) / 2, 0);
-      for(int j = -1; j <= 1; j++) for(int i = -1; i <= 1; i++)
-      {
-        real col, wt;
-        splat_eval(in, cov, x, y, i, j, &col, &wt);
-        const int px = x + i, py = y + j;
-        col *= wt;
-        if(((px & 1) == 0 && (py & 1) == 1) || ((px & 1) == 1 && (py & 1) == 0)) { rgb[1] += col; w[1] += wt; }
-        else if((px & 1) == 0 && (py & 1) == 0) { rgb[0] += col; w[0] += wt; }
-        else { rgb[2] += col; w[2] += wt; }
-      }
-    }
-    const real g = rgb[1] / FMAX(R(1e-8), w[1]);
+    const v4 cov = xt ? fetch(ga, x / 3, y / 3, 0) : fetch(ga, (x + 1) / 2, (y + 1) / 2, 0);
+    const real g = splat_px(in, cov, x, y, xt);
     v4 o = { g, g, g, R(1.0) };
     store(out, x, y, 0, o);
   }
@@ -1951,8 +2124,63 @@ static double *alloc_img(int wd, int ht, int chan, int planes)
   return (double *)calloc((size_t)wd * ht * chan * planes, sizeof(double));
 }
 
+// whole-graph near-tie mask (full resolution): the texels of the output
+// that a near tie of a branch inside the graph can reach.  A denoise flag
+// (downcov's hot-pixel and determinant tests, assemble's hard threshold)
+// spreads through the three polyphase flower levels, whose taps reach
+// 2.2 x 2^l half-resolution texels at level l, so over r_dn half-resolution
+// texels; a demosaic flag (gauss's determinant and axis snaps) over r_dm
+// full-resolution pixels.  Off unless a mask is set.
+static uint8_t *g_dev_tie = 0;
+static double g_dev_eps = 0.0;
+static int g_dev_rdn = 0, g_dev_rdm = 0;
+void or_set_develop_ties(uint8_t *mask, double eps, int r_dn, int r_dm)
+{
+  g_dev_tie = mask; g_dev_eps = eps; g_dev_rdn = r_dn; g_dev_rdm = r_dm;
+}
+
+// max filter of radius r (a (2r+1)^2 square), separable
+static void dilate(uint8_t *m, int w, int h, int r)
+{
+  if(r <= 0) return;
+  uint8_t *t = (uint8_t *)malloc((size_t)w * h);
+  int *run = (int *)malloc(sizeof(int) * (size_t)(w > h ? w : h) + sizeof(int));
+  for(int y = 0; y < h; y++)
+  {
+    run[0] = 0;
+    for(int x = 0; x < w; x++) run[x + 1] = run[x] + (m[(size_t)y * w + x] != 0);
+    for(int x = 0; x < w; x++)
+    {
+      const int a = x - r < 0 ? 0 : x - r, b = x + r + 1 > w ? w : x + r + 1;
+      t[(size_t)y * w + x] = run[b] - run[a] > 0;
+    }
+  }
+  for(int x = 0; x < w; x++)
+  {
+    run[0] = 0;
+    for(int y = 0; y < h; y++) run[y + 1] = run[y] + (t[(size_t)y * w + x] != 0);
+    for(int y = 0; y < h; y++)
+    {
+      const int a = y - r < 0 ? 0 : y - r, b = y + r + 1 > h ? h : y + r + 1;
+      m[(size_t)y * w + x] = run[b] - run[a] > 0;
+    }
+  }
+  free(t); free(run);
+}
+
+// half-resolution flags onto the full-resolution mask: each flagged block
+// covers its block x block pixels
+static void tie_upscale(uint8_t *full, int W, int H, const uint8_t *half, int Qw, int Qh, int block)
+{
+  for(int y = 0; y < H; y++) for(int x = 0; x < W; x++)
+  {
+    const int qx = x / block < Qw ? x / block : Qw - 1, qy = y / block < Qh ? y / block : Qh - 1;
+    if(half[(size_t)qy * Qw + qx]) full[(size_t)y * W + x] = 1;
+  }
+}
+
 int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double *out_rgba,
-               double *st_denoise, double *st_demosaic, double *st_colour, double *st_filmcurv)
+               double *st_denoise, double *st_demosaic, double *st_colour, double *st_filmcurv, double *st_llap)
 {
   const int xt = p->filters == 9;
   const int block = xt ? 3 : 2;
@@ -1964,6 +2192,13 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
   ((int32_t *)dpar)[8] = 1;
 
   or_buf_t raw = { (void *)cfa, W, H, 1, OR_UI16, 1 };
+  uint8_t *t_dc = 0, *t_as = 0, *t_ga = 0;
+  if(g_dev_tie)
+  {
+    t_dc = (uint8_t *)calloc((size_t)Qw * Qh, 1);
+    t_as = (uint8_t *)calloc((size_t)Qw * Qh, 1);
+    t_ga = (uint8_t *)calloc((size_t)Qw * Qh, 1);
+  }
   double *half = alloc_img(Qw, Qh, 4, 1), *s[4], *asm_ = alloc_img(Qw, Qh, 4, 1);
   for(int l = 0; l < 4; l++) s[l] = alloc_img(Qw, Qh, 4, 1);
   double *dn = alloc_img(W, H, 1, 1);
@@ -1984,7 +2219,11 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
     for(int k = 0; k < 4; k++) { f[k] = p->wb[k]; f[4 + k] = black[k]; f[8 + k] = white[k]; pc[12 + k] = 0; }
     f[16] = p->noise_a; f[17] = p->noise_b; pc[18] = l; pc[19] = block;
     or_buf_t c[2] = { { l == 0 ? half : s[l - 1], Qw, Qh, 4, OR_F16, 1 }, { s[l], Qw, Qh, 4, OR_F16, 1 } };
+    // downcov's input is the exact block average of the raw (both sides
+    // compute it alike), so only its own rounding can move a decision there
+    if(l == 0 && t_dc) or_set_ties(t_dc, Qw, Qh, g_dev_eps < 1e-5 ? g_dev_eps : 1e-5);
     err |= or_node("denoise", l == 0 ? "downcov" : "down", pc, sizeof(pc), dpar, sizeof(dpar), c, 2, Qw, Qh, 1);
+    or_set_ties(0, 0, 0, 0.0);
   }
   {
     int32_t pc[19];
@@ -1994,7 +2233,9 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
     or_buf_t c[6] = {
       { half, Qw, Qh, 4, OR_F16, 1 }, { s[0], Qw, Qh, 4, OR_F16, 1 }, { s[1], Qw, Qh, 4, OR_F16, 1 },
       { s[2], Qw, Qh, 4, OR_F16, 1 }, { s[3], Qw, Qh, 4, OR_F16, 1 }, { asm_, Qw, Qh, 4, OR_F16, 1 } };
+    if(t_as) or_set_ties(t_as, Qw, Qh, g_dev_eps);
     err |= or_node("denoise", "assemble", pc, sizeof(pc), dpar, sizeof(dpar), c, 6, Qw, Qh, 1);
+    or_set_ties(0, 0, 0, 0.0);
   }
   {
     int32_t pc[24];
@@ -2068,7 +2309,9 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
     int32_t dmpar[2] = { p->dm_colour, 0 };
     or_buf_t cin = { dn, W, H, 1, OR_F16, 1 };
     or_buf_t cg[3] = { cin, cin, { cov, Qw, Qh, 4, OR_F16, 1 } };
+    if(t_ga) or_set_ties(t_ga, Qw, Qh, g_dev_eps);
     err |= or_node("demosaic", "gauss", pc, sizeof(pc), dmpar, sizeof(dmpar), cg, 3, Qw, Qh, 1);
+    or_set_ties(0, 0, 0, 0.0);
     or_buf_t cs[3] = { cin, { cov, Qw, Qh, 4, OR_F16, 1 }, { green, W, H, 1, OR_F16, 1 } };
     err |= or_node("demosaic", "splat", pc, sizeof(pc), dmpar, sizeof(dmpar), cs, 3, W, H, 1);
     or_buf_t cf[4] = { cin, { green, W, H, 1, OR_F16, 1 }, { cov, Qw, Qh, 4, OR_F16, 1 }, { dm, W, H, 4, OR_F16, 1 } };
@@ -2076,6 +2319,24 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
   }
   free(cov); free(green); free(dn);
   if(st_demosaic) memcpy(st_demosaic, dm, sizeof(double) * W * H * 4);
+  if(g_dev_tie)
+  { // downcov flags sit at its polyphase-sorted store positions: back to the block grid
+    uint8_t *dn_t = (uint8_t *)calloc((size_t)Qw * Qh, 1);
+    for(int y = 0; y < Qh; y++) for(int x = 0; x < Qw; x++)
+    {
+      int sx, sy;
+      sorted_pos(x, y, Qw, Qh, &sx, &sy);
+      dn_t[(size_t)y * Qw + x] = t_dc[(size_t)sy * Qw + sx] | t_as[(size_t)y * Qw + x];
+    }
+    dilate(dn_t, Qw, Qh, g_dev_rdn);
+    memset(g_dev_tie, 0, (size_t)W * H);
+    tie_upscale(g_dev_tie, W, H, dn_t, Qw, Qh, block);
+    uint8_t *dm_t = (uint8_t *)calloc((size_t)W * H, 1);
+    tie_upscale(dm_t, W, H, t_ga, Qw, Qh, block);
+    dilate(dm_t, W, H, g_dev_rdm);
+    for(size_t i = 0; i < (size_t)W * H; i++) g_dev_tie[i] |= dm_t[i];
+    free(dn_t); free(dm_t); free(t_dc); free(t_as); free(t_ga);
+  }
 
   // colour: commit_params (colour/main.c:219-365) for matrix=1, white=0
   double *co = alloc_img(W, H, 4, 1);
@@ -2152,6 +2413,7 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
       err |= or_node("llap", "assemble", pca, sizeof(pca), lp, sizeof(lp), ca, 4, lw[l - 1], lh[l - 1], 1);
       if(l < nl - 1) { free(outl[l]); outl[l] = 0; }
     }
+    if(st_llap) memcpy(st_llap, outl[0], sizeof(double) * W * H);
     or_buf_t cl[3] = { { outl[0], W, H, 1, OR_F16, 1 }, { fc, W, H, 4, OR_F16, 1 }, { out_rgba, W, H, 4, OR_F32, 1 } };
     err |= or_node("llap", "colour", 0, 0, lp, sizeof(lp), cl, 3, W, H, 1);
     free(outl[0]);

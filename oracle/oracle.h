@@ -5,9 +5,12 @@
  * links, loads or calls this code.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg use it, and only as the checker.
  *
- * One source, two builds:
- *   liboracle64.so  arithmetic in double ("FP64 oracle")
- *   liboracle32.so  arithmetic in float  (same operation order as the GLSL)
+ * One source, three builds:
+ *   liboracle64.so   arithmetic in double ("FP64 oracle")
+ *   liboracle32.so   arithmetic in float  (same operation order as the GLSL)
+ *   liboracle32p.so  the float build with every transcendental and square
+ *                    root moved by one ulp by a hash of its argument: a
+ *                    second float implementation for the error-bound tests
  * Buffers are always double in host memory; with storage mode on (the
  * default) every f16 image store is rounded to the nearest binary16 value
  * (round-to-nearest-even, overflow to inf), which is what the reference's
@@ -36,6 +39,21 @@ or_buf_t;
 void or_set_storage_f16(int on);
 // 8 for the float build, 16 for the double build
 int or_real_bytes(void);
+
+// near-tie flags of the next or_node calls: a node with a data-dependent
+// branch sets mask[y*wd+x] = 1 for each output texel whose decision margin
+// is at most eps times the magnitude of the decision's operands (mask 0: off)
+void or_set_ties(uint8_t *mask, int wd, int ht, double eps);
+// a near tie counts where its other branch moves the node's result by more
+// than `effect` (absolute), or by more than one binary16 ulp with effect 0
+void or_set_tie_effect(double effect);
+// the same for or_develop, gathered over its nodes into one full-resolution
+// mask (W x H): denoise flags dilated by r_dn half-resolution texels,
+// demosaic flags by r_dm pixels (mask 0: off).  The effect threshold of
+// or_set_tie_effect applies.
+void or_set_develop_ties(uint8_t *mask, double eps, int r_dn, int r_dm);
+// liboracle32p.so only: the seed of the hash that moves its transcendentals
+void or_set_perturb_seed(uint32_t seed);
 
 // run one node with the push-constant and params blobs exactly as the
 // module's create_nodes/commit_params produced them (std140 layouts of the
@@ -97,13 +115,14 @@ typedef struct or_develop_t
 or_develop_t;
 
 // out_rgba: W*H*4 doubles (rgba f32 output of llap/colour).  stage buffers
-// are optional (may be 0): denoise W*H*1, demosaic/colour/filmcurv W*H*4.
+// are optional (may be 0): denoise W*H*1, demosaic/colour/filmcurv W*H*4,
+// llap W*H*1 (the assembled luminance that llap/colour transfers).
 int or_develop(
     const uint16_t *cfa, int W, int H,
     const or_develop_t *p,
     double *out_rgba,
     double *st_denoise, double *st_demosaic,
-    double *st_colour, double *st_filmcurv);
+    double *st_colour, double *st_filmcurv, double *st_llap);
 
 #ifdef __cplusplus
 }

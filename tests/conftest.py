@@ -16,7 +16,15 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def hip():
-    """The C-ABI binding with device 0 initialised (GPU tests only)."""
+    """The C-ABI binding with device 0 initialised (GPU tests only).
+
+    torch (used by some GPU tests for device memory) is imported first: its
+    wheel carries its own HIP runtime under the same soname, and loaded first
+    it is the one libvkdt_hip.so binds to, so the process holds one runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     from vkdt_amd import hip as H
     H.init(0)
     return H

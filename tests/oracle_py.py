@@ -39,6 +39,8 @@ _libs = {}
 
 
 def lib(bits=64):
+    """64: the double build, 32: the float build, "32p": the float build
+    with its transcendentals moved by one ulp"""
     if bits not in _libs:
         path = os.path.join(ORACLE_DIR, f"liboracle{bits}.so")
         if not os.path.exists(path):
@@ -53,7 +55,7 @@ def load(path):
     L.or_node.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                           C.POINTER(OrBuf), C.c_int, C.c_int, C.c_int, C.c_int]
     L.or_develop.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(OrDevelop), C.c_void_p,
-                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_f16_round.argtypes = [C.c_double]
     L.or_f16_round.restype = C.c_double
     L.or_mirror.argtypes = [C.c_int, C.c_int]
@@ -63,6 +65,9 @@ def load(path):
     L.or_evd2x2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_set_storage_f16.argtypes = [C.c_int]
     L.or_nprof_fit.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.or_set_ties.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    L.or_set_develop_ties.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
+    L.or_set_tie_effect.argtypes = [C.c_double]
     return L
 
 
@@ -85,9 +90,14 @@ def orbuf(arr, fmt, chan, planes=1):
     return b
 
 
-def node(bits, module, kernel, pc, params, bufs, wd, ht, dp=1, storage_f16=True):
+def node(bits, module, kernel, pc, params, bufs, wd, ht, dp=1, storage_f16=True, ties=None, tie_eps=1e-5):
+    """one node on the oracle; `ties` (uint8 array of the output extent, or
+    None) receives the node's near-tie flags at margin `tie_eps`"""
     L = lib(bits)
     L.or_set_storage_f16(1 if storage_f16 else 0)
+    if ties is not None:
+        assert ties.dtype == np.uint8 and ties.flags.c_contiguous
+        L.or_set_ties(ties.ctypes.data, ties.shape[1], ties.shape[0], tie_eps)
     conn = (OrBuf * len(bufs))(*bufs)
     pcb = None if pc is None else np.ascontiguousarray(pc)
     pab = None if params is None else np.ascontiguousarray(params)
@@ -95,24 +105,37 @@ def node(bits, module, kernel, pc, params, bufs, wd, ht, dp=1, storage_f16=True)
                    None if pcb is None else pcb.ctypes.data, 0 if pcb is None else pcb.nbytes,
                    None if pab is None else pab.ctypes.data, 0 if pab is None else pab.nbytes,
                    conn, len(bufs), wd, ht, dp)
+    L.or_set_ties(None, 0, 0, 0.0)
     if rc != 0:
         raise RuntimeError(f"oracle {module}/{kernel} failed: {rc}")
 
 
-def develop(bits, cfa, p, stages=False, storage_f16=True):
-    """Whole graph on the CPU; returns rgba float64 H x W x 4 (and stages)."""
+def develop(bits, cfa, p, stages=False, storage_f16=True, ties=None, tie_eps=2e-3, tie_radii=(16, 4),
+            tie_effect=2.5e-4):
+    """Whole graph on the CPU; returns rgba float64 H x W x 4 (and stages).
+    `ties` (uint8 H x W, or None) receives the graph's near-tie mask: margins
+    within `tie_eps` of the operands' magnitude whose other branch moves the
+    node's result by more than `tie_effect`, denoise flags dilated by
+    tie_radii[0] block texels and demosaic flags by tie_radii[1] pixels."""
     L = lib(bits)
     L.or_set_storage_f16(1 if storage_f16 else 0)
     H, W = cfa.shape
+    if ties is not None:
+        assert ties.dtype == np.uint8 and ties.shape == (H, W) and ties.flags.c_contiguous
+        L.or_set_develop_ties(ties.ctypes.data, tie_eps, tie_radii[0], tie_radii[1])
+        L.or_set_tie_effect(tie_effect)
     cfa = np.ascontiguousarray(cfa, dtype=np.uint16)
     out = np.zeros((H, W, 4), np.float64)
     st = {}
     if stages:
         st = {"denoise": np.zeros((H, W), np.float64), "demosaic": np.zeros((H, W, 4), np.float64),
-              "colour": np.zeros((H, W, 4), np.float64), "filmcurv": np.zeros((H, W, 4), np.float64)}
+              "colour": np.zeros((H, W, 4), np.float64), "filmcurv": np.zeros((H, W, 4), np.float64),
+              "llap": np.zeros((H, W), np.float64)}
     ptr = lambda k: st[k].ctypes.data if k in st else None
     rc = L.or_develop(cfa.ctypes.data, W, H, C.byref(p), out.ctypes.data,
-                      ptr("denoise"), ptr("demosaic"), ptr("colour"), ptr("filmcurv"))
+                      ptr("denoise"), ptr("demosaic"), ptr("colour"), ptr("filmcurv"), ptr("llap"))
+    L.or_set_develop_ties(None, 0.0, 0, 0)
+    L.or_set_tie_effect(0.0)
     if rc != 0:
         raise RuntimeError(f"oracle develop failed: {rc}")
     return (out, st) if stages else out

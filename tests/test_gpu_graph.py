@@ -1,11 +1,8 @@
 """The product path end to end on the GPU: DNG -> i-raw -> denoise ->
 demosaic -> colour -> filmcurv -> llap -> o-pfm through the host runtime
 (module DSOs, cfg, roi passes, node creation, arena allocation) and the HIP
-C ABI, compared with the oracle's whole-graph restatement on the same raw.
-
-Tolerance vs the FP64 oracle (f16 storage between nodes, as the reference
-stores): PSNR >= 60 dB on linear rec2020 and at most 0.1% of values off by
-more than 1e-3 (branch flips between float and double arithmetic).
+C ABI, compared with the oracle's whole-graph restatement on the same raw
+under the end-to-end bound of tests/parity.py.
 """
 import os
 
@@ -13,7 +10,7 @@ import numpy as np
 import pytest
 
 import graphspec as gs
-import oracle_py as op
+import parity
 from vkdt_amd import pfm, pipe, synth
 
 pytestmark = pytest.mark.gpu
@@ -38,14 +35,7 @@ def test_graph_vs_oracle(tmp_path, hip, kind, W, H, seed):
     ip = g.img_param("i-raw", "main")
     p = gs.DevelopParams(kind=kind, noise_a=np.float32(na), noise_b=np.float32(nb),
                          cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,))
-    ref = op.develop(64, cfa[:img.shape[0], :img.shape[1]], gs.oracle_params(p))[..., :3]
-    d = np.abs(img.astype(np.float64) - ref)
-    psnr = 10 * np.log10(1.0 / max(float(np.mean(d ** 2)), 1e-30))
-    frac = float(np.mean(d > 1e-3))
-    print(f"[gpu_graph] {kind}: max abs {d.max():.3e} frac>1e-3 {frac:.2e} psnr {psnr:.1f} dB", flush=True)
-    assert np.isfinite(img).all()
-    assert psnr >= 60.0
-    assert frac <= 1e-3
+    parity.e2e_check(cfa[:img.shape[0], :img.shape[1]], p, img, log=lambda s: print(s, flush=True), label=f" {kind}")
     g.close()
 
 
@@ -67,3 +57,37 @@ def test_cli_writes_pfm(tmp_path, hip):
     assert r.returncode == 0
     img, _ = pfm.read_pfm(out + ".pfm")
     assert img.shape == (H, W, 3) and np.isfinite(img).all()
+
+
+def test_replay_equals_eager_and_follows_params(tmp_path, hip):
+    """dt_graph_set_replay: repeated runs replay the recorded launches as one
+    hipGraph; their results equal the eager run's, and a changed parameter
+    records again (the replayed result follows it)"""
+    W, H = 256, 192
+    cfa, _ = synth.mosaic(W, H, 9, "bayer")
+    dng = str(tmp_path / "r.dng")
+    synth.write_dng(dng, cfa, "bayer")
+    na, nb = synth.noise_params(2e-4, 1e-6)
+
+    def develop(clarity, replay, runs):
+        g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb, clarity=clarity))
+        out = str(tmp_path / f"o{clarity}{replay}")
+        g.export("o-pfm", out)
+        g.set_replay(replay)
+        imgs = []
+        for _ in range(runs):
+            g.run()
+            imgs.append(pfm.read_pfm(out + ".pfm")[0].copy())
+        return g, out, imgs
+
+    _, _, eager = develop(0.2, False, 1)
+    g, out, rep = develop(0.2, True, 3)
+    for k, img in enumerate(rep):
+        assert np.array_equal(img, eager[0]), k
+    g.set_param("llap", "01", "clarity", 0.5)
+    g.run()
+    changed = pfm.read_pfm(out + ".pfm")[0]
+    g.close()
+    _, _, eager5 = develop(0.5, False, 1)
+    assert np.array_equal(changed, eager5[0])
+    assert not np.array_equal(changed, eager[0])

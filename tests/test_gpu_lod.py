@@ -8,7 +8,7 @@ import pytest
 
 import graphspec as gs
 import oracle_py as op
-from test_gpu_nodes import f16_ulps, run_gpu_graph, to_or
+from parity import f16_ulps, run_gpu_graph, to_or
 from vkdt_amd import pfm, pipe, synth
 
 pytestmark = pytest.mark.gpu

@@ -25,12 +25,12 @@ MUTANTS = [
      "{ {1.2, -0.4}, {-1.2, 0.4}, {0.4, 1.2}, {-0.4, -1.2} }", "down.comp:96-99"),
     ("band_scale_b0_0.5", "R(0.7000) / block", "R(0.5000) / block", "assemble.comp:90"),
     ("level_decay_0.9", "POW(R(0.7), R(level))", "POW(R(0.9), R(level))", "down.comp:85"),
-    ("bayer_eval_scale_0.5", "eval[0] *= R(0.2); eval[1] *= R(0.2);", "eval[0] *= R(0.5); eval[1] *= R(0.5);",
+    ("bayer_eval_scale_0.5", "const real sc = xt ? R(0.4) : R(0.2);", "const real sc = xt ? R(0.4) : R(0.5);",
      "gauss.comp:68"),
-    ("xtrans_eval_scale_0.2", "eval[0] *= R(0.4); eval[1] *= R(0.4);", "eval[0] *= R(0.2); eval[1] *= R(0.2);",
+    ("xtrans_eval_scale_0.2", "const real sc = xt ? R(0.4) : R(0.2);", "const real sc = xt ? R(0.2) : R(0.2);",
      "gauss.comp:120"),
     ("downcov_minor_scale_0.1", "eval[1] *= R(0.05);", "eval[1] *= R(0.1);", "cov.glsl:118"),
-    ("hot_pixel_factor_3", "if(t.x > R(2.0) * mean_b) continue;", "if(t.x > R(3.0) * mean_b) continue;",
+    ("hot_pixel_factor_3", "int hot = t.x > R(2.0) * mean_b;", "int hot = t.x > R(3.0) * mean_b;",
      "cov.glsl:130"),
     ("down_centre_weight_0.3", "const real t = R(0.2);", "const real t = R(0.3);", "down.comp:73"),
     ("hard_threshold_5", "FABS(d[l][0]) > R(10.0)", "FABS(d[l][0]) > R(5.0)", "assemble.comp:140"),

@@ -1,6 +1,7 @@
 """Build the native parts in-tree.
 
   vkdt_amd/lib/libvkdt_hip.so    HIP kernels + C ABI (hipcc, gfx950)
+  vkdt_amd/lib/libvkdt_calib.so  instruction-rate calibration kernels (bench.py)
   vkdt_amd/lib/libvkdt_pipe.so   host graph runtime (C, gcc)
   vkdt_amd/modules/<m>/lib<m>.so module DSOs (C, gcc)
   vkdt_amd/bin/vkdt-cli          command line front end
@@ -54,6 +55,15 @@ def build_hip():
     return out
 
 
+def build_calib():
+    """the instruction-rate calibration kernels bench.py --calibrate-valu times"""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libvkdt_calib.so")
+    run([HIPCC] + HIP_FLAGS + ["-shared", "-o", out + ".tmp", os.path.join(CSRC, "calib", "calib.hip")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_pipe():
     """host runtime + module DSOs + cli (only when their sources exist)"""
     if not os.path.isdir(PIPE) or not glob.glob(os.path.join(PIPE, "*.c")):
@@ -90,7 +100,7 @@ def build_oracle():
 
 
 def build_all(verbose=True):
-    steps = [("hip", build_hip), ("pipe", build_pipe), ("oracle", build_oracle)]
+    steps = [("hip", build_hip), ("calib", build_calib), ("pipe", build_pipe), ("oracle", build_oracle)]
     for name, fn in steps:
         fn()
         if verbose:

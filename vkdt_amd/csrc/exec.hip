@@ -216,6 +216,34 @@ int vkdt_hip_event_query(void *event)
   return hip_fail(e, "hipEventQuery");
 }
 
+// stream capture of a run's launches into a replayable graph
+int vkdt_hip_capture_begin(void *stream)
+{
+  return hip_fail(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+}
+
+int vkdt_hip_capture_end(void *stream, void **graph_exec)
+{
+  hipGraph_t gr = nullptr;
+  *graph_exec = nullptr;
+  if(int e = hip_fail(hipStreamEndCapture((hipStream_t)stream, &gr), "hipStreamEndCapture")) return e;
+  hipGraphExec_t ex = nullptr;
+  const int e = hip_fail(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(gr);
+  if(!e) *graph_exec = (void *)ex;
+  return e;
+}
+
+int vkdt_hip_graph_launch(void *graph_exec, void *stream)
+{
+  return hip_fail(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int vkdt_hip_graph_destroy(void *graph_exec)
+{
+  return graph_exec ? hip_fail(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy") : 0;
+}
+
 const char *vkdt_hip_last_error(void) { return g_err; }
 
 } // extern "C"

@@ -41,6 +41,7 @@ def lib():
         L.dt_graph_last_error.restype = cp
         L.dt_graph_set_perf.argtypes = [vp, i32]
         L.dt_graph_set_skip_dead.argtypes = [vp, i32]
+        L.dt_graph_set_replay.argtypes = [vp, i32]
         L.dt_graph_set_frame.argtypes = [vp, i32]
         L.dt_graph_frame_cnt.argtypes = [vp]
         L.dt_graph_frame_rate.argtypes = [vp]
@@ -169,6 +170,10 @@ class Graph:
 
     def set_skip_dead(self, on=True):
         self.L.dt_graph_set_skip_dead(self.g, 1 if on else 0)
+
+    def set_replay(self, on=True):
+        """record repeated runs' node launches once and replay them as one hipGraph"""
+        self.L.dt_graph_set_replay(self.g, 1 if on else 0)
 
     def set_frame(self, frame):
         """the frame the next run delivers (video sources)"""

@@ -24,6 +24,9 @@ void dt_graph_free(dt_graph_t *g)
 
 void dt_graph_set_perf(dt_graph_t *g, int on) { g->perf = on; }
 void dt_graph_set_skip_dead(dt_graph_t *g, int on) { g->skip_dead = on; }
+// record repeated runs' node launches once and replay them as a hipGraph
+// (no effect while per-node timing is on)
+void dt_graph_set_replay(dt_graph_t *g, int on) { g->replay = on; }
 // the frame a run delivers (graph-export.c:268-271 sets graph->frame per frame)
 void dt_graph_set_frame(dt_graph_t *g, int frame) { g->frame = frame; }
 int  dt_graph_frame_cnt(const dt_graph_t *g) { return g->frame_cnt; }

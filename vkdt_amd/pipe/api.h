@@ -321,6 +321,9 @@ struct dt_graph_t
   void    *ev[DT_GRAPH_MAX_NODES + 1];
   float    node_ms[DT_GRAPH_MAX_NODES];
   int      skip_dead;       // skip nodes whose outputs no kernel samples
+  int      replay;          // record a run's launches as one hipGraph and replay it
+  void    *rec;             // the recorded launches (a hipGraph executable), or 0
+  uint64_t rec_hash;        // hash of what the recorded launches saw
   char     err[256];        // last error text ("gui_msg")
 };
 

@@ -90,6 +90,8 @@ int dt_graph_init(dt_graph_t *g)
 
 static void free_device(dt_graph_t *g)
 {
+  if(g->rec) vkdt_hip_graph_destroy(g->rec);
+  g->rec = 0;
   if(g->arena) vkdt_hip_free(g->arena);
   if(g->staging) vkdt_hip_host_free(g->staging);
   g->arena = g->staging = 0;
@@ -938,6 +940,42 @@ static void fill_buf(const dt_graph_t *g, const dt_node_t *n, int c, vkdt_buf_t 
   b->plane_stride = dt_connector_bufsize(own, own->roi.wd, own->roi.ht);
 }
 
+// FNV-1a over what a run's launches see: every executed node's kernel,
+// extent, push constants, uniform blob and connector buffers
+static uint64_t fnv(uint64_t h, const void *p, size_t n)
+{
+  const uint8_t *b = (const uint8_t *)p;
+  for(size_t i = 0; i < n; i++) { h ^= b[i]; h *= 0x100000001b3ull; }
+  return h;
+}
+
+static uint64_t record_hash(const dt_graph_t *g)
+{
+  uint64_t h = 0xcbf29ce484222325ull;
+  for(int k = 0; k < g->num_exec; k++)
+  {
+    const dt_node_t *n = g->node + g->exec[k];
+    if(node_is_source(n) || node_is_sink(n)) continue;
+    const dt_module_t *m = n->module;
+    h = fnv(h, &n->name, sizeof(n->name));
+    h = fnv(h, &n->kernel, sizeof(n->kernel));
+    h = fnv(h, &n->wd, sizeof(n->wd));
+    h = fnv(h, &n->ht, sizeof(n->ht));
+    h = fnv(h, &n->dp, sizeof(n->dp));
+    h = fnv(h, n->push_constant, n->push_constant_size);
+    if(m->committed_param_size) h = fnv(h, m->committed_param, m->committed_param_size);
+    else if(m->param_size) h = fnv(h, m->param, m->param_size);
+    for(int c = 0; c < n->num_connectors; c++)
+    {
+      vkdt_buf_t b;
+      memset(&b, 0, sizeof(b));
+      fill_buf(g, n, c, &b);
+      h = fnv(h, &b, sizeof(b));
+    }
+  }
+  return h;
+}
+
 /* ------------------------------------------------------------------------
  * run
  * --------------------------------------------------------------------- */
@@ -1072,31 +1110,57 @@ int dt_graph_run(dt_graph_t *g, dt_graph_run_t run)
     dt_module_t *m = g->module + order[i];
     if(m->so->commit_params) m->so->commit_params(g, m);
   }
-  // record: one launch per node
+  // record: one launch per node.  With replay on (and no per-node timing),
+  // a run whose launches see the same buffers, push constants and uniforms
+  // as the recorded one replays that recording as one hipGraph instead.
   if(g->perf)
     for(int k = 0; k <= g->num_exec; k++)
       if(!g->ev[k] && vkdt_hip_event_create(&g->ev[k])) return gerr(g, -3, "event: %s", vkdt_hip_last_error());
-  if(g->perf) vkdt_hip_event_record(g->ev[0], g->stream);
-  for(int k = 0; k < g->num_exec; k++)
+  const int replay = g->replay && !g->perf;
+  const uint64_t h = replay ? record_hash(g) : 0;
+  if(replay && g->rec && g->rec_hash == h)
   {
-    dt_node_t *n = g->node + g->exec[k];
-    if(!node_is_source(n) && !node_is_sink(n))
+    if(vkdt_hip_graph_launch(g->rec, g->stream)) return gerr(g, -3, "replay: %s", vkdt_hip_last_error());
+  }
+  else
+  {
+    if(g->rec) vkdt_hip_graph_destroy(g->rec);
+    g->rec = 0;
+    if(replay && vkdt_hip_capture_begin(g->stream)) return gerr(g, -3, "capture: %s", vkdt_hip_last_error());
+    if(g->perf) vkdt_hip_event_record(g->ev[0], g->stream);
+    int e = 0;
+    for(int k = 0; k < g->num_exec && !e; k++)
     {
-      vkdt_buf_t conn[DT_MAX_CONNECTORS];
-      for(int c = 0; c < n->num_connectors; c++) fill_buf(g, n, c, conn + c);
-      for(int c = 0; c < n->num_connectors; c++)
-        if((n->connector[c].flags & s_conn_clear) && dt_connector_output(n->connector + c) && conn[c].ptr &&
-           vkdt_hip_buf_clear(conn + c, g->stream))
-          return gerr(g, -3, "clear %" PRItkn ": %s", dt_token_str(n->name), vkdt_hip_last_error());
-      const dt_module_t *m = n->module;
-      const void *params = m->committed_param_size ? (const void *)m->committed_param : (const void *)m->param;
-      const size_t psize = m->committed_param_size ? (size_t)m->committed_param_size : (size_t)m->param_size;
-      const int e = vkdt_hip_node_launch(n->name, n->kernel, n->push_constant, n->push_constant_size,
-                                         params, psize, conn, n->num_connectors, n->wd, n->ht, n->dp, g->stream);
-      if(e) return gerr(g, e, "node %" PRItkn "/%" PRItkn ": %s", dt_token_str(n->name), dt_token_str(n->kernel),
-                        vkdt_hip_last_error());
+      dt_node_t *n = g->node + g->exec[k];
+      if(!node_is_source(n) && !node_is_sink(n))
+      {
+        vkdt_buf_t conn[DT_MAX_CONNECTORS];
+        for(int c = 0; c < n->num_connectors; c++) fill_buf(g, n, c, conn + c);
+        for(int c = 0; c < n->num_connectors && !e; c++)
+          if((n->connector[c].flags & s_conn_clear) && dt_connector_output(n->connector + c) && conn[c].ptr &&
+             vkdt_hip_buf_clear(conn + c, g->stream))
+            e = gerr(g, -3, "clear %" PRItkn ": %s", dt_token_str(n->name), vkdt_hip_last_error());
+        const dt_module_t *m = n->module;
+        const void *params = m->committed_param_size ? (const void *)m->committed_param : (const void *)m->param;
+        const size_t psize = m->committed_param_size ? (size_t)m->committed_param_size : (size_t)m->param_size;
+        if(!e && (e = vkdt_hip_node_launch(n->name, n->kernel, n->push_constant, n->push_constant_size,
+                                           params, psize, conn, n->num_connectors, n->wd, n->ht, n->dp, g->stream)))
+          e = gerr(g, e, "node %" PRItkn "/%" PRItkn ": %s", dt_token_str(n->name), dt_token_str(n->kernel),
+                   vkdt_hip_last_error());
+      }
+      if(g->perf) vkdt_hip_event_record(g->ev[k + 1], g->stream);
     }
-    if(g->perf) vkdt_hip_event_record(g->ev[k + 1], g->stream);
+    if(replay)
+    { // end the capture whatever happened, so the stream leaves capture mode
+      void *rec = 0;
+      const int ec = vkdt_hip_capture_end(g->stream, &rec);
+      if(e) { vkdt_hip_graph_destroy(rec); return e; }
+      if(ec) return gerr(g, -3, "capture: %s", vkdt_hip_last_error());
+      g->rec = rec;
+      g->rec_hash = h;
+      if(vkdt_hip_graph_launch(g->rec, g->stream)) return gerr(g, -3, "replay: %s", vkdt_hip_last_error());
+    }
+    else if(e) return e;
   }
   // download sinks (graph-run-nodes-download.h)
   if(run & s_graph_run_download_sink)
