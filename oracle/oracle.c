@@ -86,9 +86,14 @@ typedef double real;
 // per-pixel nodes write only their own output texel, so their row loops
 // run on all host threads (OpenMP); the accumulating nodes stay serial
 #ifdef _OPENMP
+#include <omp.h>
 #define PAR_ROWS _Pragma("omp parallel for schedule(dynamic, 4)")
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_threads(void) { return omp_get_max_threads(); }
 #else
 #define PAR_ROWS
+void or_set_threads(int n) { (void)n; }
+int or_threads(void) { return 1; }
 #endif
 
 static int g_storage_f16 = 1;

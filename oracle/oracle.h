@@ -39,6 +39,9 @@ or_buf_t;
 void or_set_storage_f16(int on);
 // 8 for the float build, 16 for the double build
 int or_real_bytes(void);
+// host threads of the per-pixel nodes' row loops (OpenMP)
+void or_set_threads(int n);
+int or_threads(void);
 
 // near-tie flags of the next or_node calls: a node with a data-dependent
 // branch sets mask[y*wd+x] = 1 for each output texel whose decision margin

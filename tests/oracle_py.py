@@ -68,6 +68,7 @@ def load(path):
     L.or_set_ties.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
     L.or_set_develop_ties.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.or_set_tie_effect.argtypes = [C.c_double]
+    L.or_set_threads.argtypes = [C.c_int]
     return L
 
 

@@ -153,3 +153,44 @@ def test_camera_matrix_and_white_balance(tmp_path):
                             [0.01763986, -0.04277061, 0.94210312]])
     ref = xyz_to_2020 @ np.linalg.inv(synth.xyz_to_cam())
     assert np.allclose(np.array(ip["cam"]).reshape(3, 3), ref, atol=1e-5)
+
+
+def test_rejects_crop_outside_the_raw(tmp_path, iraw):
+    """DefaultCropOrigin + DefaultCropSize beyond the raw buffer is refused at load"""
+    cfa = np.full((24, 32), 1000, np.uint16)
+    path = str(tmp_path / "c.dng")
+    synth.write_dng(path, cfa, "bayer", extra_tags=[(50719, 4, [0, 0]), (50720, 4, [40, 24])])
+    rc, img, err = load(iraw, path)
+    assert rc != 0 and "outside the 32x24 raw" in err
+
+
+@pytest.mark.parametrize("W,H,crop,what", [(2, 4, None, "smaller than its 2x2 CFA"),
+                                           (32, 24, (2, 24), "smaller than one CFA block")])
+def test_rejects_extents_below_the_cfa_offset(tmp_path, W, H, crop, what):
+    """a GRBG raw (CFA offset 1,0) narrower than the offset plus one CFA, or
+    cropped narrower than the offset plus one block, is refused with a
+    message instead of wrapping its extent"""
+    cfa = np.full((H, W), 1000, np.uint16)
+    path = str(tmp_path / "o.dng")
+    extra = [] if crop is None else [(50719, 4, [0, 0]), (50720, 4, list(crop))]
+    synth.write_dng(path, cfa, "bayer", pattern=[[1, 0], [2, 1]], extra_tags=extra)
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(path))
+    g.export("o-pfm", str(tmp_path / "out"))
+    with pytest.raises(pipe.PipeError):
+        g.plan()
+    assert what in g.error()
+
+
+def test_rejects_lj92_streams_taller_than_their_tile(tmp_path, iraw):
+    """a lossless-JPEG stream with more rows than its tile would write into
+    the next tile's rows: refused at load"""
+    rng = np.random.default_rng(3)
+    cfa = rng.integers(0, 16384, size=(64, 64), dtype=np.uint16)
+    path = str(tmp_path / "t.dng")
+    synth.write_dng(path, cfa, "bayer", compress=dict(tile=(32, 32), extra_rows=2))
+    rc, img, err = load(iraw, path)
+    assert rc != 0 and "does not fill its 32x32 tile" in err
+    synth.write_dng(path, cfa, "bayer", compress=dict(tile=(32, 32)))
+    rc, img, err = load(iraw, path)
+    assert rc == 0, err
+    iraw.dng_free(C.byref(img))

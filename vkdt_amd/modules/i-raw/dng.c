@@ -213,8 +213,12 @@ int dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen)
       uint32_t fw, fh, fc;   // the stream's header, checked now: decoding comes later
       if(vkdt_lj92_info(buf + tile.offset, tile.length, &fw, &fh, &fc, 0))
       { snprintf(err, errlen, "%s: tile %u: %s", filename, k, vkdt_hip_last_error()); goto done; }
-      if((uint64_t)fw * fc * fh < (uint64_t)tw * th)
-      { snprintf(err, errlen, "%s: tile %u: lj92 frame %ux%ux%u smaller than the tile", filename, k, fw, fh, fc); goto done; }
+      if((uint64_t)fw * fc * fh != (uint64_t)tw * th)
+      { // a larger frame would write into the next tile's rows
+        snprintf(err, errlen, "%s: tile %u: lj92 frame %ux%ux%u does not fill its %ux%u tile exactly", filename, k,
+                 fw, fh, fc, tw, th);
+        goto done;
+      }
       img->tiles[k] = tile;
     }
     img->ntiles = ntx * nty;
@@ -317,6 +321,14 @@ decoded:
       img->crop_aabb[0] = img->crop_aabb[1] = 0;
       img->crop_aabb[2] = W;
       img->crop_aabb[3] = H;
+    }
+    if(img->crop_aabb[2] > W || img->crop_aabb[3] > H ||
+       img->crop_aabb[0] >= img->crop_aabb[2] || img->crop_aabb[1] >= img->crop_aabb[3])
+    {
+      snprintf(err, errlen, "%s: DefaultCrop %llu,%llu..%llu,%llu outside the %ux%u raw", filename,
+               (unsigned long long)img->crop_aabb[0], (unsigned long long)img->crop_aabb[1],
+               (unsigned long long)img->crop_aabb[2], (unsigned long long)img->crop_aabb[3], W, H);
+      goto done;
     }
   }
   copy_str(&t, find(&ifd0, 271), img->maker, sizeof(img->maker));

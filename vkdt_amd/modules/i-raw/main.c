@@ -69,8 +69,10 @@ static int cfa_at(const dng_image_t *img, int row, int col)
   return img->cfa[(row % img->cfa_h) * img->cfa_w + (col % img->cfa_w)];
 }
 
-// rawloader-c/lib.rs:200-279: offset to RGGB / canonical x-trans, block rounding
-static void normalise_cfa(raw_t *r)
+// rawloader-c/lib.rs:200-279: offset to RGGB / canonical x-trans, block
+// rounding.  Returns non-zero (message in err) when the raw or its crop is
+// too small to hold one CFA block past the offset.
+static int normalise_cfa(raw_t *r, char *err, size_t errlen)
 {
   const dng_image_t *img = &r->img;
   int ox = 0, oy = 0;
@@ -94,6 +96,18 @@ static void normalise_cfa(raw_t *r)
     else if(cfa_at(img, 0, 0) == 2) { ox = 1; oy = 1; }
   }
   const uint64_t block = img->cfa_w == 6 ? 3 : 2, big = img->cfa_w;
+  if((uint64_t)img->width < (uint64_t)ox + img->cfa_w || (uint64_t)img->height < (uint64_t)oy + img->cfa_h)
+  {
+    snprintf(err, errlen, "raw %llux%llu smaller than its %ux%u CFA at offset %d,%d",
+             (unsigned long long)img->width, (unsigned long long)img->height, img->cfa_w, img->cfa_h, ox, oy);
+    return 1;
+  }
+  if(img->crop_aabb[2] < (uint64_t)ox + block || img->crop_aabb[3] < (uint64_t)oy + block)
+  {
+    snprintf(err, errlen, "crop %llux%llu smaller than one CFA block at offset %d,%d",
+             (unsigned long long)img->crop_aabb[2], (unsigned long long)img->crop_aabb[3], ox, oy);
+    return 1;
+  }
   uint64_t *b = r->crop_aabb;
   for(int k = 0; k < 4; k++) b[k] = img->crop_aabb[k];
   b[2] -= ox;
@@ -106,6 +120,7 @@ static void normalise_cfa(raw_t *r)
   r->height = ((img->height - oy) / block) * block;
   r->ox = ox;
   r->oy = oy;
+  return 0;
 }
 
 static int load_raw(dt_module_t *mod, int frame, const char *filename)
@@ -122,9 +137,16 @@ static int load_raw(dt_module_t *mod, int frame, const char *filename)
     r->frame = -1;
     return 1;
   }
+  if(normalise_cfa(r, err, sizeof(err)))
+  {
+    fprintf(stderr, "[i-raw] %s: %s\n", filename, err);
+    snprintf(mod->graph->err, sizeof(mod->graph->err), "%s: %s", filename, err);
+    dng_free(&r->img);
+    r->frame = -1;
+    return 1;
+  }
   snprintf(r->filename, sizeof(r->filename), "%s", filename);
   r->frame = frame;
-  normalise_cfa(r);
   return 0;
 }
 

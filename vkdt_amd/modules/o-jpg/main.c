@@ -40,6 +40,13 @@ static int encode(dt_module_t *module, const void *dev, void *stream)
   const float quality = dt_module_param_float(module, 1)[0];
   const int wd = module->connector[0].roi.wd, ht = module->connector[0].roi.ht;
   const int hs = quality > 92.0f ? 1 : 2, vs = quality > 90.0f ? 1 : 2;
+  // the reference's libjpeg setup (o-jpg/main.c:142-146) takes the float DCT
+  // above quality 95, the fast integer DCT below 50 and input smoothing
+  // below 80; this encoder has the accurate integer DCT without smoothing
+  // only, so files outside [80, 95] differ from the reference's
+  if(quality > 95.0f || quality < 80.0f)
+    fprintf(stderr, "[o-jpg] quality %g: encoded with the integer DCT and no smoothing; the reference's files "
+            "differ outside quality 80..95\n", quality);
   uint8_t *jpg = 0;
   size_t len = 0;
   if(vkdt_hip_jpg_encode(dev, wd, ht, wd, (int)quality, hs, vs, &jpg, &len, stream))

@@ -26,7 +26,12 @@ static int gerr(dt_graph_t *g, int code, const char *fmt, ...)
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
-  if(g) snprintf(g->err, sizeof(g->err), "%s", buf);
+  if(g)
+  { // a module's own message of this pass (graph->err, gui_msg) comes first
+    const size_t n = strnlen(g->err, sizeof(g->err));
+    if(n && n + 3 < sizeof(g->err)) snprintf(g->err + n, sizeof(g->err) - n, "; %s", buf);
+    else if(!n) snprintf(g->err, sizeof(g->err), "%s", buf);
+  }
   fprintf(stderr, "[pipe] %s\n", buf);
   return code;
 }
@@ -994,6 +999,7 @@ static int module_order(dt_graph_t *g, int *order, int *cnt)
 int dt_graph_plan(dt_graph_t *g, dt_graph_run_t run)
 {
   int order[DT_GRAPH_MAX_MODULES], cnt = 0;
+  g->err[0] = 0;
   if(module_order(g, order, &cnt)) return -1;
 
   if(run & s_graph_run_roi)
@@ -1067,6 +1073,7 @@ int dt_graph_plan(dt_graph_t *g, dt_graph_run_t run)
 int dt_graph_run(dt_graph_t *g, dt_graph_run_t run)
 {
   int order[DT_GRAPH_MAX_MODULES], cnt = 0;
+  g->err[0] = 0;
   if(run & (s_graph_run_roi | s_graph_run_create_nodes))
   {
     const int e = dt_graph_plan(g, run);

@@ -149,7 +149,10 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
         tw, th = compress.get("tile", (256, 256))
         ntx, nty = -(-W // tw), -(-H // th)
         padded = np.pad(np.asarray(cfa, np.uint16), ((0, nty * th - H), (0, ntx * tw - W)), mode="edge")
-        chunks = [lj92.encode(padded[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw], compress.get("bits", 16),
+        # extra_rows > 0 writes streams taller than their tile (negative tests)
+        ex = compress.get("extra_rows", 0)
+        padded = np.pad(padded, ((0, ex), (0, 0)), mode="edge")
+        chunks = [lj92.encode(padded[ty * th:(ty + 1) * th + ex, tx * tw:(tx + 1) * tw], compress.get("bits", 16),
                               compress.get("predictor", 1), compress.get("components", 2))
                   for ty in range(nty) for tx in range(ntx)]
     pat = cfa_pattern(kind) if pattern is None else np.asarray(pattern, np.uint8)
