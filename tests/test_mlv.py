@@ -116,8 +116,34 @@ def test_metadata(tmp_path):
     assert abs(ip["exposure"] - 0.01) < 1e-9 and abs(ip["aperture"] - 5.6) < 1e-6 and ip["focal_length"] == 35.0
     assert ip["maker"] == "Canon" and ip["model"] == "Canon EOS 5D Mark III"
     assert ip["noise"] == [1.0, 1.0]
-    assert ip["wb"] == [1.0] * 4
     assert g.frame_cnt() == 2 and abs(g.frame_rate() - 24000 / 1001) < 1e-4
+    # the body's D65 XYZ -> camera matrix (Adobe DNG Converter ColorMatrix2 of
+    # the EOS 5D Mark III, x 1e4): cam -> rec2020 = xyz -> rec2020 . inverse,
+    # the white balance is the row sums of the inverse, green-normalised
+    # (i-mlv/main.c:172-201)
+    M = np.array([[6722, -635, -963], [-4287, 12460, 2028], [-908, 2162, 5668]]) / 10000.0
+    cam_to_xyz = np.linalg.inv(M)
+    xyz_to_2020 = np.array([[1.7166511880, -0.3556707838, -0.2533662814], [-0.6666843518, 1.6164812366, 0.0157685458],
+                            [0.0176398574, -0.0427706133, 0.9421031212]])
+    assert np.allclose(np.array(ip["cam"]).reshape(3, 3), xyz_to_2020 @ cam_to_xyz, rtol=1e-5, atol=1e-6)
+    wb = cam_to_xyz.sum(1)
+    assert np.allclose(ip["wb"][:3], [wb[0] / wb[1], 1.0, wb[2] / wb[1]], rtol=1e-5)
+    g.close()
+
+
+def test_unknown_camera_takes_the_identity(tmp_path):
+    """a body missing from the coefficient table: camera rgb is taken as xyz,
+    with unit white balance (the reference's fallback)"""
+    path = str(tmp_path / "u.mlv")
+    synth.write_mlv(path, clip_frames(n=1), camera="Canon EOS 9000X")
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    g.plan()
+    ip = g.img_param("i-mlv", "main")
+    xyz_to_2020 = [1.7166511880, -0.3556707838, -0.2533662814, -0.6666843518, 1.6164812366, 0.0157685458,
+                   0.0176398574, -0.0427706133, 0.9421031212]
+    assert np.allclose(ip["cam"], xyz_to_2020, rtol=1e-6)
+    assert ip["wb"][:3] == [1.0, 1.0, 1.0]
     g.close()
 
 

@@ -17,6 +17,7 @@
  */
 #include "pipe/api.h"
 #include "mlv.h"
+#include "mlv_coeff.h"
 #include "vkdt_hip.h"
 
 #include <limits.h>
@@ -30,6 +31,8 @@ typedef struct
   mlv_clip_t clip;
 }
 mlv_mod_t;
+
+static int mat3inv_f(float *dst, const float *s);
 
 static int open_clip(dt_module_t *mod)
 {
@@ -124,13 +127,51 @@ void modify_roi_out(dt_graph_t *graph, dt_module_t *mod)
     if(fscanf(f, "%g %g", &na, &nb) == 2) { ip->noise_a = na; ip->noise_b = nb; }
     fclose(f);
   }
-  // camera rgb -> xyz is the identity (no coefficient table): the white
-  // balance baked into it is 1, and cam -> rec2020 is xyz -> rec2020
+  // main.c:172-201: camera rgb -> xyz is the inverse of the body's xyz ->
+  // camera matrix (the identity for an unknown body); the row sums of
+  // cam -> xyz are the white balance baked into it, and cam -> rec2020 is
+  // xyz -> rec2020 times cam -> xyz
+  float xyz_to_cam[9], mat[9] = { 0 };
+  if(mlv_coeff_lookup(ip->model, xyz_to_cam) || mat3inv_f(mat, xyz_to_cam))
+  {
+    fprintf(stderr, "[i-mlv] no colour matrix for camera '%s': camera rgb taken as xyz\n", ip->model);
+    memset(mat, 0, sizeof(mat));
+    mat[0] = mat[4] = mat[8] = 1.0f;
+  }
+  const double cam_to_xyz[9] = { mat[0], mat[1], mat[2], mat[3], mat[4], mat[5], mat[6], mat[7], mat[8] };
+  ip->whitebalance[0] = (float)(cam_to_xyz[0] + cam_to_xyz[1] + cam_to_xyz[2]);
+  ip->whitebalance[1] = (float)(cam_to_xyz[3] + cam_to_xyz[4] + cam_to_xyz[5]);
+  ip->whitebalance[2] = (float)(cam_to_xyz[6] + cam_to_xyz[7] + cam_to_xyz[8]);
+  ip->whitebalance[0] /= ip->whitebalance[1];
+  ip->whitebalance[2] /= ip->whitebalance[1];
+  ip->whitebalance[1] = 1.0f;
   static const float xyz_to_rec2020[9] = {
      1.7166511880f, -0.3556707838f, -0.2533662814f,
     -0.6666843518f,  1.6164812366f,  0.0157685458f,
      0.0176398574f, -0.0427706133f,  0.9421031212f };
-  for(int k = 0; k < 9; k++) ip->cam_to_rec2020[k] = xyz_to_rec2020[k];
+  float c2r[9] = { 0 };
+  for(int j = 0; j < 3; j++) for(int i = 0; i < 3; i++) for(int k = 0; k < 3; k++)
+    c2r[3 * j + i] += xyz_to_rec2020[3 * j + k] * cam_to_xyz[3 * k + i];
+  for(int k = 0; k < 9; k++) ip->cam_to_rec2020[k] = c2r[k];
+}
+
+// core/mat3.h mat3inv: the inverse of a row-major 3x3 in float; 1 if singular
+static int mat3inv_f(float *dst, const float *s)
+{
+  const float det = s[0] * (s[8] * s[4] - s[7] * s[5]) - s[3] * (s[8] * s[1] - s[7] * s[2])
+                  + s[6] * (s[5] * s[1] - s[4] * s[2]);
+  if(det > -1e-7f && det < 1e-7f) return 1;
+  const float id = 1.0f / det;
+  dst[0] =  id * (s[8] * s[4] - s[7] * s[5]);
+  dst[1] = -id * (s[8] * s[1] - s[7] * s[2]);
+  dst[2] =  id * (s[5] * s[1] - s[4] * s[2]);
+  dst[3] = -id * (s[8] * s[3] - s[6] * s[5]);
+  dst[4] =  id * (s[8] * s[0] - s[6] * s[2]);
+  dst[5] = -id * (s[5] * s[0] - s[3] * s[2]);
+  dst[6] =  id * (s[7] * s[3] - s[6] * s[4]);
+  dst[7] = -id * (s[7] * s[0] - s[6] * s[1]);
+  dst[8] =  id * (s[4] * s[0] - s[3] * s[1]);
+  return 0;
 }
 
 static int frame_bytes(dt_module_t *mod, uint8_t **data, uint64_t *size, uint32_t *n, int *bits)
