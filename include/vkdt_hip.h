@@ -286,6 +286,19 @@ int vkdt_hip_batch_set_timing(vkdt_batch_t *batch, int on);
 int vkdt_hip_batch_stage_ms(vkdt_batch_t *batch, float *ms);
 const char *vkdt_hip_batch_stage_name(int stage);
 
+// Develop a batch without allocating: no counterpart in the reference.  For
+// callers that cannot allocate device memory (code under hipGraph capture
+// elsewhere in the process, serving loops and harnesses that forbid
+// hipMalloc).  The intermediates live in the library's static device
+// scratch of VKDT_BATCH_STATIC_BYTES per GPU (held until the process ends)
+// instead of a caller workspace; images whose scratch
+// (vkdt_hip_batch_scratch_bytes) exceeds it are refused with an error.  The
+// caller still passes the mosaics and outputs.  The batch runs eagerly with
+// the built-in launch shape and the call returns when `stream` has drained.
+// Calls that overlap on one GPU share its scratch and race.
+#define VKDT_BATCH_STATIC_BYTES (1ull << 30)
+int vkdt_hip_batch_develop_static(const vkdt_develop_image_t *img, int n, void *stream);
+
 // Release the host side of a batch (after its work has completed; the
 // scratch stays the caller's).
 void vkdt_hip_batch_destroy(vkdt_batch_t *batch);

@@ -31,6 +31,7 @@
 // the 11 full-resolution curve planes (about 96 B per CFA pixel).
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "launch.h"
 #include "ops.h"
 #include "ops_hilite.h"
@@ -1153,4 +1154,55 @@ void vkdt_hip_batch_destroy(vkdt_batch_t *b)
   drop_graph(b);
   for(int s = 0; s <= VKDT_BATCH_STAGES; s++) if(b->ev[s]) (void)hipEventDestroy(b->ev[s]);
   delete b;
+}
+
+// ---------------------------------------------------------------------------
+// A batch in the library's own device scratch, for callers that cannot
+// allocate.  `scratch` is a file-scope device array of VKDT_BATCH_STATIC_BYTES;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (kept per device), never through its name.  Every
+// call on a device shares that device's array: calls that overlap on one
+// device race, and a call leaves its intermediates behind (the tile
+// counters, the only state that must start at zero, are reset by a kernel
+// at the start of each run).
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[VKDT_BATCH_STATIC_BYTES];
+
+#define STATIC_MAX_DEV 64
+static void *g_static_ptr[STATIC_MAX_DEV];
+static std::mutex g_static_mx;
+
+static int static_scratch(void **ptr)
+{
+  int dev = 0;
+  if(int e = hip_fail(hipGetDevice(&dev), "batch static: hipGetDevice")) return e;
+  if(dev < 0 || dev >= STATIC_MAX_DEV) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch static: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_static_mx);
+  if(!g_static_ptr[dev])
+  { // once per device, with that device current
+    void *p = nullptr;
+    if(int e = hip_fail(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "batch static: hipGetSymbolAddress")) return e;
+    g_static_ptr[dev] = p;
+  }
+  *ptr = g_static_ptr[dev];
+  return VKDT_HIP_OK;
+}
+
+int vkdt_hip_batch_develop_static(const vkdt_develop_image_t *img, int n, void *stream)
+{
+  size_t need = 0;
+  if(int e = vkdt_hip_batch_scratch_bytes(img, n, &need)) return e;
+  if(need > (size_t)VKDT_BATCH_STATIC_BYTES)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch static: the images need %zu bytes of scratch, the static "
+                          "scratch holds %zu", need, (size_t)VKDT_BATCH_STATIC_BYTES);
+  void *base = nullptr;
+  if(int e = static_scratch(&base)) return e;
+  vkdt_batch_t *b = nullptr;
+  if(int e = vkdt_hip_batch_create(&b, img, n, base, VKDT_BATCH_STATIC_BYTES, nullptr)) return e;
+  int e = vkdt_hip_batch_run(b, stream, 0);
+  // the tables go up from the batch's host memory: keep it until the stream has drained
+  const int es = hip_fail(hipStreamSynchronize((hipStream_t)stream), "batch static: sync");
+  vkdt_hip_batch_destroy(b);
+  return e ? e : es;
 }
