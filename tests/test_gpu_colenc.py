@@ -46,7 +46,7 @@ def test_node_vs_oracle32(hip, fmt_in):
 def decode(v, trc):
     """inverse of the sRGB (IEC 61966-2-1) and PQ (SMPTE ST 2084) encodings"""
     if trc == 2:
-        return np.where(v > 0.04045, ((v + 0.055) / 1.055) ** 2.4, v / 12.92)
+        return np.where(v > 0.04045, ((np.maximum(v, 0.04045) + 0.055) / 1.055) ** 2.4, v / 12.92)
     m1, m2 = 2610 / 16384, 2523 / 4096 * 128
     c1, c2, c3 = 3424 / 4096, 2413 / 4096 * 32, 2392 / 4096 * 32
     e = np.clip(v, 0, None) ** (1 / m2)

@@ -5,9 +5,9 @@
  * method 0 (default) builds down -> gauss -> splat -> fix (:158-202);
  * half-size (method 2, or an output scaled down by 1.5 blocks or more)
  * builds halfsize [-> shared/resample] (:92-110); a scaled output of the
- * full-resolution paths ends in shared/resample.  RCD (method 1) builds the
- * reference's node names; its kernels are not part of this path, so running
- * it fails at launch with "no HIP kernel".
+ * full-resolution paths ends in shared/resample.  RCD (method 1, Bayer)
+ * builds rcd_conv -> rcd_fill with the reference's 64x32 fill tiles
+ * (k_rcd.hip).
  */
 #include "pipe/api.h"
 
@@ -39,6 +39,9 @@ void modify_roi_out(dt_graph_t *graph, dt_module_t *module)
   {
     ro->full_wd = (ri->full_wd + 1) / 2;
     ro->full_ht = (ri->full_ht + 1) / 2;
+    // a request for block size or less: the consumer's roi wins the
+    // negotiation, and the half-size image is resampled to it
+    if(scale >= block) ro->marker = s_roi_mark_soft_fwd;
   }
   else
   {

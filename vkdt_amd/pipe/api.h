@@ -78,6 +78,18 @@ typedef struct dt_roi_t
 }
 dt_roi_t;
 
+// non-zero if the marker of a constrains more than the marker of b: hard
+// over soft over uninited, forward over backward at equal strength, and a
+// don't-care marker never (connector.h:32-39)
+static inline int dt_roi_stronger(const dt_roi_t *a, const dt_roi_t *b)
+{
+  if(a->marker == s_roi_mark_dontcare) return 0;
+  if(b->marker == s_roi_mark_dontcare) return 1;
+  const int strength = (int)(a->marker & 3u) - (int)(b->marker & 3u);
+  if(strength) return strength > 0;
+  return (a->marker & 12u) > (b->marker & 12u);
+}
+
 /* ---- connectors (connector.h) ------------------------------------------ */
 typedef struct dt_cid_t { int i, c; } dt_cid_t;
 static const dt_cid_t s_cid_unset = { -1, -1 };
@@ -380,6 +392,16 @@ const float   *dt_module_param_float(const dt_module_t *m, int parid);
 const int32_t *dt_module_param_int(const dt_module_t *m, int parid);
 const char    *dt_module_param_string(const dt_module_t *m, int parid);
 int  dt_connected(const dt_connector_t *c);
+
+/* ---- device capabilities modules consult (qvk/qvk.h) ------------------ */
+// Modules pick kernels by `qvk.float_atomics_supported` (colour, rawhist):
+// gfx950 adds floats atomically in global memory, so it is always 1 here.
+typedef struct dt_qvk_caps_t
+{
+  int float_atomics_supported;
+}
+dt_qvk_caps_t;
+extern const dt_qvk_caps_t qvk;
 
 #define CONN(A) do { int err = (A); if(err) \
   fprintf(stderr, "[create_nodes] %s:%d connection failed: %d\n", __FILE__, __LINE__, err); } while(0)

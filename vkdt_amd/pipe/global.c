@@ -140,6 +140,8 @@ static int load_so(dt_module_so_t *so, const char *dirname)
   return 0;
 }
 
+const dt_qvk_caps_t qvk = { .float_atomics_supported = 1 };
+
 int dt_pipe_global_init(const char *basedir)
 {
   if(g_pipe.num) dt_pipe_global_cleanup();

@@ -716,6 +716,11 @@ static void roi_in(dt_graph_t *g, dt_module_t *m)
       for(int i = 0; i < m->num_connectors; i++)
         if(dt_connector_input(m->connector + i)) m->connector[i].roi = m->connector[output].roi;
   }
+  // hand the request to the connected outputs (graph-run-modules.h:491-541).
+  // The request's extent always travels upstream (a scaled export scales
+  // every module on its path); its marker replaces the output's where it
+  // constrains more, so a module that keys nodes on a hard marker of its
+  // output (demosaic's resample, demosaic/main.c:145,193) sees the request.
   for(int i = 0; i < m->num_connectors; i++)
   {
     dt_connector_t *c = m->connector + i;
@@ -725,12 +730,14 @@ static void roi_in(dt_graph_t *g, dt_module_t *m)
     { // sources give what they have
       up->roi.wd = up->roi.full_wd;
       up->roi.ht = up->roi.full_ht;
+      if(up->roi.marker == s_roi_mark_uninited) up->roi.marker = s_roi_mark_hard_bck;
       c->roi = up->roi;
     }
     else if(c->roi.wd > 0)
     {
       up->roi.wd = c->roi.wd;
       up->roi.ht = c->roi.ht;
+      if(dt_roi_stronger(&c->roi, &up->roi)) up->roi.marker = c->roi.marker;
     }
     c->array_length = up->array_length;
   }
