@@ -137,7 +137,8 @@ def calibrate_valu(iters=4096, blocks=8192):
     out = hip.DeviceBuffer(blocks * 256, 1, 1, hip.FMT_F32)
     waves = blocks * 256 // 64
     res = {}
-    for op, name in enumerate(["v_fma_f32", "v_mul_f32", "v_exp_f32", "v_rcp_f32", "v_sqrt_f32"]):
+    for op, name in enumerate(["v_fma_f32", "v_mul_f32", "v_exp_f32", "v_rcp_f32", "v_sqrt_f32",
+                               "v_fma_f32+v_exp_f32", "v_fma_f32+v_add_u32"]):
         hip.check(L.vkdt_calib_run(op, C.c_void_p(out.ptr), blocks, iters, None), name)
         hip.sync()
         e0, e1 = hip.Event(), hip.Event()
