@@ -335,12 +335,31 @@ static inline int xtrans_colour(int x, int y)
  * denoise (src/pipe/modules/denoise)
  * --------------------------------------------------------------------- */
 
-// noop.comp:36-57 (gainmap path not restated: the DNGs carry no OpcodeList2)
+// noop.comp:45-55, doub.comp:106-114: the gain of CFA site (x,y) at image
+// position (px,py): texture() of the rgba gain grid at
+// clamp(p * map_os.zw - map_os.xy, 0, 1), component (x&1) + 2(y&1)
+static real dn_gain(const or_buf_t *gm, const float *map_os, real px, real py, int x, int y)
+{
+  px = FMIN(FMAX(px * R(map_os[2]) - R(map_os[0]), R(0)), R(1));
+  py = FMIN(FMAX(py * R(map_os[3]) - R(map_os[1]), R(0)), R(1));
+  const v4 g = bilin(gm, px * R(gm->wd), py * R(gm->ht), 0);
+  const int site = (x & 1) + 2 * (y & 1);
+  return site == 0 ? g.x : (site == 1 ? g.y : (site == 2 ? g.z : g.w));
+}
+
+// the gain map applies when the push constant and the module parameter
+// `gainmap` (uniform int word 8) are both 1 on a Bayer mosaic
+static int dn_gainmap_on(int push, const void *params, uint32_t filters)
+{
+  return push == 1 && params && ((const int32_t *)params)[8] == 1 && filters != 0 && filters != 9;
+}
+
+// noop.comp:36-57
 static int dn_noop(const int32_t *pci, const float *pcf, const void *params,
                    const or_buf_t *c, int wd, int ht)
 {
-  (void)params;
-  const or_buf_t *in = c + 0, *out = c + 1;
+  const or_buf_t *in = c + 0, *out = c + 1, *gm = c + 2;
+  const int gain = dn_gainmap_on(pci[17], params, (uint32_t)pci[16]);
   PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
   {
@@ -348,6 +367,12 @@ static int dn_noop(const int32_t *pci, const float *pcf, const void *params,
     real r = FMAX(R(0), (col.x - R(pcf[4])) / (R(pcf[8]) - R(pcf[4])));
     real g = FMAX(R(0), (col.y - R(pcf[5])) / (R(pcf[9]) - R(pcf[5])));
     real b = FMAX(R(0), (col.z - R(pcf[6])) / (R(pcf[10]) - R(pcf[6])));
+    if(gain)
+    { // the texel centre in the uncropped input
+      const real px = (R(0.5) + R(x) + R(pci[0])) / R(in->wd), py = (R(0.5) + R(y) + R(pci[1])) / R(in->ht);
+      const real k = dn_gain(gm, pcf + 12, px, py, x, y);
+      r *= k; g *= k; b *= k;
+    }
     v4 o = { r, g, b, R(1) };
     store(out, x, y, 0, o);
   }
@@ -699,11 +724,12 @@ static int dn_assemble(const int32_t *pci, const float *pcf, const float *par, c
   return 0;
 }
 
-// doub.comp:35-116 (gainmap path not restated)
+// doub.comp:35-116
 static int dn_doub(const int32_t *pci, const float *pcf, const float *par, const or_buf_t *c)
 {
-  const or_buf_t *in = c + 0, *crs0 = c + 1, *crs1b = c + 2, *out = c + 3;
+  const or_buf_t *in = c + 0, *crs0 = c + 1, *crs1b = c + 2, *out = c + 3, *gm = c + 4;
   const uint32_t filters = (uint32_t)pci[16];
+  const int gain = dn_gainmap_on(pci[19], par, filters);
   const real strength = R(par[0]), luma = R(par[1]), detail = R(par[2]);
   PAR_ROWS
   for(int y = 0; y < out->ht; y++) for(int x = 0; x < out->wd; x++)
@@ -750,6 +776,12 @@ static int dn_doub(const int32_t *pci, const float *pcf, const float *par, const
       val = mixr(val, FMAX(R(0), crs + sig[1] * signr(wav) * mixr(FMAX(R(0.0), aw - T), aw, tt)), uw * blendw);
     }
     val = FMAX(R(0), (val - black) / (white - black));
+    if(gain)
+    { // the block's centre on the half-resolution grid of the output
+      const real px = (R(0.5) + R((x + pci[12]) / 2)) / R(out->wd / 2);
+      const real py = (R(0.5) + R((y + pci[13]) / 2)) / R(out->ht / 2);
+      val *= dn_gain(gm, pcf + 20, px, py, x, y);
+    }
     v4 ov = { val, val, val, R(1) };
     store(out, x, y, 0, ov);
   }
@@ -2082,12 +2114,14 @@ int or_node(const char *module, const char *kernel, const void *pc, int pc_size,
   const float *pcf = (const float *)pc;
   const float *par = (const float *)params;
 #define IS(m, k) (!strcmp(module, m) && !strcmp(kernel, k))
-  if(IS("denoise", "noop"))     return nconn >= 2 ? dn_noop(pci, pcf, params, conn, wd, ht) : -1;
+  if(IS("denoise", "noop"))
+    return nconn >= (dn_gainmap_on(pci[17], params, (uint32_t)pci[16]) ? 3 : 2) ? dn_noop(pci, pcf, params, conn, wd, ht) : -1;
   if(IS("denoise", "half"))     return nconn >= 2 ? dn_half(pci, pcf, conn) : -1;
   if(IS("denoise", "downcov"))  return nconn >= 2 ? dn_downcov(pci, pcf, conn, nconn) : -1;
   if(IS("denoise", "down"))     return nconn >= 2 ? dn_down(pci, pcf, par, conn) : -1;
   if(IS("denoise", "assemble")) return nconn >= 6 ? dn_assemble(pci, pcf, par, conn) : -1;
-  if(IS("denoise", "doub"))     return nconn >= 4 ? dn_doub(pci, pcf, par, conn) : -1;
+  if(IS("denoise", "doub"))
+    return nconn >= (dn_gainmap_on(pci[19], par, (uint32_t)pci[16]) ? 5 : 4) ? dn_doub(pci, pcf, par, conn) : -1;
   if(IS("demosaic", "down"))    return nconn >= 2 ? dm_down(pci, conn) : -1;
   if(IS("demosaic", "gauss"))   return nconn >= 3 ? dm_gauss(pci, conn) : -1;
   if(IS("demosaic", "splat"))   return nconn >= 3 ? dm_splat(pci, conn) : -1;
@@ -2208,6 +2242,18 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
   for(int l = 0; l < 4; l++) s[l] = alloc_img(Qw, Qh, 4, 1);
   double *dn = alloc_img(W, H, 1, 1);
   int err = 0;
+  if(p->dn_strength <= 0.0f)
+  { // denoise/main.c:202-225: crop and black/white scaling only
+    int32_t pc[18];
+    float *f = (float *)pc;
+    pc[0] = 0; pc[1] = 0; pc[2] = W; pc[3] = H;
+    for(int k = 0; k < 4; k++) { f[4 + k] = black[k]; f[8 + k] = white[k]; pc[12 + k] = 0; }
+    pc[16] = p->filters; pc[17] = 0;
+    or_buf_t c[2] = { raw, { dn, W, H, 1, OR_F16, 1 } };
+    err |= or_node("denoise", "noop", pc, sizeof(pc), dpar, sizeof(dpar), c, 2, W, H, 1);
+  }
+  else
+  {
   {
     int32_t pch[17];
     float *f = (float *)pch;
@@ -2251,6 +2297,7 @@ int or_develop(const uint16_t *cfa, int W, int H, const or_develop_t *p, double 
     f[20] = f[21] = f[22] = f[23] = 0.0f;
     or_buf_t c[4] = { raw, { asm_, Qw, Qh, 4, OR_F16, 1 }, { half, Qw, Qh, 4, OR_F16, 1 }, { dn, W, H, 1, OR_F16, 1 } };
     err |= or_node("denoise", "doub", pc, sizeof(pc), dpar, sizeof(dpar), c, 4, W, H, 1);
+  }
   }
   free(half); free(asm_);
   for(int l = 0; l < 4; l++) free(s[l]);

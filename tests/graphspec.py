@@ -89,6 +89,9 @@ class DevelopParams:
     hl_desat: float = 0.3
     hl_soft: float = 0.6
     cam: np.ndarray = field(default_factory=lambda: cam_to_rec2020(synth.xyz_to_cam()))
+    # DNG gain maps (Bayer): dict(gains=(map_ht, map_wd, 4) float32 with one
+    # channel per site of the 2x2 block, origin=(v, h), spacing=(v, h))
+    gain_map: dict = None
 
     @property
     def filters(self):
@@ -166,6 +169,15 @@ def export_extent(W, H, max_wd=0, max_ht=0):
     return int(f32(W) / s + f32(0.5)), int(f32(H) / s + f32(0.5))
 
 
+def gain_map_os(gm):
+    """the map origin and inverse extent the denoise module pushes
+    (denoise/main.c:187-196): origin as float, 1/(spacing*(points-1)) in
+    double rounded to float"""
+    mh, mw = gm["gains"].shape[:2]
+    (ov, oh), (sv, sh) = gm["origin"], gm["spacing"]
+    return [np.float32(oh), np.float32(ov), np.float32(1.0 / (sh * (mw - 1))), np.float32(1.0 / (sv * (mh - 1)))]
+
+
 def build(W, H, p, max_wd=0, max_ht=0, prim=2, trc=0):
     """(buffers, nodes): buffers name -> (wd, ht, chan, fmt, planes);
     nodes are dicts with module, kernel, pc, params, conn (buffer names), wd, ht, dp.
@@ -182,14 +194,28 @@ def build(W, H, p, max_wd=0, max_ht=0, prim=2, trc=0):
     def buf(name, wd, ht, chan, fmt, planes=1):
         bufs[name] = (wd, ht, chan, fmt, planes)
 
-    def add(module, kernel, pc, params, conn, wd, ht, dp=1):
-        nodes.append(dict(module=module, kernel=kernel, pc=pc, params=params, conn=conn, wd=wd, ht=ht, dp=dp))
+    def add(module, kernel, pc, params, conn, wd, ht, dp=1, out=None):
+        # out: the index of the written connector (the last one unless an
+        # input binding follows it, as the gain map of noop and doub)
+        nodes.append(dict(module=module, kernel=kernel, pc=pc, params=params, conn=conn, wd=wd, ht=ht, dp=dp,
+                          out=len(conn) - 1 if out is None else out))
 
     buf("raw", W, H, 1, UI16)
     dpar = mixed_blob([("f", p.strength), ("f", p.luma), ("f", p.detail), ("f", 0.0)]
                       + [("f", e) for e in p.edges] + [("i", 1)])
     rgbc = lambda crop: [("f", v) for v in wb] + [("f", v) for v in black] + [("f", v) for v in white] + \
         [("i", v) for v in crop]
+    gm = p.gain_map
+    gm_os = [("f", v) for v in gain_map_os(gm)] if gm is not None else [("i", 0)] * 4
+    if gm is not None:   # the gain-map source (denoise/main.c:182-188): rgba f32 on the map grid
+        buf("gm", gm["gains"].shape[1], gm["gains"].shape[0], 4, F32)
+    gmc = ["gm"] if gm is not None else []
+    if p.strength <= 0:   # denoise/main.c:202-225: one noop node, crop + black/white scaling
+        buf("dn", W, H, 1, F16)
+        pc = mixed_blob([("i", v) for v in (0, 0, W, H)] + [("f", v) for v in black] + [("f", v) for v in white]
+                        + gm_os + [("i", p.filters), ("i", int(gm is not None))])
+        add("denoise", "noop", pc, dpar, ["raw", "dn"] + gmc, W, H, out=1)
+        return _build_tail(W, H, p, bufs, nodes, buf, add, wb, max_wd, max_ht, prim, trc)
     buf("half", Qw, Qh, 4, F16)
     add("denoise", "half", mixed_blob(rgbc([0, 0, W, H]) + [("i", p.filters)]), dpar, ["raw", "half"], Qw, Qh)
     for l in range(4):
@@ -200,10 +226,17 @@ def build(W, H, p, max_wd=0, max_ht=0, prim=2, trc=0):
     pc = mixed_blob(rgbc([0, 0, 0, 0]) + [("f", p.noise_a), ("f", p.noise_b), ("i", p.filters)])
     add("denoise", "assemble", pc, dpar, ["half", "s0", "s1", "s2", "s3", "asm"], Qw, Qh)
     buf("dn", W, H, 1, F16)
-    pc = mixed_blob(rgbc([0, 0, W, H]) + [("i", p.filters), ("f", p.noise_a), ("f", p.noise_b), ("i", 0)]
-                    + [("f", 0.0)] * 4)
-    add("denoise", "doub", pc, dpar, ["raw", "asm", "half", "dn"], W, H)
+    pc = mixed_blob(rgbc([0, 0, W, H]) + [("i", p.filters), ("f", p.noise_a), ("f", p.noise_b),
+                                           ("i", int(gm is not None))] + gm_os)
+    add("denoise", "doub", pc, dpar, ["raw", "asm", "half", "dn"] + gmc, W, H, out=3)
+    return _build_tail(W, H, p, bufs, nodes, buf, add, wb, max_wd, max_ht, prim, trc)
 
+
+def _build_tail(W, H, p, bufs, nodes, buf, add, wb, max_wd, max_ht, prim, trc):
+    """the nodes after denoise (its output in buffer "dn")"""
+    xt = p.kind == "xtrans"
+    block = 3 if xt else 2
+    Qw, Qh = W // block, H // block
     mos = "dn"
     if p.hilite:
         mos = "hl"

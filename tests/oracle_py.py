@@ -165,14 +165,17 @@ def nprof_fit(bits, acc):
     return a.value, b.value, n
 
 
-def run_nodes(bits, bufs, nodes, raw, storage_f16=True):
+def run_nodes(bits, bufs, nodes, raw, storage_f16=True, sources=None):
     """a restated node list (graphspec) on the oracle from the ui16 mosaic
-    `raw`; returns every buffer (float64, uint16 for ui16 buffers)"""
+    `raw` (and further source buffers, name -> array); returns every buffer
+    (float64, uint16 for ui16 buffers)"""
     data = {}
     for name, (wd, ht, chan, fmt, planes) in bufs.items():
         shape = ((planes,) if planes > 1 else ()) + (ht, wd) + ((chan,) if chan > 1 else ())
         data[name] = np.zeros(shape, np.uint16 if fmt == OR_UI16 else np.float64)
     data["raw"][...] = raw
+    for name, arr in (sources or {}).items():
+        data[name][...] = arr
     for n in nodes:
         ob = [orbuf(data[c], bufs[c][3], bufs[c][2], bufs[c][4]) for c in n["conn"]]
         node(bits, n["module"], n["kernel"], n["pc"], n["params"], ob, n["wd"], n["ht"], n["dp"],

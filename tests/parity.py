@@ -28,11 +28,14 @@ NODE_TIE_EPS = 1e-5      # identical inputs: the two sides differ in their own r
 TWIN_SEEDS = (1, 2)
 
 
-def run_gpu_graph(hip, bufs, nodes, raw):
+def run_gpu_graph(hip, bufs, nodes, raw, sources=None):
     """every node on the GPU through the C ABI, in graph order; returns the
-    host copy of every buffer"""
+    host copy of every buffer.  `sources` maps further source buffers (the
+    gain map) to their host arrays."""
     dev = {k: hip.DeviceBuffer(*v) for k, v in bufs.items()}
     dev["raw"].upload(raw)
+    for k, v in (sources or {}).items():
+        dev[k].upload(v)
     for n in nodes:
         hip.launch(n["module"], n["kernel"], n["pc"], n["params"], [dev[c] for c in n["conn"]],
                    n["wd"], n["ht"], n["dp"])
@@ -91,15 +94,17 @@ def check_nodes(bufs, nodes, host, log=print, max_ulp=MAX_NODE_ULP):
     [(index, module/kernel, max ulp outside near ties, near-tie texels)]"""
     report = []
     for i, n in enumerate(nodes):
-        ins = [to_or(host[c], bufs[c])[1] for c in n["conn"][:-1]]
-        outname = n["conn"][-1]
+        oi = n.get("out", len(n["conn"]) - 1)
+        ins = [to_or(host[c], bufs[c])[1] for c in n["conn"]]
+        outname = n["conn"][oi]
         spec = bufs[outname]
         out = np.zeros(host[outname].shape, np.float64)
         fmt = op.OR_F16 if spec[3] == gs.F16 else (op.OR_UI16 if spec[3] == gs.UI16 else op.OR_F32)
         if fmt == op.OR_UI16:
             out = np.zeros(host[outname].shape, np.uint16)
         ties = np.zeros((spec[1], spec[0]), np.uint8)
-        op.node(32, n["module"], n["kernel"], n["pc"], n["params"], ins + [op.orbuf(out, fmt, spec[2], spec[4])],
+        ins[oi] = op.orbuf(out, fmt, spec[2], spec[4])
+        op.node(32, n["module"], n["kernel"], n["pc"], n["params"], ins,
                 n["wd"], n["ht"], n["dp"], ties=ties, tie_eps=NODE_TIE_EPS)
         d = node_distance(n, host[outname], out, spec)
         flagged = ties.astype(bool)

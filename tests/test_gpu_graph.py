@@ -15,17 +15,18 @@ from vkdt_amd import pfm, pipe, synth
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("bayer", 320, 240, 21), ("xtrans", 330, 246, 22)]
+# (kind, W, H, seed, denoise strength): strength 0 is the param's default (one noop node)
+CASES = [("bayer", 320, 240, 21, 0.5), ("xtrans", 330, 246, 22, 0.5), ("bayer", 320, 240, 24, 0.0)]
 
 
-@pytest.mark.parametrize("kind,W,H,seed", CASES, ids=[c[0] for c in CASES])
-def test_graph_vs_oracle(tmp_path, hip, kind, W, H, seed):
+@pytest.mark.parametrize("kind,W,H,seed,strength", CASES, ids=[f"{c[0]}-{c[4]}" for c in CASES])
+def test_graph_vs_oracle(tmp_path, hip, kind, W, H, seed, strength):
     cfa, _ = synth.mosaic(W, H, seed, kind)
     dng = str(tmp_path / "in.dng")
     synth.write_dng(dng, cfa, kind)
     na, nb = synth.noise_params(2e-4, 1e-6)
-    print(f"[gpu_graph] {kind} {W}x{H} seed {seed}", flush=True)
-    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb))
+    print(f"[gpu_graph] {kind} {W}x{H} seed {seed} strength {strength}", flush=True)
+    g = pipe.Graph(lines=pipe.raw_graph_cfg(dng, noise_a=na, noise_b=nb, strength=strength))
     out = str(tmp_path / "out")
     g.export("o-pfm", out)
     g.run()
@@ -34,7 +35,8 @@ def test_graph_vs_oracle(tmp_path, hip, kind, W, H, seed):
     assert img.shape == (H - H % (3 if kind == "xtrans" else 2), W - W % (3 if kind == "xtrans" else 2), 3)
     ip = g.img_param("i-raw", "main")
     p = gs.DevelopParams(kind=kind, noise_a=np.float32(na), noise_b=np.float32(nb),
-                         cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,))
+                         cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,),
+                         strength=strength)
     parity.e2e_check(cfa[:img.shape[0], :img.shape[1]], p, img, log=lambda s: print(s, flush=True), label=f" {kind}")
     g.close()
 

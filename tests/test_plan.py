@@ -171,3 +171,23 @@ def test_noise_profile_graph_plan(tmp_path):
     col = [n for n in g.nodes() if n["module"] == "rawhist" and n["kernel"] == "collect"][0]
     par = np.frombuffer(col["params"], np.uint32)
     assert par[2] == synth.BAYER_FILTERS and par[3] == 512 and par[4] == 16383
+
+
+@pytest.mark.parametrize("kind,W,H", [("bayer", 200, 150), ("xtrans", 201, 150)])
+def test_strength_zero_is_one_noop(tmp_path, kind, W, H):
+    """denoise strength 0 (the param's default) plans one noop node
+    (denoise/main.c:202-225) writing the one-channel f16 mosaic demosaic reads"""
+    g, na, nb = planned(tmp_path, kind, W, H, extra=["param:denoise:01:strength:0"])
+    ip = g.img_param("i-raw", "main")
+    W, H = ip["full"]
+    p = gs.DevelopParams(kind=kind, noise_a=na, noise_b=nb, cam=np.array(ip["cam"], np.float32).reshape(3, 3),
+                         wb=tuple(ip["wb"][:3]) + (ip["wb"][3],), strength=0.0)
+    _, ref = gs.build(W, H, p)
+    nodes = {n["id"]: n for n in g.nodes()}
+    live = [nodes[k] for k in g.exec_nodes() if nodes[k]["module"] not in ("i-raw", "o-pfm")]
+    assert [(n["module"], n["kernel"]) for n in live] == [(r["module"], r["kernel"]) for r in ref]
+    noop = live[0]
+    assert (noop["module"], noop["kernel"]) == ("denoise", "noop")
+    assert noop["pc"][:72] == np.ascontiguousarray(ref[0]["pc"]).tobytes()
+    out = noop["conns"][1]["buf"]
+    assert (out.chan, out.format, out.wd, out.ht) == (1, gs.F16, W, H)

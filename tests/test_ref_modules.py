@@ -17,6 +17,7 @@ than on this repository's restatement of it.  Only the host side is compared
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from vkdt_amd import pipe, synth
@@ -28,7 +29,7 @@ PKG = os.path.join(ROOT, "vkdt_amd")
 # the modules of the raw-development graph (and of its exports and noise
 # profile) whose reference sources build against this API; i-raw needs the
 # reference's rawloader (rawler), nprof its core/ log and file helpers
-MODULES = ["demosaic", "hilite", "colour", "filmcurv", "llap", "o-pfm", "colenc", "rawhist"]
+MODULES = ["denoise", "demosaic", "hilite", "colour", "filmcurv", "llap", "o-pfm", "colenc", "rawhist"]
 
 pytestmark = pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "pipe", "modules")),
                                 reason="reference tree absent")
@@ -128,6 +129,15 @@ def plan_cases(tmp_path):
                                        extra=extra + ([f"param:demosaic:01:method:{method}"] if method else []))
             cases[f"{kind}-h{int(hil)}-m{method}-w{mw}-p{prim}t{trc}-{len(extra)}"] = (lines, mw, prim, trc)
         cases[f"{kind}-noise-profile"] = (pipe.noise_profile_cfg(path), None, 2, 0)
+        # a DNG with four GainMap opcodes in OpcodeList2: denoise's gain-map source
+        rng = np.random.default_rng(2)
+        maps = [rng.uniform(0.8, 1.2, size=(4, 5)).astype(np.float32) for _ in range(4)]
+        gpath = str(tmp_path / f"{kind}-gm.dng")
+        lst = synth.opcode_list(synth.bayer_gain_maps(W, H, maps, spacing=(1 / 3.0, 0.25), origin=(0.0, 0.0)))
+        synth.write_dng(gpath, cfa, kind, extra_tags=((51009, 7, list(lst)),))
+        for hil in (False, True):
+            cases[f"{kind}-gainmap-h{int(hil)}"] = (pipe.raw_graph_cfg(gpath, noise_a=na, noise_b=nb, hilite=hil),
+                                                  0, 2, 0)
     return cases
 
 
@@ -159,6 +169,8 @@ def test_reference_module_sources_plan_the_same_graphs(tmp_path):
     for key in cases:
         a, b = ours[key], drop_identity_resample(theirs[key])
         assert drop_identity_resample(a) == a, key
+        if "gainmap" in key:
+            assert any(n[:2] == ("denoise", "gainmap") for n in a), key
         assert [n[:2] for n in a] == [n[:2] for n in b], key
         for na, nb in zip(a, b):
             what = f"{key}: {na[0]}/{na[1]}"

@@ -182,6 +182,30 @@ DEV float4 bilin_h4(const dimg &b, float s, float t)
   return r;
 }
 
+DEV float4 ld_f4(const dimg &b, int x, int y)
+{ // one 16-byte load of an rgba f32 texel
+  return *(const float4 *)((const float *)b.p + 4 * ((long)y * b.pitch + x));
+}
+
+// texture() of an rgba f32 image at texture-space (s,t), as bilin_h4
+DEV float4 bilin_f4(const dimg &b, float s, float t)
+{
+  const float fs = s - 0.5f, ft = t - 0.5f;
+  const float fls = floorf(fs), flt = floorf(ft);
+  const int i0 = (int)fls, j0 = (int)flt;
+  const float a = fs - fls, c = ft - flt;
+  const int x0 = mirror(i0, b.wd), x1 = mirror(i0 + 1, b.wd);
+  const int y0 = mirror(j0, b.ht), y1 = mirror(j0 + 1, b.ht);
+  const float4 p00 = ld_f4(b, x0, y0), p10 = ld_f4(b, x1, y0);
+  const float4 p01 = ld_f4(b, x0, y1), p11 = ld_f4(b, x1, y1);
+  float4 r;
+  r.x = (p00.x * (1.f - a) + p10.x * a) * (1.f - c) + (p01.x * (1.f - a) + p11.x * a) * c;
+  r.y = (p00.y * (1.f - a) + p10.y * a) * (1.f - c) + (p01.y * (1.f - a) + p11.y * a) * c;
+  r.z = (p00.z * (1.f - a) + p10.z * a) * (1.f - c) + (p01.z * (1.f - a) + p11.z * a) * c;
+  r.w = (p00.w * (1.f - a) + p10.w * a) * (1.f - c) + (p01.w * (1.f - a) + p11.w * a) * c;
+  return r;
+}
+
 // bilinear with an integer base texel (x0,y0) and fixed weights (a,c):
 // the exact form of texture() at texel-space (x0 + .5 + a, y0 + .5 + c)
 DEV float4 bilin_fixed_h4(const dimg &b, int x0, int y0, float a, float c)

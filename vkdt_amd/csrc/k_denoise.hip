@@ -34,15 +34,23 @@ static void read_params(dn_args &a, const node_call &c) { dn_params_init(a, (con
 
 // ---- noop (noop.comp:36-57, strength <= 0) -------------------------------
 
-__global__ void __launch_bounds__(256) k_dn_noop(dimg in, dimg out, dn_args a)
+__global__ void __launch_bounds__(256) k_dn_noop(dimg in, dimg out, dimg gm, dn_args a)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
   // ui16 single channel reads as (v, 0, 0)
   const float v = fetch_u16(in, x + a.crop[0], y + a.crop[1]);
-  const float r = fmaxf(0.0f, (v - a.black[0]) / (a.white[0] - a.black[0]));
-  const float g = fmaxf(0.0f, (0.0f - a.black[1]) / (a.white[1] - a.black[1]));
-  const float b = fmaxf(0.0f, (0.0f - a.black[2]) / (a.white[2] - a.black[2]));
+  float r = fmaxf(0.0f, (v - a.black[0]) / (a.white[0] - a.black[0]));
+  float g = fmaxf(0.0f, (0.0f - a.black[1]) / (a.white[1] - a.black[1]));
+  float b = fmaxf(0.0f, (0.0f - a.black[2]) / (a.white[2] - a.black[2]));
+  if(a.gainmap)
+  { // position of the texel centre in the input, the uncropped raw
+    const float px = (0.5f + (float)x + (float)a.crop[0]) / (float)in.wd;
+    const float py = (0.5f + (float)y + (float)a.crop[1]) / (float)in.ht;
+    const float gain = dn_gain_at(gm, a, px, py, x, y);
+    r *= gain; g *= gain; b *= gain;
+  }
+  if(out.chan == 1) { st_h1(out, x, y, r); return; }   // the mosaic's one channel
   st_h4(out, x, y, make_float4(r, g, b, 1.0f));
 }
 
@@ -78,11 +86,32 @@ __global__ void __launch_bounds__(256) k_dn_assemble(dimg s0, dimg s1, dimg s2, 
   st_h4(out, x, y, dn_assemble_px(s0, s1, s2, s3, s4, a, m, x, y));
 }
 
-__global__ void __launch_bounds__(256) k_dn_doub(dimg in, dimg crs0, dimg crs1b, dimg out, dn_args a)
+__global__ void __launch_bounds__(256) k_dn_doub(dimg in, dimg crs0, dimg crs1b, dimg out, dimg gm, dn_args a)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  st_h1(out, x, y, dn_doub_px(in, crs0, crs1b, out.wd, out.ht, a, x, y));
+  float v = dn_doub_px(in, crs0, crs1b, out.wd, out.ht, a, x, y);
+  if(a.gainmap)
+  { // the block's centre on the half-resolution grid of the output
+    const float px = (0.5f + (float)((x + a.crop[0]) / 2)) / (float)(out.wd / 2);
+    const float py = (0.5f + (float)((y + a.crop[1]) / 2)) / (float)(out.ht / 2);
+    v *= dn_gain_at(gm, a, px, py, x, y);
+  }
+  st_h1(out, x, y, v);
+}
+
+// the gain map binding of noop (connector 2) and doub (connector 4): the
+// map applies when the push constant and the module's `gainmap` parameter
+// (uniform word 8) are both 1 on a Bayer mosaic (filters neither 0 nor 9);
+// it is then an rgba f32 grid of at least one texel
+static int read_gainmap(dn_args &a, const node_call &c, int conn, int gflag, const float *map_os, const char *what)
+{
+  const int param = ((const int32_t *)c.params)[8];
+  a.gainmap = gflag == 1 && param == 1 && a.filters != 0 && a.filters != 9;
+  for(int k = 0; k < 4; k++) a.map_os[k] = map_os[k];
+  if(!a.gainmap) return 0;
+  if(c.nconn <= conn) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "%s: gain map binding missing", what);
+  return check_conn(c, conn, VKDT_FMT_F32, 4, 1, what);
 }
 
 // ---- host launchers --------------------------------------------------------
@@ -91,17 +120,17 @@ int launch_denoise_noop(const node_call &c)
 {
   if(int e = check_blob(c, 18 * 4, 9 * 4, "denoise/noop")) return e;
   if(int e = check_conn(c, 0, VKDT_FMT_UI16, 1, 1, "denoise/noop input")) return e;
-  if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "denoise/noop output")) return e;
+  const uint32_t och = c.nconn > 1 && c.conn[1].chan == 1 ? 1 : 4;   // the mosaic channel, or rgba
+  if(int e = check_conn(c, 1, VKDT_FMT_F16, och, 1, "denoise/noop output")) return e;
   const int32_t *pi = (const int32_t *)c.pc;
   const float *pf = (const float *)c.pc;
   dn_args a = {};
   for(int k = 0; k < 4; k++) { a.crop[k] = pi[k]; a.black[k] = pf[4 + k]; a.white[k] = pf[8 + k]; }
   a.filters = (uint32_t)pi[16];
-  a.gainmap = pi[17];
   read_params(a, c);
-  if(a.gainmap) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "denoise/noop: dng gain maps are not supported");
+  if(int e = read_gainmap(a, c, 2, pi[17], pf + 12, "denoise/noop gainmap")) return e;
   hipLaunchKernelGGL(k_dn_noop, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), a.gainmap ? to_dimg(c.conn[2]) : dimg{}, a);
   return launch_post("denoise/noop");
 }
 
@@ -198,10 +227,10 @@ int launch_denoise_doub(const node_call &c)
   a.filters = (uint32_t)pi[16];
   a.noise_a = pf[17];
   a.noise_b = pf[18];
-  a.gainmap = pi[19];
   read_params(a, c);
-  if(a.gainmap) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "denoise/doub: dng gain maps are not supported");
+  if(int e = read_gainmap(a, c, 4, pi[19], pf + 20, "denoise/doub gainmap")) return e;
   hipLaunchKernelGGL(k_dn_doub, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), a);
+                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]),
+                     a.gainmap ? to_dimg(c.conn[4]) : dimg{}, a);
   return launch_post("denoise/doub");
 }

@@ -23,13 +23,27 @@ struct dn_args
   int      level;
   int      block;
   uint32_t filters;
-  int      gainmap;
+  int      gainmap;          // apply the Bayer gain map (push and module parameter both 1)
+  float    map_os[4];        // gain-map origin (x, y) and inverse extent (x, y)
   float    strength, luma, detail, edges[4];
   float    edge_gain[3];     // exp2(12*edges.rgb + edges.a) of noise_sigma
   float    level_scale;      // 0.7^level / block factor (down.comp:85)
 };
 
 struct asm_mats { float rgb_to_yuv[9], yuv_to_rgb[9]; };
+
+// noop.comp:45-55, doub.comp:106-114: the gain of CFA site (x,y) at image
+// position (px,py) in [0,1]^2: the rgba gain grid (one channel per site of
+// the 2x2 block) sampled by texture() at (p * map_os.zw - map_os.xy)
+// clamped to [0,1]
+DEV float dn_gain_at(const dimg &gm, const dn_args &a, float px, float py, int x, int y)
+{
+  px = clampf(px * a.map_os[2] - a.map_os[0], 0.0f, 1.0f);
+  py = clampf(py * a.map_os[3] - a.map_os[1], 0.0f, 1.0f);
+  const float4 g = bilin_f4(gm, px * (float)gm.wd, py * (float)gm.ht);
+  const int site = (x & 1) + 2 * (y & 1);
+  return site == 0 ? g.x : (site == 1 ? g.y : (site == 2 ? g.z : g.w));
+}
 
 // host: the module parameters strength, luma, detail, edges[4]
 // (denoise/params, float words 0..2 and 4..7)
@@ -142,6 +156,54 @@ DEV void load_tile4(float4 *tile, const dimg &in, int ox, int oy)
 template<int R>
 struct dn_cov_smem { float4 tile[DN_TN(R)]; float lum[DN_TN(R)], rlum[DN_TN(R)], rlum2[DN_TN(R)], lum25[DN_TN(R)]; };
 
+// The structure tensors of the 5x5 window at tile index c0 (offset (-2,-2)),
+// the choice between them and its eigen decomposition; mean_b is the
+// window's mean luminance.
+template<int R>
+DEV void dn_cov_tensor(const dn_cov_smem<R> &s, int c0, float ev[2], float e0[2], float e1[2], float &mean_b)
+{
+  float mw0 = 0, mw1 = 0, mb0 = 0, mb1 = 0, smw = 0, smb = 0;
+#pragma unroll
+  for(int j = 0; j < 5; j++)
+#pragma unroll
+    for(int i = 0; i < 5; i++)
+    {
+      const float px = s.lum[c0 + j * TW + i], rp = s.rlum[c0 + j * TW + i];
+      mw0 += (float)(i - 2) * px; mw1 += (float)(j - 2) * px;
+      smw += px;
+      mb0 += (float)(i - 2) * rp; mb1 += (float)(j - 2) * rp;
+      smb += rp;
+    }
+  mw0 /= smw; mw1 /= smw;
+  mb0 /= smb; mb1 /= smb;
+  float Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0;
+  mean_b = 0;
+#pragma unroll
+  for(int j = 0; j < 5; j++)
+#pragma unroll
+    for(int i = 0; i < 5; i++)
+    {
+      const int t = c0 + j * TW + i;
+      const float px = s.lum[t];
+      mean_b += s.lum25[t];
+      float p2 = px * px;
+      float dx = (float)(i - 2) - mw0, dy = (float)(j - 2) - mw1;
+      Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
+      sw += p2;
+      dx = (float)(i - 2) - mb0; dy = (float)(j - 2) - mb1;
+      p2 = s.rlum2[t];
+      Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
+      sb += p2;
+    }
+#pragma unroll
+  for(int q = 0; q < 4; q++) { Sw[q] /= sw; Sb[q] /= sb; }
+  float S[4];
+  const bool usew = det2(Sw) < det2(Sb);
+#pragma unroll
+  for(int q = 0; q < 4; q++) S[q] = usew ? Sw[q] : Sb[q];
+  evd2x2(ev, e0, e1, S);
+}
+
 template<int R>
 DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const dn_args &a, int X0, int Y0,
                          dn_cov_smem<R> &s)
@@ -166,46 +228,8 @@ DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const
     const int y = Y0 + threadIdx.y + BY * k;
     if(x >= out.wd || y >= out.ht) continue;
     const int c0 = (threadIdx.y + BY * k) * TW + threadIdx.x;   // tile index of offset (-2,-2)
-    float mw0 = 0, mw1 = 0, mb0 = 0, mb1 = 0, smw = 0, smb = 0;
-#pragma unroll
-    for(int j = 0; j < 5; j++)
-#pragma unroll
-      for(int i = 0; i < 5; i++)
-      {
-        const float px = s.lum[c0 + j * TW + i], rp = s.rlum[c0 + j * TW + i];
-        mw0 += (float)(i - 2) * px; mw1 += (float)(j - 2) * px;
-        smw += px;
-        mb0 += (float)(i - 2) * rp; mb1 += (float)(j - 2) * rp;
-        smb += rp;
-      }
-    mw0 /= smw; mw1 /= smw;
-    mb0 /= smb; mb1 /= smb;
-    float Sw[4] = {0, 0, 0, 0}, Sb[4] = {0, 0, 0, 0}, sw = 0, sb = 0, mean_b = 0;
-#pragma unroll
-    for(int j = 0; j < 5; j++)
-#pragma unroll
-      for(int i = 0; i < 5; i++)
-      {
-        const int t = c0 + j * TW + i;
-        const float px = s.lum[t];
-        mean_b += s.lum25[t];
-        float p2 = px * px;
-        float dx = (float)(i - 2) - mw0, dy = (float)(j - 2) - mw1;
-        Sw[0] += p2 * dx * dx; Sw[1] += p2 * dx * dy; Sw[2] += p2 * dy * dx; Sw[3] += p2 * dy * dy;
-        sw += p2;
-        dx = (float)(i - 2) - mb0; dy = (float)(j - 2) - mb1;
-        p2 = s.rlum2[t];
-        Sb[0] += p2 * dx * dx; Sb[1] += p2 * dx * dy; Sb[2] += p2 * dy * dx; Sb[3] += p2 * dy * dy;
-        sb += p2;
-      }
-#pragma unroll
-    for(int q = 0; q < 4; q++) { Sw[q] /= sw; Sb[q] /= sb; }
-    float S[4];
-    const bool usew = det2(Sw) < det2(Sb);
-#pragma unroll
-    for(int q = 0; q < 4; q++) S[q] = usew ? Sw[q] : Sb[q];
-    float ev[2], e0[2], e1[2];
-    evd2x2(ev, e0, e1, S);
+    float ev[2], e0[2], e1[2], mean_b;
+    dn_cov_tensor<R>(s, c0, ev, e0, e1, mean_b);
     ev[1] *= 0.05f;
     ev[0] = clampf(ev[0], 0.01f, 25.0f);
     ev[1] = clampf(ev[1], 0.01f, 25.0f);

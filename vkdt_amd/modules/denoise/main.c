@@ -2,14 +2,86 @@
  * denoise: noise-profile driven decimated wavelet denoising on the CFA.
  *
  * Node graph and push constants as in src/pipe/modules/denoise/main.c:
- * strength <= 0 -> one `noop` node (crop + black/white scaling, :202-225);
+ * strength <= 0 -> one `noop` node (crop + black/white scaling, :202-225),
+ * whose output is the one-channel mosaic the demosaic and hilite kernels
+ * read (the reference declares it rgba and its readers sample .r);
  * otherwise half -> downcov -> down x3 -> assemble on the block grid and
- * doub back at full resolution (:227-326).  DNG gain maps are not carried
- * by this path's raw input, so the gain-map source node is never built.
+ * doub back at full resolution (:227-326).  A Bayer DNG whose OpcodeList2
+ * starts with four GainMap opcodes, one per CFA site (:12-43), adds a
+ * `gainmap` source node (the four maps interleaved as rgba f32, uploaded by
+ * read_source, :62-78) that noop and doub sample; otherwise their gain-map
+ * binding is a dummy and the push constants say so.
  */
 #include "pipe/api.h"
+#include "pipe/dng_opcode.h"
 
 #include <stdlib.h>
+
+typedef struct dn_data_t
+{
+  const dt_dng_gain_map_t *gm[4];   // by CFA site of the emitted buffer: (0,0) (1,0) (0,1) (1,1)
+}
+dn_data_t;
+
+int init(dt_module_t *mod)
+{
+  mod->data = calloc(1, sizeof(dn_data_t));
+  return mod->data ? 0 : 1;
+}
+
+void cleanup(dt_module_t *mod)
+{
+  free(mod->data);
+  mod->data = 0;
+}
+
+// The four GainMap opcodes of a Bayer raw, sorted by the CFA site of the
+// emitted (offset-normalised) buffer they apply to.  As the reference
+// checks it (denoise/main.c:12-43): the list's first four opcodes must all
+// be gain maps over one plane with row and column pitch 2 and one map
+// plane, at least one axis with two or more points, one per site, on the
+// same grid (points, spacing, origin) over the same 2x2 block region.
+// Returns 1 and fills d->gm, else 0.
+static int bayer_gain_maps(const dt_dng_opcode_list_t *ops, uint32_t ox, uint32_t oy, dn_data_t *d)
+{
+  for(int k = 0; k < 4; k++) d->gm[k] = 0;
+  if(!ops || ops->count < 4) return 0;
+  for(int i = 0; i < 4; i++) if(ops->ops[i].id != s_dngop_gain_map || !ops->ops[i].data) return 0;
+  for(int i = 0; i < 4; i++)
+  {
+    const dt_dng_gain_map_t *g = ops->ops[i].data;
+    const dt_dng_region_t *r = &g->region;
+    if(r->plane != 0 || r->planes != 1 || g->map_planes != 1 || r->row_pitch != 2 || r->col_pitch != 2) return 0;
+    if(g->map_points_h < 2 && g->map_points_v < 2) return 0;
+    d->gm[(((r->top + oy) & 1) << 1) + ((r->left + ox) & 1)] = g;
+  }
+  for(int k = 0; k < 4; k++) if(!d->gm[k]) return 0;
+  const dt_dng_gain_map_t *a = d->gm[0];
+  for(int k = 1; k < 4; k++)
+  {
+    const dt_dng_gain_map_t *b = d->gm[k];
+    if(a->map_points_h != b->map_points_h || a->map_points_v != b->map_points_v ||
+       a->map_spacing_h != b->map_spacing_h || a->map_spacing_v != b->map_spacing_v ||
+       a->map_origin_h != b->map_origin_h || a->map_origin_v != b->map_origin_v ||
+       a->region.top / 2 != b->region.top / 2 || a->region.left / 2 != b->region.left / 2 ||
+       a->region.bottom / 2 != b->region.bottom / 2 || a->region.right / 2 != b->region.right / 2)
+      return 0;
+  }
+  return 1;
+}
+
+// the gain-map source: gains of the four sites interleaved as rgba
+int read_source(dt_module_t *mod, void *mapped, dt_read_source_params_t *p)
+{
+  const dn_data_t *d = mod->data;
+  if(p->node->kernel != dt_token("gainmap") || !d || !d->gm[0]) return 0;
+  const uint32_t wd = d->gm[0]->map_points_h, ht = d->gm[0]->map_points_v;
+  float *out = mapped;
+  for(uint32_t j = 0; j < ht; j++)
+    for(uint32_t i = 0; i < wd; i++)
+      for(int c = 0; c < 4; c++) out[4 * ((size_t)j * wd + i) + c] = d->gm[c]->map_gain[(size_t)j * wd + i];
+  return 0;
+}
 
 void modify_roi_in(dt_graph_t *graph, dt_module_t *module)
 {
@@ -74,7 +146,25 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
   for(int k = 0; k < 4; k++) black[k] = ip->black[k] / 65535.0f;
   for(int k = 0; k < 4; k++) white[k] = ip->white[k] / 65535.0f;
   const uint32_t na = fbits(ip->noise_a), nb = fbits(ip->noise_b);
-  const uint32_t gainmap = 0;
+
+  // DNG gain maps (denoise/main.c:172-200): map origin and inverse extent
+  dn_data_t *dat = module->data;
+  const dt_image_metadata_dngop_t *dngop = dt_metadata_find(ip->meta, s_image_metadata_dngop);
+  uint32_t gainmap = dngop && dat ? (uint32_t)bayer_gain_maps(dngop->op_list[1], dngop->ox, dngop->oy, dat) : 0;
+  uint32_t gm_ox = 0, gm_oy = 0, gm_sx = 0, gm_sy = 0;
+  int id_gm = -1;
+  if(gainmap)
+  {
+    const dt_dng_gain_map_t *g = dat->gm[0];
+    dt_roi_t groi = { .wd = g->map_points_h, .ht = g->map_points_v };
+    id_gm = dt_node_add(graph, module, "denoise", "gainmap", g->map_points_h, g->map_points_v, 1, 0, 0, 1,
+                        "source", "source", "rgba", "f32", &groi);
+    gm_ox = fbits((float)g->map_origin_h);
+    gm_oy = fbits((float)g->map_origin_v);
+    gm_sx = fbits((float)(1.0 / (g->map_spacing_h * (g->map_points_h - 1))));
+    gm_sy = fbits((float)(1.0 / (g->map_spacing_v * (g->map_points_v - 1))));
+  }
+  else if(dat) for(int k = 0; k < 4; k++) dat->gm[k] = 0;
 
   const float strength = dt_module_param_float(module, dt_module_get_param(module->so, dt_token("strength")))[0];
   if(strength <= 0.0f)
@@ -84,14 +174,15 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
       (int32_t)crop[0], (int32_t)crop[1], (int32_t)crop[2], (int32_t)crop[3],
       (int32_t)fbits(black[0]), (int32_t)fbits(black[1]), (int32_t)fbits(black[2]), (int32_t)fbits(black[3]),
       (int32_t)fbits(white[0]), (int32_t)fbits(white[1]), (int32_t)fbits(white[2]), (int32_t)fbits(white[3]),
-      0, 0, 0, 0, (int32_t)ip->filters, (int32_t)gainmap };
+      (int32_t)gm_ox, (int32_t)gm_oy, (int32_t)gm_sx, (int32_t)gm_sy, (int32_t)ip->filters, (int32_t)gainmap };
     const int id = dt_node_add(graph, module, "denoise", "noop", module->connector[1].roi.wd,
                                module->connector[1].roi.ht, 1, sizeof(pc), pc, 3,
                                "input", "read", "rgba", "f16", dt_no_roi,
-                               "output", "write", "rgba", "f16", &module->connector[1].roi,
+                               "output", "write", "rggb", "f16", &module->connector[1].roi,
                                "gainmap", "read", "rgba", "*", dt_no_roi);
     dt_connector_copy(graph, module, 0, id, 0);
-    dt_connector_copy(graph, module, 0, id, 2);
+    if(gainmap) CONN(dt_node_connect(graph, id_gm, 0, id, 2));
+    else dt_connector_copy(graph, module, 0, id, 2);
     dt_connector_copy(graph, module, 1, id, 1);
     return;
   }
@@ -162,7 +253,10 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
     pc[17] = (int32_t)na;
     pc[18] = (int32_t)nb;
     pc[19] = (int32_t)gainmap;
-    pc[20] = pc[21] = pc[22] = pc[23] = 0;
+    pc[20] = (int32_t)gm_ox;
+    pc[21] = (int32_t)gm_oy;
+    pc[22] = (int32_t)gm_sx;
+    pc[23] = (int32_t)gm_sy;
     const int id_doub = dt_node_add(graph, module, "denoise", "doub", module->connector[1].roi.wd,
                                     module->connector[1].roi.ht, 1, sizeof(pc), pc, 5,
                                     "orig", "read", "rggb", "f16", dt_no_roi,
@@ -172,7 +266,8 @@ void create_nodes(dt_graph_t *graph, dt_module_t *module)
                                     "gainmap", "read", "rgba", "*", dt_no_roi);
     CONN(dt_node_connect(graph, id_asm, 5, id_doub, 1));
     CONN(dt_node_connect(graph, id_half, 1, id_doub, 2));
-    CONN(dt_node_connect(graph, id_half, 1, id_doub, 4));   // dummy gain map binding
+    if(gainmap) CONN(dt_node_connect(graph, id_gm, 0, id_doub, 4));
+    else CONN(dt_node_connect(graph, id_half, 1, id_doub, 4));   // dummy gain map binding
     dt_connector_copy(graph, module, 0, id_doub, 0);
     dt_connector_copy(graph, module, 1, id_doub, 3);
     CONN(dt_node_connect(graph, id_half, 1, id_down[0], 0));

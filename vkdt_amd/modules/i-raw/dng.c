@@ -12,6 +12,7 @@
  * message.
  */
 #include "dng.h"
+#include "pipe/dng_opcode.h"
 #include "vkdt_hip.h"
 
 #include <math.h>
@@ -331,6 +332,16 @@ decoded:
       goto done;
     }
   }
+  { // OpcodeList2 of the raw IFD (the list applied to the raw data after linearisation)
+    const entry_t *ol = find(ri, 51009);
+    if(ol && ol->cnt > 0 && type_size(ol->type) == 1)
+    {
+      img->oplist2 = malloc(ol->cnt);
+      if(!img->oplist2) { snprintf(err, errlen, "out of memory"); goto done; }
+      memcpy(img->oplist2, buf + ol->off, ol->cnt);
+      img->oplist2_len = ol->cnt;
+    }
+  }
   copy_str(&t, find(&ifd0, 271), img->maker, sizeof(img->maker));
   copy_str(&t, find(&ifd0, 272), img->model, sizeof(img->model));
   img->iso = 0.0f;
@@ -346,9 +357,12 @@ void dng_free(dng_image_t *img)
   free(img->data);
   free(img->packed);
   free(img->tiles);
+  free(img->oplist2);
   img->data = 0;
   img->packed = 0;
   img->tiles = 0;
+  img->oplist2 = 0;
+  img->oplist2_len = 0;
   img->packed_len = 0;
   img->ntiles = 0;
 }
@@ -369,4 +383,111 @@ int dng_decode_host(dng_image_t *img, char *err, size_t errlen)
       return 1;
     }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// DNG opcode lists (DNG specification 1.4, chapter 7): a big-endian count,
+// then per opcode its id, the DNG version it needs, flags (bit 0 optional,
+// bit 1 may be skipped for previews), the parameter size in bytes and the
+// parameters.  The acceptance rules follow the raw input of the reference
+// (i-raw/dng_opcode_decode.c:285-345): the records must tile the list
+// exactly, and each known opcode must have the parameter size its counts
+// imply (the formulas below), or the whole list is dropped.
+// ---------------------------------------------------------------------------
+
+static uint32_t be32(const uint8_t *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+static float be_f32(const uint8_t *p) { const uint32_t u = be32(p); float f; memcpy(&f, &u, 4); return f; }
+static double be_f64(const uint8_t *p)
+{
+  const uint64_t u = (uint64_t)be32(p) << 32 | be32(p + 4);
+  double f;
+  memcpy(&f, &u, 8);
+  return f;
+}
+
+// the parameter size an opcode of this id must have, given its first
+// parameters (p, n bytes available); -1 for ids without a rule
+static int64_t opcode_size(uint32_t id, const uint8_t *p, uint32_t n)
+{
+  if(id == 1 || id == 2 || id == 5 || id == 14) { if(n < 4) return -2; }
+  switch(id)
+  {
+    case 1: return 20 + 48 * (int64_t)be32(p);                 // WarpRectilinear
+    case 2: return 20 + 32 * (int64_t)be32(p);                 // WarpFisheye
+    case 3: return 56;                                         // FixVignetteRadial
+    case 4: return 8;                                          // FixBadPixelsConstant
+    case 5: return n < 12 ? -2 : 12 + 8 * (int64_t)be32(p + 4) + 16 * (int64_t)be32(p + 8);   // FixBadPixelsList
+    case 6: return 16;                                         // TrimBounds
+    case 7: return n < 36 ? -2 : 36 + 2 * (int64_t)be32(p + 32);          // MapTable
+    case 8: return n < 36 ? -2 : 36 + 8 * (int64_t)be32(p + 32);          // MapPolynomial
+    case 9:                                                    // GainMap
+    {
+      if(n < 76) return -2;
+      const uint64_t vh = (uint64_t)be32(p + 32) * be32(p + 36);
+      if(vh > (1ull << 32)) return -2;                         // cannot match a 32-bit size
+      const uint64_t cnt = vh * be32(p + 72);
+      if(cnt > (1ull << 30)) return -2;
+      return 76 + 4 * (int64_t)cnt;
+    }
+    case 14: return 24 + 76 * (int64_t)be32(p);                // WarpRectilinear2
+    default: return -1;
+  }
+}
+
+struct dt_dng_opcode_list_t *dng_opcode_list_decode(const uint8_t *data, size_t len)
+{
+  if(!data || len < 4) return 0;
+  const uint32_t count = be32(data);
+  if(count == 0) return 0;
+  size_t o = 4;
+  for(uint32_t i = 0; i < count; i++)
+  { // the records must tile the list exactly
+    if(o + 16 > len) return 0;
+    const uint32_t sz = be32(data + o + 12);
+    if(sz > len - o - 16) return 0;
+    o += 16 + (size_t)sz;
+  }
+  if(o != len) return 0;
+  dt_dng_opcode_list_t *list = calloc(1, sizeof(*list) + (size_t)count * sizeof(dt_dng_opcode_t));
+  if(!list) return 0;
+  list->count = (int)count;
+  o = 4;
+  for(uint32_t i = 0; i < count; i++)
+  {
+    const uint32_t id = be32(data + o), flags = be32(data + o + 8), sz = be32(data + o + 12);
+    const uint8_t *p = data + o + 16;
+    dt_dng_opcode_t *op = list->ops + i;
+    op->id = (dt_dng_opcode_id_t)id;
+    op->optional = (flags & 1) != 0;
+    op->preview_skip = (flags & 2) != 0;
+    const int64_t want = opcode_size(id, p, sz);
+    if(want != -1 && want != (int64_t)sz) { dng_opcode_list_free(list); return 0; }
+    if(id == s_dngop_gain_map)
+    {
+      const uint64_t cnt = (uint64_t)be32(p + 32) * be32(p + 36) * be32(p + 72);
+      dt_dng_gain_map_t *gm = calloc(1, sizeof(*gm) + cnt * sizeof(float));
+      if(!gm) { dng_opcode_list_free(list); return 0; }
+      dt_dng_region_t *r = &gm->region;
+      r->top = be32(p); r->left = be32(p + 4); r->bottom = be32(p + 8); r->right = be32(p + 12);
+      r->plane = be32(p + 16); r->planes = be32(p + 20); r->row_pitch = be32(p + 24); r->col_pitch = be32(p + 28);
+      gm->map_points_v = be32(p + 32);
+      gm->map_points_h = be32(p + 36);
+      gm->map_spacing_v = be_f64(p + 40);
+      gm->map_spacing_h = be_f64(p + 48);
+      gm->map_origin_v = be_f64(p + 56);
+      gm->map_origin_h = be_f64(p + 64);
+      gm->map_planes = be32(p + 72);
+      for(uint64_t k = 0; k < cnt; k++) gm->map_gain[k] = be_f32(p + 76 + 4 * k);
+      op->data = gm;
+    }
+    o += 16 + (size_t)sz;
+  }
+  return list;
+}
+
+void dng_opcode_list_free(struct dt_dng_opcode_list_t *list)
+{
+  if(!list) return;
+  for(int i = 0; i < list->count; i++) free(list->ops[i].data);
+  free(list);
 }

@@ -32,6 +32,9 @@ typedef struct dng_image_t
   size_t    packed_len;
   vkdt_lj92_tile_t *tiles;
   uint32_t  ntiles;
+  // OpcodeList2 of the raw IFD as stored (big-endian records), or 0
+  uint8_t  *oplist2;
+  uint32_t  oplist2_len;
 }
 dng_image_t;
 
@@ -40,3 +43,11 @@ int  dng_load(const char *filename, dng_image_t *img, char *err, size_t errlen);
 void dng_free(dng_image_t *img);
 // fills data from the lossless-JPEG streams (no-op when data is there)
 int  dng_decode_host(dng_image_t *img, char *err, size_t errlen);
+
+// decodes a stored opcode list into pipe/dng_opcode.h structs (GainMap
+// parameters decoded, other opcodes' data left 0).  Returns 0 for an empty
+// list, or when the list's length or any opcode's parameter size is
+// inconsistent (the whole list is then ignored).
+struct dt_dng_opcode_list_t;
+struct dt_dng_opcode_list_t *dng_opcode_list_decode(const uint8_t *data, size_t len);
+void dng_opcode_list_free(struct dt_dng_opcode_list_t *list);

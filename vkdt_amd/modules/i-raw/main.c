@@ -10,6 +10,7 @@
  * kernels always see RGGB or the canonical X-Trans layout.
  */
 #include "pipe/api.h"
+#include "pipe/dng_opcode.h"
 #include "dng.h"
 
 #include <limits.h>
@@ -24,8 +25,19 @@ typedef struct raw_t
   uint64_t width, height; // after offset and block rounding
   uint32_t filters;
   uint64_t crop_aabb[4];
+  dt_dng_opcode_list_t *ops[3];     // decoded OpcodeList1..3 (only 2 is read), or 0
+  dt_image_metadata_dngop_t dngop;  // attached to img_param.meta while ops[] has a list
 }
 raw_t;
+
+static void free_ops(raw_t *r)
+{
+  for(int i = 0; i < 3; i++)
+  {
+    dng_opcode_list_free(r->ops[i]);
+    r->ops[i] = 0;
+  }
+}
 
 // matrices.h: CAT16 adaptations of XYZ from the given white to D65
 static const float cat16_a_to_d65[9]   = { 9.50674182e-01f, -1.87430902e-01f, 2.62831155e-01f, -2.56724729e-02f, 1.03231456e+00f, -1.15608371e-02f, -2.74089665e-03f, 9.09809774e-02f, 2.81290019e+00f };
@@ -128,6 +140,7 @@ static int load_raw(dt_module_t *mod, int frame, const char *filename)
   raw_t *r = mod->data;
   if(!strcmp(r->filename, filename) && r->frame == frame && (r->img.data || r->img.packed)) return 0;
   dng_free(&r->img);
+  free_ops(r);
   r->filename[0] = 0;
   char err[512];
   if(dng_load(filename, &r->img, err, sizeof(err)))
@@ -145,6 +158,12 @@ static int load_raw(dt_module_t *mod, int frame, const char *filename)
     r->frame = -1;
     return 1;
   }
+  // OpcodeList2 of the raw IFD, decoded for the modules that apply it
+  // (denoise's gain maps; i-raw/main.c:60-92)
+  r->ops[1] = dng_opcode_list_decode(r->img.oplist2, r->img.oplist2_len);
+  r->dngop = (dt_image_metadata_dngop_t){ .type = s_image_metadata_dngop, .next = 0,
+                                          .ox = (uint32_t)r->ox, .oy = (uint32_t)r->oy,
+                                          .op_list = { r->ops[0], r->ops[1], r->ops[2] } };
   snprintf(r->filename, sizeof(r->filename), "%s", filename);
   r->frame = frame;
   return 0;
@@ -160,6 +179,7 @@ void cleanup(dt_module_t *mod)
 {
   if(!mod->data) return;
   dng_free(&((raw_t *)mod->data)->img);
+  free_ops(mod->data);
   free(mod->data);
   mod->data = 0;
 }
@@ -233,6 +253,7 @@ void modify_roi_out(dt_graph_t *graph, dt_module_t *mod)
   mod->img_param.whitebalance[1] = 1.0f;
   mod->img_param.filters = r->filters;
   mod->connector[0].chan = dt_token("rggb");
+  mod->img_param.meta = (r->ops[0] || r->ops[1] || r->ops[2]) ? (dt_image_metadata_t *)&r->dngop : 0;
 
   // camera rgb -> rec2020: xyz_to_rec2020 * inverse(xyz_to_cam), with a
   // CAT16 adaptation to D65 if the matrix is for another illuminant

@@ -70,6 +70,7 @@ def lib():
                                              C.c_float * 2, C.c_uint32 * 2]
         L.dt_graph_commit.argtypes = [vp]
         L.dt_graph_read_source.argtypes = [vp, i32, vp]
+        L.dt_graph_node_read_source.argtypes = [vp, i32, vp]
         rc = L.dt_pipe_global_init(PKG_DIR.encode())
         if rc != 0:
             raise ImportError(f"no module classes found under {PKG_DIR}/modules")

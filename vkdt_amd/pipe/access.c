@@ -163,7 +163,20 @@ int dt_graph_read_source(dt_graph_t *g, int m, void *buf)
 {
   if(m < 0 || m >= g->num_modules || !g->module[m].so->read_source) return 1;
   dt_read_source_params_t p = { 0 };
+  for(int n = 0; n < g->num_nodes && !p.node; n++)   // the module's first source node
+    if(g->node[n].module == g->module + m && g->node[n].connector[0].type == dt_token("source")) p.node = g->node + n;
   return g->module[m].so->read_source(g->module + m, buf, &p);
+}
+
+// read_source of one source node (a module with several sources tells them
+// apart by the node, as the runtime's upload passes it)
+int dt_graph_node_read_source(dt_graph_t *g, int n, void *buf)
+{
+  if(n < 0 || n >= g->num_nodes || g->node[n].connector[0].type != dt_token("source")) return 1;
+  dt_module_t *m = g->node[n].module;
+  if(!m || !m->so->read_source) return 1;
+  dt_read_source_params_t p = { .node = g->node + n, .a = 0 };
+  return m->so->read_source(m, buf, &p);
 }
 
 // run every module's commit_params (the uniform upload) without the device

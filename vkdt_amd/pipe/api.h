@@ -182,6 +182,55 @@ typedef enum dt_colour_trc_t          // module.h:44-60
 }
 dt_colour_trc_t;
 
+// extra metadata an input module attaches to its image params, as a singly
+// linked list of typed records it owns (metadata.h)
+typedef struct dt_image_metadata_t
+{
+  uint32_t type;
+  void    *next;
+}
+dt_image_metadata_t;
+
+#define s_image_metadata_text 1
+typedef struct dt_image_metadata_text_t
+{
+  uint32_t    type;           // s_image_metadata_text
+  void       *next;
+  const char *text;
+}
+dt_image_metadata_text_t;
+
+#define s_image_metadata_dngop 2
+typedef struct dt_image_metadata_dngop_t
+{
+  uint32_t type;              // s_image_metadata_dngop
+  void    *next;
+  uint32_t ox, oy;            // CFA offset of the emitted buffer in the raw
+  void    *op_list[3];        // dt_dng_opcode_list_t of OpcodeList1..3, or 0
+}
+dt_image_metadata_dngop_t;
+
+static inline dt_image_metadata_t *dt_metadata_append(dt_image_metadata_t *meta, dt_image_metadata_t *add)
+{
+  add->next = meta;
+  return add;
+}
+
+static inline void *dt_metadata_find(dt_image_metadata_t *meta, uint32_t type)
+{
+  for(; meta; meta = (dt_image_metadata_t *)meta->next)
+    if(meta->type == type) return meta;
+  return 0;
+}
+
+static inline void *dt_metadata_remove(dt_image_metadata_t *meta, dt_image_metadata_t *rem)
+{
+  if(!meta) return 0;
+  if(meta == rem) return rem->next;
+  meta->next = dt_metadata_remove((dt_image_metadata_t *)meta->next, rem);
+  return meta;
+}
+
 typedef struct dt_image_params_t
 {
   float    black[4];          // black point (raw units)
@@ -200,6 +249,7 @@ typedef struct dt_image_params_t
   float    noise_a;           // raw noise, gaussian part
   float    noise_b;           // raw noise, poissonian part
   dt_token_t input_name;
+  dt_image_metadata_t *meta;  // owned by the input module that attached it
 }
 dt_image_params_t;
 

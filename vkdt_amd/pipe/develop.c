@@ -101,11 +101,14 @@ int dt_graph_develop_image(dt_graph_t *g, vkdt_develop_image_t *d)
   d->filters = pc[16];
   memcpy(&d->noise_a, pc + 17, 4);
   memcpy(&d->noise_b, pc + 18, 4);
-  if(pc[19]) return fail(g, "dng gain maps are not supported");
 
   int sz = 0;
   const void *p = node_params(doub, &sz);
   if(sz < 32) return fail(g, "denoise params");
+  // a Bayer gain map (push constant and module parameter both 1) is applied
+  // by the per-node doub only
+  if(pc[19] == 1 && d->filters != 9 && (sz < 36 || ((const int32_t *)p)[8] == 1))
+    return fail(g, "the batched path does not apply dng gain maps (the per-node graph does)");
   memcpy(d->dn_params, p, 32);
   p = node_params(fix, &sz);
   d->dm_fixup = sz >= 4 ? ((const int32_t *)p)[0] : 0;

@@ -256,6 +256,47 @@ def write_dng(path, cfa, kind="bayer", black=512, white=16383, as_shot_neutral=(
     return path
 
 
+def gain_map_opcode(gains, top=0, left=0, bottom=None, right=None, spacing=None, origin=(0.0, 0.0),
+                    plane=0, planes=1, row_pitch=2, col_pitch=2, flags=0, height=None, width=None):
+    """one GainMap opcode (id 9, DNG specification 1.4 chapter 7) as stored in
+    an opcode list, big endian: the region, the map grid of ``gains`` (an
+    array of map_points_v x map_points_h [x planes] float32), its spacing and
+    origin (v, h) in relative image coordinates.  ``spacing`` defaults to a
+    grid spanning the unit square."""
+    import struct
+    g = np.asarray(gains, np.float32)
+    if g.ndim == 2:
+        g = g[..., None]
+    mv, mh, mp = g.shape
+    if spacing is None:
+        spacing = (1.0 / max(mv - 1, 1), 1.0 / max(mh - 1, 1))
+    bottom = height if bottom is None else bottom
+    right = width if right is None else right
+    params = struct.pack(">8I", top, left, bottom, right, plane, planes, row_pitch, col_pitch)
+    params += struct.pack(">2I4d", mv, mh, spacing[0], spacing[1], origin[0], origin[1])
+    params += struct.pack(">I", mp) + g.astype(">f4").tobytes()
+    return struct.pack(">4I", 9, 0x01030000, flags, len(params)) + params
+
+
+def opcode_list(ops):
+    """an opcode list (OpcodeList1/2/3 tag value): big-endian count, then the
+    opcodes' records (from gain_map_opcode, or raw (id, params) pairs)"""
+    import struct
+    recs = []
+    for op in ops:
+        if isinstance(op, tuple):
+            oid, params = op
+            op = struct.pack(">4I", oid, 0x01030000, 0, len(params)) + params
+        recs.append(op)
+    return struct.pack(">I", len(recs)) + b"".join(recs)
+
+
+def bayer_gain_maps(W, H, gains4, **kw):
+    """the four GainMap opcodes of a Bayer raw, one per site of the 2x2 block
+    (top, left) = (0,0) (0,1) (1,0) (1,1), gains4[k] the k-th site's map"""
+    return [gain_map_opcode(gains4[k], top=k // 2, left=k % 2, height=H, width=W, **kw) for k in range(4)]
+
+
 def pack_msb16(samples, bits):
     """samples (uint16) as the bit stream of Magic Lantern raw frames: each
     sample's low `bits` bits most significant first, cut into 16-bit words
