@@ -200,36 +200,48 @@ def run_noop(raw, gm, pc, params, wd, ht, bits=64):
     return out
 
 
-def test_noop_constant_gains_scale_each_site():
+def check_noop_constant_gains_scale_each_site(bits=64):
     rng = np.random.default_rng(11)
     W, H = 40, 30
     raw = rng.integers(0, 16384, size=(H, W), dtype=np.uint16)
     k = np.array([0.7, 1.1, 1.3, 0.9])
     gm = np.broadcast_to(k, (3, 4, 4)).copy()
     os_ = [0.0, 0.0, 1.0, 1.0]
-    base = run_noop(raw, None, noop_pc(W, H, os_, gainmap=0), dn_params(), W, H, bits=32)
-    with_g = run_noop(raw, gm, noop_pc(W, H, os_), dn_params(), W, H, bits=32)
+    base = run_noop(raw, None, noop_pc(W, H, os_, gainmap=0), dn_params(), W, H, bits=bits)
+    with_g = run_noop(raw, gm, noop_pc(W, H, os_), dn_params(), W, H, bits=bits)
     site = (np.arange(W)[None, :] & 1) + 2 * (np.arange(H)[:, None] & 1)
-    want = (base[..., 0].astype(np.float32) * np.float32(k)[site]).astype(np.float64)
-    assert np.array_equal(with_g[..., 0], want)
+    kk = k.astype(np.float32) if bits == 32 else k     # the f32 build reads the grid as float
+    want = (base[..., 0].astype(kk.dtype) * kk[site]).astype(np.float64)
+    # texture() of a constant grid is the constant up to the lerp's rounding
+    assert np.allclose(with_g[..., 0], want, rtol=1e-6 if bits == 32 else 1e-14, atol=0)
     # the module parameter off, or an x-trans mosaic: no gain
-    assert np.array_equal(run_noop(raw, gm, noop_pc(W, H, os_), dn_params(gainmap=0), W, H, bits=32), base)
-    xt = run_noop(raw, gm, noop_pc(W, H, os_, filters=9), dn_params(), W, H, bits=32)
+    assert np.array_equal(run_noop(raw, gm, noop_pc(W, H, os_), dn_params(gainmap=0), W, H, bits=bits), base)
+    xt = run_noop(raw, gm, noop_pc(W, H, os_, filters=9), dn_params(), W, H, bits=bits)
     assert np.array_equal(xt, run_noop(raw, None, noop_pc(W, H, os_, gainmap=0, filters=9), dn_params(), W, H,
-                                       bits=32))
+                                       bits=bits))
 
 
-@pytest.mark.parametrize("origin,scale", [((0.0, 0.0), (1.0, 1.0)), ((0.1, 0.05), (1.25, 0.8))])
-def test_noop_linear_map_is_interpolated_at_the_texel_position(origin, scale):
-    """gains linear in the map's columns and rows come back as the same
-    linear function of the texture coordinate pos*n - 0.5, clamped to the
-    grid, with pos = clamp(((x + 0.5 + crop)/W) * sx - ox, 0, 1)"""
-    W, H, cx, cy = 48, 36, 4, 2
-    raw = np.full((H, W), 8000, np.uint16)
-    mw, mh = 6, 5
+def linear_grid(mh, mw):
     a = np.array([1.0, 0.9, 1.1, 0.95]); bx = np.array([0.2, -0.1, 0.05, 0.3]); by = np.array([-0.15, 0.1, 0.2, 0.0])
     ii, jj = np.meshgrid(np.arange(mw), np.arange(mh))
     gm = a + bx * ii[..., None] / (mw - 1) + by * jj[..., None] / (mh - 1)
+
+    def at(px, py, site):
+        """the grid's value at texture position (px, py): linear in the texel
+        coordinate p*n - 0.5, clamped to the grid (mirrored edges)"""
+        u = np.clip(px * mw - 0.5, 0, mw - 1) / (mw - 1)
+        v = np.clip(py * mh - 0.5, 0, mh - 1) / (mh - 1)
+        return a[site] + bx[site] * u + by[site] * v
+    return gm, at
+
+
+def check_noop_linear_map(origin=(0.1, 0.05), scale=(1.25, 0.8)):
+    """gains linear in the map's columns and rows come back as the same
+    linear function of the texture coordinate, at pos = clamp(((x + 0.5 +
+    crop)/W) * sx - ox, 0, 1)"""
+    W, H, cx, cy = 48, 36, 4, 2
+    raw = np.full((H, W), 8000, np.uint16)
+    gm, at = linear_grid(5, 6)
     os_ = [float(np.float32(v)) for v in (origin[1], origin[0], scale[1], scale[0])]   # as pushed
     wd, ht = W - 2 * cx, H - 2 * cy
     base = run_noop(raw, None, noop_pc(W, H, os_, gainmap=0, crop=(cx, cy)), dn_params(), wd, ht)
@@ -238,40 +250,85 @@ def test_noop_linear_map_is_interpolated_at_the_texel_position(origin, scale):
     y = np.arange(ht)[:, None]
     px = np.clip((x + 0.5 + cx) / W * os_[2] - os_[0], 0, 1)
     py = np.clip((y + 0.5 + cy) / H * os_[3] - os_[1], 0, 1)
-    u = np.clip(px * mw - 0.5, 0, mw - 1) / (mw - 1)
-    v = np.clip(py * mh - 0.5, 0, mh - 1) / (mh - 1)
-    site = (x & 1) + 2 * (y & 1)
-    want = a[site] + bx[site] * u + by[site] * v
+    want = at(px, py, (x & 1) + 2 * (y & 1))
     assert np.allclose(out[..., 0] / base[..., 0], want, rtol=0, atol=1e-12)
 
 
-def test_doub_constant_gains_scale_each_site():
+def run_doub(raw, crs0, crs1, gm, gainmap, map_os, W, H, bits=64):
+    black, white = np.float32(512 / 65535.0), np.float32(16383 / 65535.0)
+    pc = gs.mixed_blob([("f", 1.0)] * 4 + [("f", black)] * 4 + [("f", white)] * 4
+                       + [("i", 0), ("i", 0), ("i", W), ("i", H)]
+                       + [("i", BAYER), ("f", 1e-4), ("f", 1e-6), ("i", gainmap)] + [("f", v) for v in map_os])
+    out = np.zeros((H, W))
+    bufs = [op.orbuf(raw, op.OR_UI16, 1), op.orbuf(crs0, op.OR_F32, 4), op.orbuf(crs1, op.OR_F32, 4),
+            op.orbuf(out, op.OR_F32, 1)]
+    if gm is not None:
+        bufs.append(op.orbuf(np.ascontiguousarray(gm, np.float64), op.OR_F32, 4))
+    op.node(bits, "denoise", "doub", pc, dn_params(), bufs, W, H, storage_f16=False)
+    return out
+
+
+def doub_inputs(W, H, seed=12):
+    rng = np.random.default_rng(seed)
+    raw = rng.integers(600, 15000, size=(H, W), dtype=np.uint16)
+    crs0 = rng.uniform(0.05, 0.8, size=(H // 2, W // 2, 4))
+    crs1 = rng.uniform(0.05, 0.8, size=(H // 2, W // 2, 4))
+    return raw, crs0, crs1
+
+
+def check_doub_constant_gains_scale_each_site(bits=64):
     """doub.comp:106-114 on a constant map: the denoised value times the
     gain of its CFA site, and nothing else changes"""
-    rng = np.random.default_rng(12)
     W, H = 32, 24
-    Qw, Qh = W // 2, H // 2
-    raw = rng.integers(600, 15000, size=(H, W), dtype=np.uint16)
-    crs0 = rng.uniform(0.05, 0.8, size=(Qh, Qw, 4))
-    crs1 = rng.uniform(0.05, 0.8, size=(Qh, Qw, 4))
+    raw, crs0, crs1 = doub_inputs(W, H)
     k = np.array([1.2, 0.85, 1.05, 0.75])
     gm = np.broadcast_to(k, (4, 5, 4)).copy()
-    black, white = np.float32(512 / 65535.0), np.float32(16383 / 65535.0)
-
-    def run(gainmap, gmap):
-        pc = gs.mixed_blob([("f", 1.0)] * 4 + [("f", black)] * 4 + [("f", white)] * 4
-                           + [("i", 0), ("i", 0), ("i", W), ("i", H)]
-                           + [("i", BAYER), ("f", 1e-4), ("f", 1e-6), ("i", gainmap)]
-                           + [("f", 0.0), ("f", 0.0), ("f", 1.0), ("f", 1.0)])
-        out = np.zeros((H, W))
-        bufs = [op.orbuf(raw, op.OR_UI16, 1), op.orbuf(crs0, op.OR_F32, 4), op.orbuf(crs1, op.OR_F32, 4),
-                op.orbuf(out, op.OR_F32, 1)]
-        if gmap is not None:
-            bufs.append(op.orbuf(gmap, op.OR_F32, 4))
-        op.node(32, "denoise", "doub", pc, dn_params(), bufs, W, H, storage_f16=False)
-        return out
-
-    base = run(0, None)
-    out = run(1, gm)
+    base = run_doub(raw, crs0, crs1, None, 0, [0, 0, 1, 1], W, H, bits)
+    out = run_doub(raw, crs0, crs1, gm, 1, [0, 0, 1, 1], W, H, bits)
     site = (np.arange(W)[None, :] & 1) + 2 * (np.arange(H)[:, None] & 1)
-    assert np.array_equal(out, (base.astype(np.float32) * np.float32(k)[site]).astype(np.float64))
+    kk = k.astype(np.float32) if bits == 32 else k
+    assert np.allclose(out, (base.astype(kk.dtype) * kk[site]).astype(np.float64), rtol=1e-6 if bits == 32 else 1e-14,
+                       atol=0)
+
+
+def check_doub_linear_map_at_the_block_centre():
+    """doub samples the grid at its 2x2 block's centre on the half-resolution
+    grid: pos = (0.5 + (x + crop)//2) / (W//2), the same for the block's four
+    sites (which then read their own channels)"""
+    W, H = 36, 28
+    raw, crs0, crs1 = doub_inputs(W, H, seed=13)
+    gm, at = linear_grid(4, 5)
+    os_ = [float(np.float32(v)) for v in (0.02, 0.04, 1.1, 0.9)]
+    base = run_doub(raw, crs0, crs1, None, 0, os_, W, H)
+    out = run_doub(raw, crs0, crs1, gm, 1, os_, W, H)
+    x = np.arange(W)[None, :]
+    y = np.arange(H)[:, None]
+    px = np.clip((0.5 + x // 2) / (W // 2) * os_[2] - os_[0], 0, 1)
+    py = np.clip((0.5 + y // 2) / (H // 2) * os_[3] - os_[1], 0, 1)
+    want = at(px, py, (x & 1) + 2 * (y & 1))
+    ok = base > 1e-6
+    assert np.allclose(out[ok] / base[ok], np.broadcast_to(want, out.shape)[ok], rtol=0, atol=1e-12)
+
+
+# the gain checks the planted-mistake test runs beside the stencil pins
+CHECKS = [check_noop_constant_gains_scale_each_site, check_noop_linear_map, check_doub_constant_gains_scale_each_site,
+          check_doub_linear_map_at_the_block_centre]
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_noop_constant_gains_scale_each_site(bits):
+    check_noop_constant_gains_scale_each_site(bits)
+
+
+@pytest.mark.parametrize("origin,scale", [((0.0, 0.0), (1.0, 1.0)), ((0.1, 0.05), (1.25, 0.8))])
+def test_noop_linear_map_is_interpolated_at_the_texel_position(origin, scale):
+    check_noop_linear_map(origin, scale)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_doub_constant_gains_scale_each_site(bits):
+    check_doub_constant_gains_scale_each_site(bits)
+
+
+def test_doub_linear_map_at_the_block_centre():
+    check_doub_linear_map_at_the_block_centre()

@@ -3,7 +3,8 @@
 Each case plants one mistake in a copy of oracle/oracle.c (text
 substitution in a temporary directory; the oracle itself is untouched),
 builds it with the oracle's own flags, and runs every check of
-tests/test_oracle_pins_stencils.py against the build.  A mutant that passes
+tests/test_oracle_pins_stencils.py and the gain-map checks of
+tests/test_gainmap.py against the build.  A mutant that passes
 all of them would be a mistake the pins cannot see, so the test requires at
 least one failure per mutant.  The first four are the denoise/demosaic
 mistakes that the closed forms of test_oracle_pins.py alone did not catch.
@@ -15,6 +16,7 @@ import subprocess
 import pytest
 
 import oracle_py as op
+import test_gainmap as gain
 import test_oracle_pins_stencils as pins
 
 ORACLE = os.path.join(op.ROOT, "oracle")
@@ -48,6 +50,12 @@ MUTANTS = [
      "doub.comp:96"),
     ("doub_sigma_red_green", "(val - crs1) / FMAX(sig[0] + sig[2], R(1e-8))", "(val - crs1) / FMAX(sig[0] + sig[1], R(1e-8))",
      "doub.comp:89"),
+    ("gain_site_transposed", "const int site = (x & 1) + 2 * (y & 1);\n  return site == 0 ? g.x",
+     "const int site = 2 * (x & 1) + (y & 1);\n  return site == 0 ? g.x", "noop.comp:52"),
+    ("gain_origin_ignored", "px = FMIN(FMAX(px * R(map_os[2]) - R(map_os[0]), R(0)), R(1));",
+     "px = FMIN(FMAX(px * R(map_os[2]), R(0)), R(1));", "noop.comp:50"),
+    ("doub_gain_at_the_site", "const real px = (R(0.5) + R((x + pci[12]) / 2)) / R(out->wd / 2);",
+     "const real px = (R(0.5) + R(x + pci[12])) / R(out->wd);", "doub.comp:108"),
 ]
 
 
@@ -82,7 +90,7 @@ def test_mutant_fails_a_pin(mutants, mid):
     saved = op._libs.get(64)
     op._libs[64] = lib
     try:
-        for check in pins.CHECKS:
+        for check in pins.CHECKS + gain.CHECKS:
             try:
                 check()
             except AssertionError:
