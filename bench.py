@@ -434,6 +434,9 @@ STREAMS, FRAME_W, FRAME_H, POOL_FRAMES = 8, 4096, 2160, 8
 STILLS = [(6000, 4000), (7008, 4672), (8256, 5504), (9504, 6336)]
 
 
+SERVE_P99_MS = 50.0   # the latency bound the serving capacity is quoted at
+
+
 def serve_workload(seconds, rate=600.0, p_still=0.2, seed=31):
     """Poisson arrivals at `rate`; each is a still with probability p_still,
     else the next frame of a uniformly chosen stream"""
@@ -516,29 +519,48 @@ def run_serve(args, tune=None):
     torch.cuda.synchronize()
     print(f"[bench] serving profiles ready; plain ms {plain_ms}", file=sys.stderr, flush=True)
 
-    work = serve_workload(args.serve_seconds)
-    reqs = []
-    for i, (key, t) in enumerate(work):
-        prof = profiles[key[:2]]
-        reqs.append(serve.Request(i, prof, host[key].data_ptr(), t, tag=key[0]))
-    srv = serve.Server(200 << 30, slots=3, max_batch=48, tune=tune)
-    try:
-        wall = srv.run(reqs)
-    finally:
-        srv.close()
-    steady = [r for r in reqs if r.arrival >= 0.5]
-    s = serve.summary(steady, wall - 0.5)
+    def serve_at(rate, seconds):
+        work = serve_workload(seconds, rate=rate)
+        reqs = []
+        for i, (key, t) in enumerate(work):
+            prof = profiles[key[:2]]
+            reqs.append(serve.Request(i, prof, host[key].data_ptr(), t, tag=key[0]))
+        srv = serve.Server(200 << 30, slots=3, max_batch=48, tune=tune)
+        try:
+            wall = srv.run(reqs)
+        finally:
+            srv.close()
+        steady = [r for r in reqs if r.arrival >= 0.5]
+        s = serve.summary(steady, wall - 0.5)
+        # the plain path as one FIFO server over the same arrivals
+        fin, lat = 0.0, []
+        for key, t in work:
+            ms = plain_ms["frame"] if key[0] == "frame" else plain_ms[key[:2]]
+            fin = max(fin, t) + ms * 1e-3
+            if t >= 0.5:
+                lat.append((fin - t) * 1e3)
+        px = sum(r.profile.pixels for r in reqs)
+        busy = sum(plain_ms["frame"] if k[0] == "frame" else plain_ms[k[:2]] for k, _ in work) * 1e-3
+        sizes = [b[0] for b in srv.batches]
+        return work, reqs, srv, s, lat, px, busy, sizes
+
+    work, reqs, srv, s, lat, px, busy, sizes = serve_at(600.0, args.serve_seconds)
     nst = sum(1 for r in reqs if r.tag == "still")
-    # the plain path as one FIFO server over the same arrivals
-    fin, lat = 0.0, []
-    for key, t in work:
-        ms = plain_ms["frame"] if key[0] == "frame" else plain_ms[key[:2]]
-        fin = max(fin, t) + ms * 1e-3
-        if t >= 0.5:
-            lat.append((fin - t) * 1e3)
-    px = sum(r.profile.pixels for r in reqs)
-    busy = sum(plain_ms["frame"] if k[0] == "frame" else plain_ms[k[:2]] for k, _ in work) * 1e-3
-    sizes = [b[0] for b in srv.batches]
+    # capacity: offered load against p99, arrivals of the same mix at rising rates
+    sweep = []
+    for rate in [float(r) for r in args.serve_sweep.split(",") if r.strip()]:
+        w2, r2, _, s2, lat2, px2, _, sz2 = serve_at(rate, args.sweep_seconds)
+        sweep.append({"rate_per_s": rate, "offered_mpix_per_s": round(px2 / args.sweep_seconds / 1e6, 1),
+                      "served_mpix_per_s": s2["mpix_per_s"], "p50_ms": s2["p50_ms"], "p99_ms": s2["p99_ms"],
+                      "mean_batch": round(float(np.mean(sz2)), 2),
+                      "plain_fifo_p99_ms": round(float(np.percentile(lat2, 99)), 3)})
+        print(f"[bench] serving {rate:.0f} req/s: {sweep[-1]}", file=sys.stderr, flush=True)
+    ok = [e for e in sweep if e["p99_ms"] <= SERVE_P99_MS and e["served_mpix_per_s"] >= 0.97 * e["offered_mpix_per_s"]]
+    capacity = {"p99_bound_ms": SERVE_P99_MS,
+                "rate_per_s": ok[-1]["rate_per_s"] if ok else None,
+                "mpix_per_s": ok[-1]["served_mpix_per_s"] if ok else None,
+                "method": f"highest swept rate ({args.sweep_seconds:g} s of arrivals each) whose p99 stays within "
+                          f"{SERVE_P99_MS} ms and whose served rate is within 3% of the offered rate"}
     return {
         "workload": f"Poisson {len(work) / args.serve_seconds:.0f} req/s for {args.serve_seconds:g} s: "
                     f"{STREAMS} streams of {FRAME_W}x{FRAME_H} RGGB 12-bit frames + 24-61 MP 14-bit stills"
@@ -551,6 +573,7 @@ def run_serve(args, tune=None):
         "plain_fifo": {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                        "capacity_mpix_per_s": round(px / busy / 1e6, 1),
                        "method": "FIFO over the same arrivals with the per-node graph's measured time per shape"},
+        "sweep": sweep, "capacity": capacity,
     }
 
 
@@ -571,6 +594,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="batch mode: skip timing the plain path beside it")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="batch mode: also run the serving loop on this many seconds of arrivals (0: skip)")
+    ap.add_argument("--serve-sweep", default="750,900,1050,1200,1500",
+                    help="batch mode: arrival rates (req/s) of the capacity sweep, comma separated ('' to skip)")
+    ap.add_argument("--sweep-seconds", type=float, default=3.0, help="seconds of arrivals per swept rate")
     ap.add_argument("--breakdown", default="", help="plain mode: write per-kernel timings as JSON to this path")
     ap.add_argument("--calibrate-valu", action="store_true",
                     help="only time chains of v_fma/v_mul/v_exp/v_rcp/v_sqrt_f32 (the VALU issue rates)")
