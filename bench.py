@@ -594,7 +594,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="batch mode: skip timing the plain path beside it")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="batch mode: also run the serving loop on this many seconds of arrivals (0: skip)")
-    ap.add_argument("--serve-sweep", default="750,900,1050,1200,1500",
+    ap.add_argument("--serve-sweep", default="650,700,750,900,1050,1200",
                     help="batch mode: arrival rates (req/s) of the capacity sweep, comma separated ('' to skip)")
     ap.add_argument("--sweep-seconds", type=float, default=3.0, help="seconds of arrivals per swept rate")
     ap.add_argument("--breakdown", default="", help="plain mode: write per-kernel timings as JSON to this path")

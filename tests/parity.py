@@ -147,3 +147,37 @@ def e2e_check(cfa, p, gpu, log=print, label=""):
     assert psnr >= 60.0 and psnr64 >= 60.0, stats
     assert n_gpu <= 2 * max(n_twin) + 2, stats
     return stats
+
+
+def e2e_check_nodes(bufs, nodes, cfa, gpu, sources=None, log=print, label=""):
+    """e2e_check for graphs the whole-graph oracle does not build (half size,
+    scaled exports, RCD, gain maps): the restated node list (graphspec) run on
+    the oracle's float build, its perturbed twins and the FP64 build, against
+    the GPU's rgba, by the same three criteria"""
+    ref_all = op.run_nodes(32, bufs, nodes, cfa, sources=sources)
+    tol = errbar.tolerance(ref_all["fc"], ref_all["out0"])
+    if tol.shape != ref_all["final"].shape[:2]:          # a scaled export: the bound of the output extent
+        tol = np.full(ref_all["final"].shape[:2], 1e-3)
+    ref = ref_all["final"][..., :3]
+    gpu = np.asarray(gpu, np.float64)[..., :3]
+    d = np.abs(gpu - ref).max(-1)
+    n_gpu = int((d > tol).sum())
+    twin = op.lib("32p")
+    n_twin = []
+    for s in TWIN_SEEDS:
+        twin.or_set_perturb_seed(C.c_uint32(s))
+        t = op.run_nodes("32p", bufs, nodes, cfa, sources=sources)["final"][..., :3]
+        n_twin.append(int((np.abs(t - ref).max(-1) > tol).sum()))
+    psnr = 10 * np.log10(1.0 / max(float(np.mean((gpu - ref) ** 2)), 1e-30))
+    ref64 = op.run_nodes(64, bufs, nodes, cfa, sources=sources)["final"][..., :3]
+    d64 = np.abs(gpu - ref64)
+    psnr64 = 10 * np.log10(1.0 / max(float(np.mean(d64 ** 2)), 1e-30))
+    stats = dict(max_abs=float(d.max()), n_over_1e3=int((d > 1e-3).sum()), n_over_tol=n_gpu, n_twin=n_twin,
+                 psnr=psnr, max_abs64=float(d64.max()), psnr64=psnr64, pixels=int(d.size))
+    log(f"[e2e{label}] vs float oracle (node list): max abs {stats['max_abs']:.3e}, >1e-3 {stats['n_over_1e3']}, "
+        f">bound {n_gpu} (perturbed float twins: {n_twin}), psnr {psnr:.1f} dB; vs FP64: max abs "
+        f"{stats['max_abs64']:.3e}, psnr {psnr64:.1f} dB")
+    assert np.isfinite(gpu).all()
+    assert psnr >= 60.0 and psnr64 >= 60.0, stats
+    assert n_gpu <= 2 * max(n_twin) + 2, stats
+    return stats

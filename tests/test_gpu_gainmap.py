@@ -2,8 +2,8 @@
 noop and doub with the four-site gain map bound, per node against the float
 oracle on the GPU's own inputs (at most 4 binary16 ulps outside near ties,
 as every node), and the product graph from a DNG whose OpcodeList2 holds
-the maps (i-raw decode, the gain-map source upload) against the FP64 oracle
-running the same node list."""
+the maps (i-raw decode, the gain-map source upload) against the oracle
+running the same node list (float build, perturbed twins, FP64 build)."""
 import numpy as np
 import pytest
 
@@ -67,15 +67,13 @@ def test_product_graph_with_opcode_list(tmp_path, hip):
                          cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,),
                          gain_map=gmd)
     bufs, nodes = gs.build(W, H, p)
-    ref = op.run_nodes(64, bufs, nodes, cfa, sources={"gm": gmd["gains"]})["final"][..., :3]
-    d = np.abs(img.astype(np.float64) - ref)
-    psnr = 10 * np.log10(1.0 / max(float(np.mean(d ** 2)), 1e-30))
+    parity.e2e_check_nodes(bufs, nodes, cfa, img, sources={"gm": gmd["gains"]}, log=lambda m: print(m, flush=True),
+                           label=" gain map")
     p0 = gs.DevelopParams(kind="bayer", noise_a=np.float32(na), noise_b=np.float32(nb),
                           cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,))
     b0, n0 = gs.build(W, H, p0)
     ref0 = op.run_nodes(64, b0, n0, cfa)["final"][..., :3]
     psnr0 = 10 * np.log10(1.0 / max(float(np.mean((img.astype(np.float64) - ref0) ** 2)), 1e-30))
-    print(f"[gpu_gainmap] product graph vs FP64 oracle with the maps: max abs {d.max():.3e}, psnr {psnr:.1f} dB; "
-          f"against the oracle without them: psnr {psnr0:.1f} dB", flush=True)
-    assert psnr >= 60.0 and psnr0 < 45.0
+    print(f"[gpu_gainmap] the product graph against the oracle without the maps: psnr {psnr0:.1f} dB", flush=True)
+    assert psnr0 < 45.0
     g.close()

@@ -7,13 +7,15 @@ test_gpu_nodes.py:
   * ragged mosaics (extent not a multiple of the block): doub's trailing
     sites come from the threads past the node extent;
   * the product graph (cfg -> module DSOs -> nodes -> HIP) with hilite
-    between denoise and demosaic against the FP64 oracle.
+    between denoise and demosaic against the oracle's whole graph (float
+    build, perturbed twins, FP64 build: tests/parity.py).
 """
 import numpy as np
 import pytest
 
 import graphspec as gs
 import oracle_py as op
+import parity
 from vkdt_amd import pfm, pipe, synth
 
 pytestmark = pytest.mark.gpu
@@ -131,7 +133,7 @@ def test_ragged_mosaic_trailing_sites(hip, kind, W, H):
 
 
 @pytest.mark.parametrize("kind,W,H,seed", [("bayer", 320, 240, 41), ("xtrans", 330, 246, 42)])
-def test_product_graph_with_hilite_vs_oracle64(tmp_path, hip, kind, W, H, seed):
+def test_product_graph_with_hilite_vs_oracle(tmp_path, hip, kind, W, H, seed):
     cfa, _ = synth.mosaic(W, H, seed, kind, clip_frac=0.04)
     dng = str(tmp_path / "in.dng")
     synth.write_dng(dng, cfa, kind, as_shot_neutral=(0.5, 1.0, 0.7))
@@ -146,17 +148,12 @@ def test_product_graph_with_hilite_vs_oracle64(tmp_path, hip, kind, W, H, seed):
     ip = g.img_param("i-raw", "main")
     p = gs.DevelopParams(kind=kind, noise_a=np.float32(na), noise_b=np.float32(nb), hilite=True,
                          cam=np.array(ip["cam"], np.float32).reshape(3, 3), wb=tuple(ip["wb"][:3]) + (1.0,))
-    ref = op.develop(64, cfa[:img.shape[0], :img.shape[1]], gs.oracle_params(p))[..., :3]
+    mos = cfa[:img.shape[0], :img.shape[1]]
+    parity.e2e_check(mos, p, img, log=lambda m: print(m, flush=True), label=f" hilite {kind}")
+    ref = op.develop(32, mos, gs.oracle_params(p))[..., :3]
     p.hilite = False
-    ref0 = op.develop(64, cfa[:img.shape[0], :img.shape[1]], gs.oracle_params(p))[..., :3]
-    d = np.abs(img.astype(np.float64) - ref)
-    psnr = 10 * np.log10(1.0 / max(float(np.mean(d ** 2)), 1e-30))
-    frac = float(np.mean(d > 1e-3))
+    ref0 = op.develop(32, mos, gs.oracle_params(p))[..., :3]
     moved = float(np.mean(np.abs(ref - ref0) > 1e-3))
-    print(f"[gpu_hilite] {kind}: max abs {d.max():.3e} frac>1e-3 {frac:.2e} psnr {psnr:.1f} dB; "
-          f"hilite changes {moved:.3f} of the output", flush=True)
+    print(f"[gpu_hilite] {kind}: hilite changes {moved:.3f} of the output", flush=True)
     assert moved > 0.001
-    assert np.isfinite(img).all()
-    assert psnr >= 60.0
-    assert frac <= 1e-3
     g.close()

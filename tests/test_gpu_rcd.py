@@ -1,11 +1,19 @@
 """RCD demosaic (method 1) on the GPU: rcd_conv's three outputs and
 rcd_fill's tiles against the float oracle on the GPU's own inputs, and the
-graph end to end against the FP64 oracle running the same node list."""
+graph end to end against the oracle running the same node list (its float
+build, perturbed twins and FP64 build, tests/parity.py).
+
+Per node the RCD kernels keep a fraction bound (at most 0.1% of values
+beyond 4 binary16 ulps): rcd_fill picks directions by comparing
+discriminators (rcd_fill.comp), and the RCD restatement does not flag its
+near ties, so a flipped choice cannot be told from a mistake by the ulp
+count alone.  Measured: rcd_conv 0 ulps, rcd_fill 1.7e-4 of its sites."""
 import numpy as np
 import pytest
 
 import graphspec as gs
 import oracle_py as op
+import parity
 from parity import f16_ulps, run_gpu_graph, to_or
 from vkdt_amd import synth
 
@@ -41,11 +49,4 @@ def test_rcd_nodes_and_end_to_end(hip, W, H, seed, hilite, max_wd):
             d = f16_ulps(host[cn].astype(np.float64), o)
             print(f"  {n['kernel']} -> {cn}: max {d.max():.3g} ulp, frac>4ulp {np.mean(d > 4):.2e}", flush=True)
             assert np.mean(d > 4) <= 1e-3, (n["kernel"], cn)
-    ref = op.run_nodes(64, bufs, nodes, cfa)["final"]
-    gpu = host["final"].astype(np.float64)
-    d = np.abs(gpu[..., :3] - ref[..., :3])
-    psnr = 10 * np.log10(1.0 / max(float(np.mean(d ** 2)), 1e-30))
-    print(f"[gpu_rcd] e2e vs FP64: max abs {d.max():.3e}, frac>1e-3 {np.mean(d > 1e-3):.2e}, psnr {psnr:.1f} dB",
-          flush=True)
-    assert np.isfinite(gpu).all()
-    assert psnr >= 60.0 and np.mean(d > 1e-3) <= 1e-3
+    parity.e2e_check_nodes(bufs, nodes, cfa, host["final"], log=lambda m: print(m, flush=True), label=" rcd")

@@ -235,6 +235,23 @@ DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const
     ev[1] = clampf(ev[1], 0.01f, 25.0f);
     if(cov.p) st_h4(cov, x + a.crop[0], y + a.crop[1], make_float4(ev[0], ev[1], e0[0], e0[1]));
     const float ri0 = -0.5f * frcp(ev[0]), ri1 = -0.5f * frcp(ev[1]);
+    // the anisotropic weight of offset o = (i-2, j-2) equals that of -o bit
+    // for bit (negating both offsets negates each rounded product and their
+    // rounded sum exactly, and the sums are squared), so the exponentials
+    // are taken for the half plane j > 2 or (j = 2, i >= 2) and mirrored
+    float wgt[5][5];
+#pragma unroll
+    for(int j = 2; j < 5; j++)
+#pragma unroll
+      for(int i = 0; i < 5; i++)
+      {
+        if(j == 2 && i < 2) continue;
+        const float x0 = (float)(i - 2) * e0[0] + (float)(j - 2) * e0[1];
+        const float x1 = (float)(i - 2) * e1[0] + (float)(j - 2) * e1[1];
+        const float w = fmaxf(1e-9f, fexp(x0 * x0 * ri0 + x1 * x1 * ri1));
+        wgt[j][i] = w;
+        wgt[4 - j][4 - i] = w;
+      }
     float r0 = 0, r1 = 0, r2 = 0, wt = 0;
     const float hot = 2.0f * mean_b;
 #pragma unroll
@@ -244,9 +261,7 @@ DEV void dn_downcov_tile(const dimg &in, const dimg &out, const dimg &cov, const
       {
         const float4 v = s.tile[c0 + j * TW + i];
         if(v.x > hot) continue;   // hot pixel
-        const float x0 = (float)(i - 2) * e0[0] + (float)(j - 2) * e0[1];
-        const float x1 = (float)(i - 2) * e1[0] + (float)(j - 2) * e1[1];
-        const float w = fmaxf(1e-9f, fexp(x0 * x0 * ri0 + x1 * x1 * ri1));
+        const float w = wgt[j][i];
         r0 += w * v.x; r1 += w * v.y; r2 += w * v.z;
         wt += w;
       }
