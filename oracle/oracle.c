@@ -2102,6 +2102,121 @@ int or_nprof_fit(const double *h, int wd, float *a_out, float *b_out)
 }
 
 /* ------------------------------------------------------------------------
+ * i-mcraw (src/pipe/modules/i-mcraw): the GPU codec of MotionCam frames
+ * --------------------------------------------------------------------- */
+
+static uint32_t mc_count(const uint8_t *bits, uint32_t i)
+{ // decode.comp:24-30: two 4-bit counts per byte, low nibble first
+  return (i & 1) ? (uint32_t)(bits[i / 2] >> 4) : (uint32_t)(bits[i / 2] & 15);
+}
+
+static uint32_t mc_bytes(uint32_t cnt)
+{ // prefix.comp:40-55
+  static const uint32_t sz[16] = { 0, 8, 16, 24, 32, 40, 48, 64, 64, 80, 80, 128, 128, 128, 128, 128 };
+  return sz[cnt & 15];
+}
+
+void or_mcraw_prefix(const uint8_t *bits, uint32_t n, uint32_t *off)
+{ // prefix.comp: inclusive prefix sum of the block sizes
+  uint32_t acc = 0;
+  for(uint32_t i = 0; i < n; i++) off[i] = acc += mc_bytes(mc_count(bits, i));
+}
+
+// byte r of a block, zero past the data (the decoder's loads of bytes that
+// do not exist)
+static uint32_t mc_at(const uint8_t *dat, size_t len, uint32_t off, int r)
+{
+  return (size_t)off + (size_t)r < len ? dat[off + r] : 0u;
+}
+
+// bits [bit, bit+n) of byte reg (h = 0) or reg + 1 (h = 1): GLSL's
+// bitfieldExtract(uvec2(shm[reg], shm[reg+1]), bit, n), component h
+static uint32_t mc_bits(const uint8_t *dat, size_t len, uint32_t off, int reg, int h, int bit, int n)
+{
+  return (mc_at(dat, len, off, reg + h) >> bit) & ((1u << n) - 1u);
+}
+
+void or_mcraw_block(const uint8_t *dat, size_t len, uint32_t off, uint32_t cnt, uint32_t out[64])
+{
+#define B(reg, bit, n) mc_bits(dat, len, off, (reg), h, (bit), (n))
+  for(int s = 0; s < 32; s++) for(int h = 0; h < 2; h++)
+  {
+    uint32_t v = 0;
+    switch(cnt)
+    {
+      case 0: v = 0; break;
+      case 1: v = B(2 * (s % 4), s / 4, 1); break;                   // decode.comp:38-51
+      case 2: v = B(2 * (s % 8), 2 * (s / 8), 2); break;             // :53-63
+      case 3:                                                        // :65-137
+      {
+        static const int base[8] = { 0, 0, 0, 8, 8, 8, 16, 16 }, sh[8] = { 0, 3, 6, 0, 3, 6, 0, 3 };
+        const int g = s / 4, q = s % 4;
+        v = B(base[g] + 2 * q, sh[g], 3);
+        if(g == 2) v |= B(16 + 2 * q, 6, 1) << 2;
+        if(g == 5) v |= B(16 + 2 * q, 7, 1) << 2;
+        break;
+      }
+      case 4: v = B((s / 8) * 8 + (2 * s) % 8, (s / 4) % 2 ? 4 : 0, 4); break;   // :139-148
+      case 5:                                                        // :150-182
+        if(s < 20) v = B(2 * s, 0, 5);
+        else if(s < 24) v = B(2 * (s - 20), 5, 3) | B(24 + 2 * (s - 20), 5, 2) << 3;
+        else if(s < 28) v = B(8 + 2 * (s - 24), 5, 3) | B(32 + 2 * (s - 24), 5, 2) << 3;
+        else v = B(16 + 2 * (s - 28), 5, 3) | B(24 + 2 * (s - 28), 7, 1) << 3 | B(32 + 2 * (s - 28), 7, 1) << 4;
+        break;
+      case 6:                                                        // :184-211
+        if(s < 24) v = B(2 * s, 0, 6);
+        else
+        {
+          const int r = s < 28 ? 2 * (s - 24) : 24 + 2 * (s - 28);
+          v = B(r, 6, 2) | B(r + 8, 6, 2) << 2 | B(r + 16, 6, 2) << 4;
+        }
+        break;
+      case 7: case 8: v = B(2 * s, 0, 8); break;                     // :213-220
+      case 9: case 10:                                               // :222-237
+      {
+        const int lo = s < 16 ? 2 * s : 40 + 2 * (s - 16), hi = (s < 16 ? 32 : 72) + 2 * (s % 4);
+        v = B(lo, 0, 8) | B(hi, 2 * ((s % 16) / 4), 2) << 8;
+        break;
+      }
+      default:                                                       // :239-251
+        v = mc_at(dat, len, off, 4 * s + 2 * h) | mc_at(dat, len, off, 4 * s + 2 * h + 1) << 8;
+    }
+    out[2 * s + h] = v;
+  }
+#undef B
+}
+
+void or_mcraw_blocks(const uint8_t *dat, size_t len, const uint32_t *off, const uint32_t *cnt, uint32_t n,
+                     uint32_t *out)
+{
+  for(uint32_t i = 0; i < n; i++) or_mcraw_block(dat, len, off[i], cnt[i], out + 64 * (size_t)i);
+}
+
+void or_mcraw_decode(const uint8_t *bits, const uint32_t *off, const uint16_t *ref, const uint8_t *dat, size_t len,
+                     int ox, int oy, int owd, int oht, uint16_t *out, int wd, int ht)
+{ // decode.comp:254-338
+  const uint32_t nb = (uint32_t)((uint64_t)owd * oht / 64), bpr = (uint32_t)owd / 64;
+  for(uint32_t idx = 0; idx < nb; idx++)
+  {
+    uint32_t v[64];
+    or_mcraw_block(dat, len, idx ? off[idx - 1] : 0u, mc_count(bits, idx), v);
+    const uint32_t k = idx / 4;
+    for(int s = 0; s < 32; s++) for(int h = 0; h < 2; h++)
+    {
+      // tile k of 64x4; lanes 16..31 on row 2; phase idx&3; lane pairs 2 apart
+      const int x = 64 * (int)(k % bpr) - (s >= 16 ? 64 : 0) + (int)(idx & 1) + 4 * s + 2 * h - ox;
+      const int y = 4 * (int)(k / bpr) + (s >= 16 ? 2 : 0) + (int)((idx >> 1) & 1) - oy;
+      if(x < 0 || y < 0 || x >= wd || y >= ht) continue;
+      const uint32_t r = v[2 * s + h] + ref[idx];
+      // R16 UNORM of r/65536: round(min(r/65536, 1) * 65535) to nearest, ties
+      // to even, in exact arithmetic (double holds r * 65535 exactly)
+      const double f = r >= 65536u ? 65535.0 : (double)r * 65535.0 / 65536.0;
+      out[(size_t)y * wd + x] = (uint16_t)nearbyint(f);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
  * node dispatch
  * --------------------------------------------------------------------- */
 

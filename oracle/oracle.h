@@ -17,6 +17,7 @@
  * R16F/RGBA16F storage images do.
  */
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -116,6 +117,21 @@ typedef struct or_develop_t
   float   hl_white, hl_desat, hl_soft;
 }
 or_develop_t;
+
+// i-mcraw's GPU codec (i-mcraw/prefix.comp, decode.comp).  bits: 4-bit
+// counts, two per byte, low nibble first; or_mcraw_prefix writes the
+// inclusive prefix sum of the n blocks' data sizes; or_mcraw_block decodes
+// one block at byte `off` of dat (len bytes, past them zeros) into its 64
+// offsets from the reference, lane order (value 2s+h of register lane s);
+// or_mcraw_decode writes the ui16 frame (wd x ht, the R16 UNORM store of
+// (offset + ref)/65536) of an owd x oht coded frame shifted by (ox, oy).
+void or_mcraw_prefix(const uint8_t *bits, uint32_t n, uint32_t *off);
+void or_mcraw_block(const uint8_t *dat, size_t len, uint32_t off, uint32_t cnt, uint32_t out[64]);
+// n blocks at once: block i at byte off[i] with count cnt[i] into out[64 i ..]
+void or_mcraw_blocks(const uint8_t *dat, size_t len, const uint32_t *off, const uint32_t *cnt, uint32_t n,
+                     uint32_t *out);
+void or_mcraw_decode(const uint8_t *bits, const uint32_t *off, const uint16_t *ref, const uint8_t *dat, size_t len,
+                     int ox, int oy, int owd, int oht, uint16_t *out, int wd, int ht);
 
 // out_rgba: W*H*4 doubles (rgba f32 output of llap/colour).  stage buffers
 // are optional (may be 0): denoise W*H*1, demosaic/colour/filmcurv W*H*4,

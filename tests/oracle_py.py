@@ -69,6 +69,11 @@ def load(path):
     L.or_set_develop_ties.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.or_set_tie_effect.argtypes = [C.c_double]
     L.or_set_threads.argtypes = [C.c_int]
+    L.or_mcraw_prefix.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    L.or_mcraw_block.argtypes = [C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.or_mcraw_blocks.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+    L.or_mcraw_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
     return L
 
 

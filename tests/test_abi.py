@@ -42,11 +42,12 @@ def test_abi_version_and_registry():
                  ("hilite", "half"), ("hilite", "reduce"), ("hilite", "assemble"), ("hilite", "doub"),
                  ("rawhist", "collect"), ("rawhist", "map"), ("demosaic", "halfsize"), ("shared", "resample"),
                  ("colenc", "main"), ("demosaic", "rcd_conv"), ("demosaic", "rcd_fill"), ("hist", "collect"),
-                 ("hist", "map"), ("autoexp", "hist"), ("autoexp", "lum"), ("autoexp", "exp")]:
+                 ("hist", "map"), ("autoexp", "hist"), ("autoexp", "lum"), ("autoexp", "exp"),
+                 ("i-mcraw", "prefix"), ("i-mcraw", "decode")]:
         assert lib.vkdt_hip_node_supported(tok(m), tok(k)) == 1, (m, k)
     # kernels of modules outside this path are refused by name
     assert lib.vkdt_hip_node_supported(tok("demosaic"), tok("coldumb")) == 0
-    assert lib.vkdt_hip_node_supported(tok("i-mcraw"), tok("decode")) == 0
+    assert lib.vkdt_hip_node_supported(tok("i-vid"), tok("decode")) == 0
 
 
 def test_launch_rejects_bad_nodes_without_touching_the_device():

@@ -3,8 +3,9 @@
 Each case plants one mistake in a copy of oracle/oracle.c (text
 substitution in a temporary directory; the oracle itself is untouched),
 builds it with the oracle's own flags, and runs every check of
-tests/test_oracle_pins_stencils.py and the gain-map checks of
-tests/test_gainmap.py against the build.  A mutant that passes
+tests/test_oracle_pins_stencils.py, the gain-map checks of
+tests/test_gainmap.py and the codec checks of tests/test_mcraw.py against the
+build.  A mutant that passes
 all of them would be a mistake the pins cannot see, so the test requires at
 least one failure per mutant.  The first four are the denoise/demosaic
 mistakes that the closed forms of test_oracle_pins.py alone did not catch.
@@ -17,6 +18,7 @@ import pytest
 
 import oracle_py as op
 import test_gainmap as gain
+import test_mcraw as mcraw
 import test_oracle_pins_stencils as pins
 
 ORACLE = os.path.join(op.ROOT, "oracle")
@@ -56,6 +58,12 @@ MUTANTS = [
      "px = FMIN(FMAX(px * R(map_os[2]), R(0)), R(1));", "noop.comp:50"),
     ("doub_gain_at_the_site", "const real px = (R(0.5) + R((x + pci[12]) / 2)) / R(out->wd / 2);",
      "const real px = (R(0.5) + R(x + pci[12])) / R(out->wd);", "doub.comp:108"),
+    ("mcraw_decode3_high_bit", "if(g == 2) v |= B(16 + 2 * q, 6, 1) << 2;", "if(g == 2) v |= B(16 + 2 * q, 5, 1) << 2;",
+     "i-mcraw/decode.comp:131"),
+    ("mcraw_decode5_shift", "B(24 + 2 * (s - 20), 5, 2) << 3", "B(24 + 2 * (s - 20), 5, 2) << 2",
+     "i-mcraw/decode.comp:172"),
+    ("mcraw_decode10_offset", "B(hi, 2 * ((s % 16) / 4), 2) << 8", "B(hi, 2 * (s / 4), 2) << 8",
+     "i-mcraw/decode.comp:232"),
 ]
 
 
@@ -90,7 +98,7 @@ def test_mutant_fails_a_pin(mutants, mid):
     saved = op._libs.get(64)
     op._libs[64] = lib
     try:
-        for check in pins.CHECKS + gain.CHECKS:
+        for check in pins.CHECKS + gain.CHECKS + mcraw.CHECKS:
             try:
                 check()
             except AssertionError:

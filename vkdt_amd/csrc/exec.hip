@@ -107,6 +107,8 @@ static const reg_entry g_registry[] = {
   { tkn("colenc"),   tkn("main"),     launch_colenc_main },
   { tkn("demosaic"), tkn("rcd_conv"), launch_demosaic_rcd_conv },
   { tkn("demosaic"), tkn("rcd_fill"), launch_demosaic_rcd_fill },
+  { tkn("i-mcraw"),  tkn("prefix"),   launch_mcraw_prefix },
+  { tkn("i-mcraw"),  tkn("decode"),   launch_mcraw_decode },
 };
 
 static launch_fn find(uint64_t module, uint64_t kernel)
@@ -144,7 +146,9 @@ int vkdt_hip_node_launch(uint64_t module, uint64_t kernel, const void *pc, size_
   }
   if(nconn < 0 || (nconn && !conn)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "bad connector list");
   if(wd == 0 || ht == 0) return VKDT_HIP_OK;   // empty dispatch
-  if(wd > (1u << 20) || ht > (1u << 20) || dp > 4096)
+  // node extents: images up to 2^20 texels a side; one-row nodes up to 2^28
+  // (i-mcraw/decode's is blocks x 8, a million for a 4096 x 2160 frame)
+  if(wd > (ht == 1 ? (1u << 28) : (1u << 20)) || ht > (1u << 20) || dp > 4096)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "dispatch %ux%ux%u too large", wd, ht, dp);
   node_call c = { pc, pc_size, params, params_size, conn, nconn, wd, ht, dp ? dp : 1, (hipStream_t)stream };
   return fn(c);

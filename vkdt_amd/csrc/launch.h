@@ -100,3 +100,5 @@ int launch_shared_resample(const node_call &);
 int launch_colenc_main(const node_call &);
 int launch_demosaic_rcd_conv(const node_call &);
 int launch_demosaic_rcd_fill(const node_call &);
+int launch_mcraw_prefix(const node_call &);
+int launch_mcraw_decode(const node_call &);
