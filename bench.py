@@ -16,6 +16,9 @@ per-node graph per image) are reported beside it.
 (module DSOs, one HIP launch per node).  The DNG is decoded and uploaded
 once before timing; a step is one pass of every node over the resident raw.
 
+--mode mcraw: i-mcraw's frame codec (prefix + decode nodes) on one
+4096 x 2160 frame resident in HBM; its own JSON line (frames in Mpix/s).
+
 Prints one JSON line: value is CFA Mpix/s (after i-raw block rounding);
 roofline is for the kernel or stage with the largest share of a step,
 timed with HIP events on the stream it runs on inside the measured run;
@@ -154,6 +157,83 @@ def calibrate_valu(iters=4096, blocks=8192):
         print(json.dumps({"op": name, **res[name]}), flush=True)
     out.free()
     return res
+
+
+# i-mcraw: bits per value stored by each 4-bit block count (counts 7, 9 and
+# 11..15 alias 8, 10 and 16 bits), i-mcraw/prefix.comp:40-55
+MC_WIDTH = [0, 1, 2, 3, 4, 5, 6, 8, 8, 10, 10, 16, 16, 16, 16, 16]
+
+
+def run_mcraw(args):
+    """i-mcraw's frame codec on one 4096 x 2160 frame (the serving workload's
+    video frames), resident in HBM: a step is the prefix node (block offsets
+    from the 4-bit counts) and the decode node (64-value blocks into the ui16
+    mosaic).  Two frames: counts uniform over 2..6 (2-6 bits a value, a
+    noisy 12-bit video frame) and all 16 bits (the codec's worst case).  The
+    roofline is the decode kernel's: algorithmic bytes = counts (n/2) +
+    offsets (4 n) + references (2 n) + the coded data, read, and the frame
+    (2 B a sample), written; n = 138240 blocks."""
+    import numpy as np
+    from vkdt_amd import hip
+    hip.init(0)
+    owd, oht = 4096, 2160
+    nb = owd * oht // 64
+    rblock = -(-nb // 64) * 64
+    res = {}
+    for name, lo, hi in (("counts 2..6", 2, 7), ("16 bit", 11, 12)):
+        rng = np.random.default_rng(7)
+        cnt = rng.integers(lo, hi, size=rblock).astype(np.uint8)
+        cnt[nb:] = 0
+        bits = (cnt[0::2] | (cnt[1::2] << 4)).astype(np.uint8)
+        refs = rng.integers(0, 4096, size=rblock).astype(np.uint16)
+        dlen = int(sum(MC_WIDTH[c] * 8 for c in cnt[:nb]))
+        data = rng.integers(0, 256, size=max(dlen, 1), dtype=np.uint8)
+        b_bits = hip.DeviceBuffer(bits.size, 1, 1, hip.FMT_UI8)
+        b_ref = hip.DeviceBuffer(rblock, 1, 1, hip.FMT_UI16)
+        b_dat = hip.DeviceBuffer(data.size, 1, 1, hip.FMT_UI8)
+        b_off = hip.DeviceBuffer(rblock, 1, 1, hip.FMT_UI32)
+        b_scr = hip.DeviceBuffer((rblock + 511) // 512 + 1, 4, 1, hip.FMT_UI32)
+        b_out = hip.DeviceBuffer(owd, oht, 1, hip.FMT_UI16)
+        b_bits.upload(bits)
+        b_ref.upload(refs)
+        b_dat.upload(data)
+        pc = np.array([0, 0, owd, oht], np.int32)
+        prefix = lambda: hip.launch("i-mcraw", "prefix", None, None, [b_bits, b_off, b_scr], (nb + 511) // 512 * 8, 1)
+        decode = lambda: hip.launch("i-mcraw", "decode", pc, None, [b_out, b_bits, b_off, b_ref, b_dat], nb * 8, 1)
+        for _ in range(args.warmup):
+            prefix()
+            decode()
+        hip.sync()
+        e0, e1, e2 = hip.Event(), hip.Event(), hip.Event()
+        e0.record()
+        for _ in range(args.steps):
+            prefix()
+            decode()
+        e1.record()
+        for _ in range(args.steps):
+            decode()
+        e2.record()
+        hip.sync()
+        ms_step = e0.elapsed_ms(e1) / args.steps
+        ms_dec = e1.elapsed_ms(e2) / args.steps
+        alg = nb // 2 + 4 * nb + 2 * nb + dlen + 2 * owd * oht
+        res[name] = {"ms_per_step": round(ms_step, 4), "decode_ms": round(ms_dec, 4),
+                     "Mpix_s": round(owd * oht / (ms_step * 1e3), 1), "frames_s": round(1e3 / ms_step, 1),
+                     "data_bytes": dlen, "decode_bytes": alg,
+                     "decode_GB_s": round(alg / (ms_dec * 1e6), 1)}
+        for b in (b_bits, b_ref, b_dat, b_off, b_scr, b_out):
+            b.free()
+    typ = res["counts 2..6"]
+    return {
+        "metric": "Mpix/s i-mcraw frame decode (prefix + decode)", "value": typ["Mpix_s"], "unit": "Mpix/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": typ["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16",
+        "data": "synthetic (seeded block counts, references and coded bytes)",
+        "config": {"workload": "i-mcraw 4096x2160 frame, block counts uniform over 2..6", "frames": res},
+        "roofline": {"bound": "hbm", "achieved": typ["decode_GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(typ["decode_GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "i-mcraw/decode"},
+    }
 
 
 def plain_graph(dng, kind, noise_a, noise_b, out_dir, extra=(), hilite=False):
@@ -582,9 +662,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 20 plain, 5 batch)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default: 3 plain, 2 batch)")
-    ap.add_argument("--mode", default="batch", choices=["batch", "plain"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "plain", "mcraw"],
                     help="batch: 64 mixed stills in one batched hipGraph vs the plain path; "
-                         "plain: the per-image path on one still")
+                         "plain: the per-image path on one still; mcraw: i-mcraw's frame codec on one "
+                         "4096x2160 frame")
     ap.add_argument("--kind", default="bayer", choices=["bayer", "xtrans"], help="plain mode: the still's CFA")
     ap.add_argument("--hilite", action="store_true",
                     help="insert hilite between denoise and demosaic (the stock darkroom chain)")
@@ -619,7 +700,9 @@ def main():
         args.steps = 5 if args.mode == "batch" else 20
     if args.warmup is None:
         args.warmup = 2 if args.mode == "batch" else 3
-    if args.mode == "plain":
+    if args.mode == "mcraw":
+        line = run_mcraw(args)
+    elif args.mode == "plain":
         from vkdt_amd import hip
         hip.init(0)
         line = run_plain(args)

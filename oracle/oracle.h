@@ -133,6 +133,12 @@ void or_mcraw_blocks(const uint8_t *dat, size_t len, const uint32_t *off, const 
 void or_mcraw_decode(const uint8_t *bits, const uint32_t *off, const uint16_t *ref, const uint8_t *dat, size_t len,
                      int ox, int oy, int owd, int oht, uint16_t *out, int wd, int ht);
 
+// filmcurv's colour spaces, n triples each way (pins only): which = 0
+// rec2020 -> oklab, 1 oklab -> rec2020, 2 xyY -> dt UCS JCh, 3 JCh -> xyY,
+// 4 xy -> Munsell (hue, chroma, 0), 5 Munsell (hue, chroma) -> xy,
+// 6 rec2020 -> xyY.  Returns -1 for another `which`.
+int or_fc_space(int which, const double *in, double *out, int n);
+
 // out_rgba: W*H*4 doubles (rgba f32 output of llap/colour).  stage buffers
 // are optional (may be 0): denoise W*H*1, demosaic/colour/filmcurv W*H*4,
 // llap W*H*1 (the assembled luminance that llap/colour transfers).

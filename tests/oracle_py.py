@@ -69,6 +69,7 @@ def load(path):
     L.or_set_develop_ties.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.or_set_tie_effect.argtypes = [C.c_double]
     L.or_set_threads.argtypes = [C.c_int]
+    L.or_fc_space.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
     L.or_mcraw_prefix.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
     L.or_mcraw_block.argtypes = [C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32, C.c_void_p]
     L.or_mcraw_blocks.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
@@ -186,3 +187,13 @@ def run_nodes(bits, bufs, nodes, raw, storage_f16=True, sources=None):
         node(bits, n["module"], n["kernel"], n["pc"], n["params"], ob, n["wd"], n["ht"], n["dp"],
              storage_f16=storage_f16)
     return data
+
+
+def fc_space(bits, which, v):
+    """filmcurv's colour space `which` (or_fc_space) on triples v (... x 3)"""
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros_like(v)
+    n = v.size // 3
+    if lib(bits).or_fc_space(which, v.ctypes.data, out.ctypes.data, n) != 0:
+        raise ValueError(f"no colour space {which}")
+    return out

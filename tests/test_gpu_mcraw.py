@@ -4,8 +4,10 @@ offsets and the decode node's ui16 frame equal the oracle's bit for bit
 the one the test-side encoder coded (through the R16 UNORM store of v/65536).
 
 Frames: 4096x2160 (the serving workload's video frames) with a BGGR offset
-(1, 1), blocks of every count including the aliases 7, 9 and 11..15, and a
-small frame whose blocks all use one count."""
+(1, 1) and without one (the decode's 8-byte row stores), blocks of every
+count including the aliases 7, 9 and 11..15, small frames whose blocks all
+use one count, and counts that start off the allocation's 16-byte alignment
+(the scan's byte loads)."""
 import numpy as np
 import pytest
 
@@ -15,17 +17,19 @@ from vkdt_amd import hip as H
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht):
+def run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht, shift=0):
+    """prefix and decode through the C ABI; `shift` bytes move the counts
+    off the allocation's alignment (the scan's byte-load path)"""
     nb = owd * oht // 64
     rblock = -(-nb // 64) * 64                          # i-mcraw/main.cc:371-377
     nbits = (rblock + 1) // 2
-    bb = np.zeros(nbits, np.uint8)
-    bb[:bits.size] = bits
+    bb = np.zeros(nbits + shift, np.uint8)
+    bb[shift:shift + bits.size] = bits
     rr = np.zeros(rblock, np.uint16)
     rr[:refs.size] = refs
     dd = np.zeros(max(2 * owd * oht, data.size), np.uint8)
     dd[:data.size] = data
-    b_bits = hip.DeviceBuffer(nbits, 1, 1, H.FMT_UI8)
+    b_bits = hip.DeviceBuffer(nbits + shift, 1, 1, H.FMT_UI8)
     b_ref = hip.DeviceBuffer(rblock, 1, 1, H.FMT_UI16)
     b_dat = hip.DeviceBuffer(dd.size, 1, 1, H.FMT_UI8)
     b_off = hip.DeviceBuffer(rblock, 1, 1, H.FMT_UI32)
@@ -36,9 +40,10 @@ def run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht):
         b_ref.upload(rr)
         b_dat.upload(dd)
         b_scr.clear()
-        hip.launch("i-mcraw", "prefix", None, None, [b_bits, b_off, b_scr], (nb + 511) // 512 * 8, 1)
+        cnt = H.Buf(b_bits.ptr + shift, nbits, 1, nbits, 1, H.FMT_UI8, 1, nbits)
+        hip.launch("i-mcraw", "prefix", None, None, [cnt, b_off, b_scr], (nb + 511) // 512 * 8, 1)
         pc = np.array([ox, oy, owd, oht], np.int32)
-        hip.launch("i-mcraw", "decode", pc, None, [b_out, b_bits, b_off, b_ref, b_dat], nb * 8, 1)
+        hip.launch("i-mcraw", "decode", pc, None, [b_out, cnt, b_off, b_ref, b_dat], nb * 8, 1)
         hip.sync()
         return b_off.download()[0], b_out.download()
     finally:
@@ -46,9 +51,12 @@ def run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht):
             b.free()
 
 
-@pytest.mark.parametrize("owd,oht,ox,oy,single", [(4096, 2160, 1, 1, None), (256, 32, 0, 1, 3), (192, 16, 1, 0, 10)],
-                         ids=["4096x2160-bggr", "256x32-count3", "192x16-count10"])
-def test_decode_equals_oracle_and_the_coded_frame(hip, owd, oht, ox, oy, single):
+@pytest.mark.parametrize("owd,oht,ox,oy,single,shift",
+                         [(4096, 2160, 1, 1, None, 0), (4096, 2160, 0, 0, None, 0), (256, 32, 0, 1, 3, 0),
+                          (192, 16, 1, 0, 10, 0), (1024, 72, 0, 0, None, 4)],
+                         ids=["4096x2160-bggr", "4096x2160-rggb", "256x32-count3", "192x16-count10",
+                              "1024x72-unaligned-counts"])
+def test_decode_equals_oracle_and_the_coded_frame(hip, owd, oht, ox, oy, single, shift):
     rng = np.random.default_rng(owd + oht)
     nb = owd * oht // 64
     widths = np.array([0, 1, 2, 3, 4, 5, 6, 8, 10, 16])
@@ -62,7 +70,7 @@ def test_decode_equals_oracle_and_the_coded_frame(hip, owd, oht, ox, oy, single)
     bits, refs, data = mc.encode(coded, counts=lambda i, c: np.where(i % 4 == 1, alias[c], c))
     wd, ht = ((owd - ox) // 2) * 2, ((oht - oy) // 2) * 2
     print(f"[gpu_mcraw] {owd}x{oht} offset ({ox},{oy}) -> {wd}x{ht}: {nb} blocks, {data.nbytes} bytes", flush=True)
-    off_gpu, out_gpu = run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht)
+    off_gpu, out_gpu = run_gpu(hip, bits, refs, data, ox, oy, owd, oht, wd, ht, shift)
     off = mc.oracle_prefix(bits, nb)
     assert np.array_equal(off_gpu[:nb], off)
     ref = mc.oracle_decode(bits, off, refs, data, ox, oy, owd, oht, wd, ht)

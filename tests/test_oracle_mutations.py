@@ -4,7 +4,8 @@ Each case plants one mistake in a copy of oracle/oracle.c (text
 substitution in a temporary directory; the oracle itself is untouched),
 builds it with the oracle's own flags, and runs every check of
 tests/test_oracle_pins_stencils.py, the gain-map checks of
-tests/test_gainmap.py and the codec checks of tests/test_mcraw.py against the
+tests/test_gainmap.py, the codec checks of tests/test_mcraw.py and the
+filmcurv colour-mode checks of tests/test_filmcurv_modes.py against the
 build.  A mutant that passes
 all of them would be a mistake the pins cannot see, so the test requires at
 least one failure per mutant.  The first four are the denoise/demosaic
@@ -17,6 +18,7 @@ import subprocess
 import pytest
 
 import oracle_py as op
+import test_filmcurv_modes as film
 import test_gainmap as gain
 import test_mcraw as mcraw
 import test_oracle_pins_stencils as pins
@@ -64,6 +66,20 @@ MUTANTS = [
      "i-mcraw/decode.comp:172"),
     ("mcraw_decode10_offset", "B(hi, 2 * ((s % 16) / 4), 2) << 8", "B(hi, 2 * (s / 4), 2) << 8",
      "i-mcraw/decode.comp:232"),
+    ("oklab_m1_by_rows", "col3(lms, M, rgb);", "row3(lms, M, rgb);", "colourspaces.glsl:23-27"),
+    ("ucs_mix_transposed",
+     "const real U = R(-1.124983854323892) * us + R(-0.980483721769325) * vs;\n"
+     "  const real V = R(1.86323315098672) * us + R(1.971853092390862) * vs;",
+     "const real U = R(-1.124983854323892) * us + R(1.86323315098672) * vs;\n"
+     "  const real V = R(-0.980483721769325) * us + R(1.971853092390862) * vs;", "dtucs.glsl:35-36"),
+    ("agx_inset_by_rows", "col3(c, agx, c0);", "row3(c, agx, c0);", "filmcurv/main.comp:33-43"),
+    ("munsell_triangle_flipped", "xy[k] = hu >= cu ?", "xy[k] = hu <= cu ?", "munsell.glsl:58"),
+    ("lum_ratio_weights_swapped", "R(0.2627) * c0[0] + R(0.6780) * c0[1]", "R(0.6780) * c0[0] + R(0.2627) * c0[1]",
+     "filmcurv/main.comp:125-126"),
+    ("ucs_mode_keeps_curve_hue", "j1[2] = j0[2];", "j1[2] = j1[2];", "filmcurv/main.comp:91"),
+    ("agx_hue_mix_0.6", "lerp_hue(h0[0], h1[0], R(0.4))", "lerp_hue(h0[0], h1[0], R(0.6))", "filmcurv/main.comp:46"),
+    ("lum_ratio_square_root", "L0 * POW(lum1 / lum0, R(1.0) / R(3.0))", "L0 * POW(lum1 / lum0, R(1.0) / R(2.0))",
+     "filmcurv/main.comp:135"),
 ]
 
 
@@ -77,7 +93,7 @@ def build_mutant(tmp, mid, old, new):
     so = os.path.join(d, "liboracle64.so")
     # the oracle's own flags (oracle/Makefile), double build; jpeg.c is not needed by the checks
     r = subprocess.run(["gcc", "-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fopenmp", "-shared",
-                        "-I" + ORACLE, "-o", so, os.path.join(d, "oracle.c"), os.path.join(ORACLE, "jpeg.c"), "-lm"],
+                        "-I" + ORACLE, "-I" + os.path.join(op.ROOT, "vkdt_amd", "csrc"), "-o", so, os.path.join(d, "oracle.c"), os.path.join(ORACLE, "jpeg.c"), "-lm"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return so
@@ -98,7 +114,7 @@ def test_mutant_fails_a_pin(mutants, mid):
     saved = op._libs.get(64)
     op._libs[64] = lib
     try:
-        for check in pins.CHECKS + gain.CHECKS + mcraw.CHECKS:
+        for check in pins.CHECKS + gain.CHECKS + mcraw.CHECKS + film.CHECKS:
             try:
                 check()
             except AssertionError:

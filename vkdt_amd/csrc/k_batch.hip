@@ -439,7 +439,7 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
       const int xq = q.x, y = q.y;
       const float4 f = dm_fix_r<XT>(mosl, cov, r, xq, y, green);
       const float4 c = r16(co_px(r16(f), co));
-      st_h4(film, xq, y, fc_px(c, fc));
+      st_h4(film, xq, y, fc_px_basic(c, fc));
     }
     return;
   }
@@ -456,7 +456,7 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
     if(x >= W || y >= H) continue;
     const float4 f = dm_fix_px<XT>(mos, cov, r, x, y, green);
     const float4 c = r16(co_px(r16(f), co));
-    st_h4(film, x, y, fc_px(c, fc));
+    st_h4(film, x, y, fc_px_basic(c, fc));
   }
 }
 
@@ -846,7 +846,7 @@ static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m
   for(int k = 0; k < 4; k++) b.dn_doub.crop[k] = d.crop[k];
   if(const char *err = co_args_init(b.co, d.co_params))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: colour: %s", idx, err);
-  if(const char *err = fc_args_init(b.fc, d.fc_params))
+  if(const char *err = fc_args_init(b.fc, d.fc_params, sizeof(d.fc_params), true))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "batch image %d: filmcurv: %s", idx, err);
   ll_args_init(b.ll, d.ll_params, m.ng);
   return VKDT_HIP_OK;

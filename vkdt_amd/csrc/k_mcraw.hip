@@ -37,9 +37,10 @@ DEV uint32_t mc_count(const uint8_t *bits, uint32_t i)
 }
 
 // ---- prefix: inclusive prefix sum of the block sizes ----------------------
-// three passes without inter-group waits: 256 threads x 16 blocks per group
-// scan locally and publish their total; one group scans the totals; every
-// group adds its exclusive total back
+// two launches, no waits between groups: k_mc_part_sums writes the data
+// bytes of each part of 4096 blocks into the scratch connector; k_mc_prefix
+// scans its part locally and adds the sum of the parts before it (at most a
+// few dozen words, read by every group)
 
 #define MC_ROWS 16
 #define MC_WG 256
@@ -60,65 +61,90 @@ DEV uint32_t block_inclusive_scan(uint32_t v, uint32_t *sh)
   return sh[t];
 }
 
-__global__ void __launch_bounds__(MC_WG) k_mc_scan_local(const uint8_t *bits, uint32_t *off, uint32_t *part, uint32_t n)
+// this thread's 16 blocks of the part: inclusive sums of their data bytes
+// (vec: the counts are 8-byte aligned for one load)
+DEV uint32_t mc_thread_blocks(const uint8_t *bits, uint32_t base, uint32_t n, int vec, uint32_t loc[MC_ROWS])
 {
-  __shared__ uint32_t sh[MC_WG];
-  const uint32_t base = blockIdx.x * MC_PART + threadIdx.x * MC_ROWS;
-  uint32_t loc[MC_ROWS], acc = 0;
-#pragma unroll
-  for(int r = 0; r < MC_ROWS; r++)
+  uint32_t acc = 0;
+  if(vec && base + MC_ROWS <= n)
   {
-    const uint32_t i = base + r;
-    acc += i < n ? mc_block_bytes(mc_count(bits, i)) : 0u;
-    loc[r] = acc;
-  }
-  const uint32_t incl = block_inclusive_scan(acc, sh);
-  const uint32_t excl = incl - acc;
+    const uint2 q = *(const uint2 *)(bits + base / 2);
 #pragma unroll
-  for(int r = 0; r < MC_ROWS; r++)
-    if(base + r < n) off[base + r] = loc[r] + excl;
-  if(threadIdx.x == MC_WG - 1) part[blockIdx.x] = incl;
+    for(int r = 0; r < MC_ROWS; r++)
+    {
+      acc += mc_block_bytes(((r < 8 ? q.x : q.y) >> (4 * (r & 7))) & 15u);
+      loc[r] = acc;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for(int r = 0; r < MC_ROWS; r++)
+    {
+      const uint32_t i = base + r;
+      acc += i < n ? mc_block_bytes(mc_count(bits, i)) : 0u;
+      loc[r] = acc;
+    }
+  }
+  return acc;
 }
 
-__global__ void __launch_bounds__(MC_WG) k_mc_scan_parts(uint32_t *part, uint32_t np)
-{ // in place: group totals -> exclusive prefix of the totals
+__global__ void __launch_bounds__(MC_WG) k_mc_part_sums(const uint8_t *bits, uint32_t *part, uint32_t n, int vec)
+{
   __shared__ uint32_t sh[MC_WG];
+  uint32_t loc[MC_ROWS];
+  const uint32_t acc = mc_thread_blocks(bits, blockIdx.x * MC_PART + threadIdx.x * MC_ROWS, n, vec, loc);
+  block_inclusive_scan(acc, sh);
+  if(threadIdx.x == MC_WG - 1) part[blockIdx.x] = sh[MC_WG - 1];
+}
+
+// vec: bits 8-byte and off 16-byte aligned (vector loads and stores)
+__global__ void __launch_bounds__(MC_WG) k_mc_prefix(const uint8_t *bits, uint32_t *off, const uint32_t *part,
+                                                     uint32_t n, int vec)
+{
+  __shared__ uint32_t sh[MC_WG];
+  // the parts before this one
   uint32_t carry = 0;
-  for(uint32_t c0 = 0; c0 < np; c0 += MC_WG)
+  for(uint32_t j = threadIdx.x; j < blockIdx.x; j += MC_WG) carry += part[j];
+  block_inclusive_scan(carry, sh);
+  carry = sh[MC_WG - 1];
+  __syncthreads();
+  const uint32_t base = blockIdx.x * MC_PART + threadIdx.x * MC_ROWS;
+  uint32_t loc[MC_ROWS];
+  const uint32_t acc = mc_thread_blocks(bits, base, n, vec, loc);
+  const uint32_t excl = block_inclusive_scan(acc, sh) - acc + carry;
+  if(vec && base + MC_ROWS <= n)
   {
-    const uint32_t i = c0 + threadIdx.x;
-    const uint32_t v = i < np ? part[i] : 0u;
-    const uint32_t incl = block_inclusive_scan(v, sh);
-    const uint32_t tot = sh[MC_WG - 1];
-    if(i < np) part[i] = carry + incl - v;
-    carry += tot;
-    __syncthreads();
+    uint4 *o = (uint4 *)(off + base);
+#pragma unroll
+    for(int r = 0; r < MC_ROWS; r += 4)
+      o[r / 4] = make_uint4(loc[r] + excl, loc[r + 1] + excl, loc[r + 2] + excl, loc[r + 3] + excl);
   }
-}
-
-__global__ void __launch_bounds__(MC_WG) k_mc_scan_add(uint32_t *off, const uint32_t *part, uint32_t n)
-{
-  if(blockIdx.x == 0) return;
-  const uint32_t add = part[blockIdx.x];
-  const uint32_t base = blockIdx.x * MC_PART;
-  for(uint32_t r = threadIdx.x; r < MC_PART; r += MC_WG)
-    if(base + r < n) off[base + r] += add;
+  else
+  {
+#pragma unroll
+    for(int r = 0; r < MC_ROWS; r++)
+      if(base + r < n) off[base + r] = loc[r] + excl;
+  }
 }
 
 // ---- decode ------------------------------------------------------------------
-// A block is decoded by 32 lanes (half a wave); lane s yields the values of
-// two adjacent register lanes, the pair read from bytes `reg` and `reg + 1`
-// of the block at the same bit offset.  Bytes past the data buffer read as 0.
-
+// The reference's decoder reads the values of two adjacent register lanes
+// 2s and 2s + 1 as a pair, from bytes `reg` and `reg + 1` of the block at the
+// same bit offset.  Here a block is decoded by a whole wave, one value a
+// lane: h = 0 the pair's first (byte reg), h = 1 its second (byte reg + 1).
+// The decoders below are written for the pair and return the lane's value
+// in both components.  Bytes past the data buffer read as 0.
 struct mc_block
 {
   const uint8_t *d;
   uint32_t off, len;
+  int h;
   __device__ uint32_t at(int r) const { return off + (uint32_t)r < len ? d[off + r] : 0u; }
   __device__ uint2 pair(int reg, int bit, int n) const
   {
-    const uint32_t m = (1u << n) - 1u;
-    return make_uint2((at(reg) >> bit) & m, (at(reg + 1) >> bit) & m);
+    const uint32_t m = (1u << n) - 1u, v = (at(reg + h) >> bit) & m;
+    return make_uint2(v, v);
   }
 };
 
@@ -170,19 +196,21 @@ DEV uint2 mc_decode6(const mc_block &b, int s)
   return b.pair(r0 + 2 * q, 6, 2) | shl(b.pair(r0 + 8 + 2 * q, 6, 2), 2) | shl(b.pair(r0 + 16 + 2 * q, 6, 2), 4);
 }
 
-DEV uint2 mc_decode8(const mc_block &b, int s) { return make_uint2(b.at(2 * s), b.at(2 * s + 1)); }
+DEV uint2 mc_decode8(const mc_block &b, int s) { const uint32_t v = b.at(2 * s + b.h); return make_uint2(v, v); }
 
 DEV uint2 mc_decode10(const mc_block &b, int s)
 { // eight low bits from bytes 0..31 / 40..71, two high bits from 32..39 / 72..79
   const int rl = s < 16 ? 2 * s : 40 + 2 * (s - 16);
   const int rh = (s < 16 ? 32 : 72) + 2 * (s % 4);
   const int off = 2 * ((s & 15) / 4);
-  return make_uint2(b.at(rl), b.at(rl + 1)) | shl(b.pair(rh, off, 2), 8);
+  const uint32_t lo = b.at(rl + b.h);
+  return make_uint2(lo, lo) | shl(b.pair(rh, off, 2), 8);
 }
 
 DEV uint2 mc_decode16(const mc_block &b, int s)
 { // little-endian 16-bit values
-  return make_uint2(b.at(4 * s) | (b.at(4 * s + 1) << 8), b.at(4 * s + 2) | (b.at(4 * s + 3) << 8));
+  const uint32_t v = b.at(4 * s + 2 * b.h) | (b.at(4 * s + 2 * b.h + 1) << 8);
+  return make_uint2(v, v);
 }
 
 DEV uint2 mc_decode(const mc_block &b, uint32_t cnt, int s)
@@ -209,32 +237,58 @@ struct mc_args { int ox, oy, owd, oht; uint32_t nblocks, data_len; };
 // v/65536 >= 1 saturates
 DEV uint32_t mc_unorm(uint32_t v) { return v >= 65536u ? 65535u : (v <= 32768u ? v : v - 1u); }
 
-DEV void mc_store(const dimg &out, int x, int y, uint32_t v)
+// A wave decodes one 64x4 tile, its four blocks 4k..4k+3 in turn, so that
+// every lane of the wave takes the same count's path (decode.comp:314-337):
+// block 4k+q holds CFA phase (q&1, q>>1) of rows 0 and 2; its register lane
+// 2s+h (lane 2s+h of the wave, s < 16 on row 0, s >= 16 on row 2) lands at
+// column 4(s%16) + 2h.  So each lane ends with two adjacent columns of two
+// rows and stores each row as one 4-byte store where the destination is
+// aligned and inside the image.  The loads of the four blocks' counts,
+// offsets and references are issued before their data loads.
+__global__ void __launch_bounds__(256) k_mc_decode(dimg out, const uint8_t *__restrict__ bits,
+                                                   const uint32_t *__restrict__ off, const uint16_t *__restrict__ ref,
+                                                   const uint8_t *__restrict__ dat, mc_args a)
 {
-  if(x < 0 || y < 0 || x >= out.wd || y >= out.ht) return;
-  ((uint16_t *)out.p)[(long)y * out.pitch + x] = (uint16_t)mc_unorm(v);
-}
-
-__global__ void __launch_bounds__(256) k_mc_decode(dimg out, const uint8_t *bits, const uint32_t *off,
-                                                   const uint16_t *ref, const uint8_t *dat, mc_args a)
-{
-  const uint32_t idx = blockIdx.x * 8 + threadIdx.x / 32;   // two blocks per wave
-  if(idx >= a.nblocks) return;
-  const int s = threadIdx.x % 32;
-  const uint32_t cnt = mc_count(bits, idx);
-  const mc_block b = { dat, idx > 0 ? off[idx - 1] : 0u, a.data_len };
-  uint2 res = mc_decode(b, cnt, s);
-  res.x += ref[idx];
-  res.y += ref[idx];
-  // decode.comp:314-337: tile k = idx/4 of 64x4, phase idx&3, lanes 16..31 on row 2
-  const uint32_t k = idx / 4, bpr = (uint32_t)a.owd / 64;
-  int px = 64 * (int)(k % bpr), py = 4 * (int)(k / bpr);
-  if(s >= 16) { px -= 64; py += 2; }
-  px += (idx & 1);
-  py += (idx >> 1) & 1;
-  px += 4 * s;
-  mc_store(out, px - a.ox, py - a.oy, res.x);
-  mc_store(out, px + 2 - a.ox, py - a.oy, res.y);
+  const uint32_t k = blockIdx.x * 4 + threadIdx.x / 64;
+  const uint32_t b0 = 4 * k;
+  if(b0 >= a.nblocks) return;
+  const int l = threadIdx.x % 64, s = l / 2, h = l % 2;
+  const int nq = (int)min(4u, a.nblocks - b0);
+  uint32_t cnt[4], start[4], rv[4];
+#pragma unroll
+  for(int q = 0; q < 4; q++)
+  {
+    const uint32_t i = b0 + q;
+    const bool have = q < nq;
+    cnt[q] = have ? mc_count(bits, i) : 0u;
+    start[q] = have && i > 0 ? off[i - 1] : 0u;
+    rv[q] = have ? ref[i] : 0u;
+  }
+  uint32_t v[4];
+#pragma unroll
+  for(int q = 0; q < 4; q++)
+  {
+    const mc_block b = { dat, start[q], a.data_len, h };
+    v[q] = q < nq ? mc_unorm(mc_decode(b, cnt[q], s).x + rv[q]) : 0u;
+  }
+  const uint32_t bpr = (uint32_t)a.owd / 64;
+  const int x = 64 * (int)(k % bpr) + 4 * (s % 16) + 2 * h - a.ox;
+  const int y = 4 * (int)(k / bpr) + (s >= 16 ? 2 : 0) - a.oy;
+  uint16_t *o = (uint16_t *)out.p;
+#pragma unroll
+  for(int r = 0; r < 2; r++)
+  { // row y + r: phase 2r at column x, phase 2r + 1 at x + 1
+    const int yy = y + r;
+    if(yy < 0 || yy >= out.ht) continue;
+    const long at = (long)yy * out.pitch + x;
+    if(2 * r + 1 < nq && x >= 0 && x + 1 < out.wd && ((((uintptr_t)o) >> 1) + at) % 2 == 0)
+      *(uint32_t *)(o + at) = v[2 * r] | (v[2 * r + 1] << 16);
+    else
+    {
+      if(2 * r < nq && x >= 0 && x < out.wd) o[at] = (uint16_t)v[2 * r];
+      if(2 * r + 1 < nq && x + 1 >= 0 && x + 1 < out.wd) o[at + 1] = (uint16_t)v[2 * r + 1];
+    }
+  }
 }
 
 // ---- launchers ---------------------------------------------------------------
@@ -256,10 +310,11 @@ int launch_mcraw_prefix(const node_call &c)
   if(conn_elems(c.conn[2]) < np)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "i-mcraw/prefix: scratch holds %zu of %u partials",
                           conn_elems(c.conn[2]), np);
+  const uint8_t *bits = (const uint8_t *)c.conn[0].ptr;
   uint32_t *off = (uint32_t *)c.conn[1].ptr, *part = (uint32_t *)c.conn[2].ptr;
-  hipLaunchKernelGGL(k_mc_scan_local, dim3(np), dim3(MC_WG), 0, c.stream, (const uint8_t *)c.conn[0].ptr, off, part, n);
-  hipLaunchKernelGGL(k_mc_scan_parts, dim3(1), dim3(MC_WG), 0, c.stream, part, np);
-  hipLaunchKernelGGL(k_mc_scan_add, dim3(np), dim3(MC_WG), 0, c.stream, off, (const uint32_t *)part, n);
+  const int vec = ((uintptr_t)bits % 8 == 0) && ((uintptr_t)off % 16 == 0);
+  hipLaunchKernelGGL(k_mc_part_sums, dim3(np), dim3(MC_WG), 0, c.stream, bits, part, n, vec);
+  hipLaunchKernelGGL(k_mc_prefix, dim3(np), dim3(MC_WG), 0, c.stream, bits, off, (const uint32_t *)part, n, vec);
   return launch_post("i-mcraw/prefix");
 }
 
@@ -280,7 +335,8 @@ int launch_mcraw_decode(const node_call &c)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "i-mcraw/decode: %u blocks exceed the bit count, offset or "
                           "reference buffers", a.nblocks);
   a.data_len = (uint32_t)std::min<size_t>(conn_elems(c.conn[4]), 0xffffffffu);
-  hipLaunchKernelGGL(k_mc_decode, dim3((a.nblocks + 7) / 8), dim3(256), 0, c.stream, to_dimg(c.conn[0]),
+  const uint32_t ntiles = (a.nblocks + 3) / 4;
+  hipLaunchKernelGGL(k_mc_decode, dim3((ntiles + 3) / 4), dim3(256), 0, c.stream, to_dimg(c.conn[0]),
                      (const uint8_t *)c.conn[1].ptr, (const uint32_t *)c.conn[2].ptr, (const uint16_t *)c.conn[3].ptr,
                      (const uint8_t *)c.conn[4].ptr, a);
   return launch_post("i-mcraw/decode");

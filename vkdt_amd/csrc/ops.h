@@ -728,13 +728,6 @@ DEV float4 co_px(float4 t, const co_args &a)
                      clampf(rgb[2] * a.expo, -65535.0f, 65535.0f), 1.0f);
 }
 
-struct fc_args { float il, k, bias; int mode; };
-
-DEV float weibull(float x, float il, float k)
-{ // filmcurv/params.glsl:16-20
-  return 1.0f - fexp(-fpow(fmaxf(x, 1e-7f) * il, k));
-}
-
 // shared.glsl:371-387
 DEV void adjust_colour_dng(float c0[3], float c1[3])
 {
@@ -749,13 +742,8 @@ DEV void adjust_colour_dng(float c0[3], float c1[3])
   if(fx) { t = c1[2]; c1[2] = c1[1]; c1[1] = t; }
 }
 
-DEV float4 fc_px(float4 t, const fc_args &a)
-{
-  float c0[3] = { t.x + a.bias, t.y + a.bias, t.z + a.bias };
-  float c1[3] = { weibull(c0[0], a.il, a.k), weibull(c0[1], a.il, a.k), weibull(c0[2], a.il, a.k) };
-  if(a.mode == 3) adjust_colour_dng(c0, c1);
-  return make_float4(c1[0], c1[1], c1[2], 1.0f);
-}
+// filmcurv: fc_args, the per-channel and colour modes, fc_args_init
+#include "ops_filmcurv.h"
 
 // host: colour's committed params (colour/params, 242 words) to the matrix
 // path's arguments.  Returns an error text for the paths not implemented
@@ -795,18 +783,6 @@ static inline const char *co_args_init(co_args &a, const void *params)
     a.ratio[j] = dst / src;
   }
   a.expo = p[3];
-  return 0;
-}
-
-// host: filmcurv's params (light, contrast, bias, colour)
-static inline const char *fc_args_init(fc_args &a, const void *params)
-{
-  const float *p = (const float *)params;
-  a.il = fmaxf(5e-3f, p[0]);
-  a.k = fmaxf(1e-4f, p[1]);
-  a.bias = p[2];
-  a.mode = ((const int32_t *)params)[3];
-  if(a.mode != 1 && a.mode != 3) return "colour mode not supported (1 and 3 are)";
   return 0;
 }
 

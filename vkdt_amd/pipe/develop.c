@@ -117,6 +117,10 @@ int dt_graph_develop_image(dt_graph_t *g, vkdt_develop_image_t *d)
   memcpy(d->co_params, p, sizeof(d->co_params));
   p = node_params(fc, &sz);
   if(sz < 16) return fail(g, "filmcurv params");
+  // the colour modes 0, 2, 4 and 5 run on the per-node filmcurv only
+  const int32_t fc_mode = ((const int32_t *)p)[3];
+  if(fc_mode != 1 && fc_mode != 3)
+    return fail(g, "the batched path runs filmcurv colour modes 1 and 3 (the per-node graph runs all)");
   memcpy(d->fc_params, p, 16);
   p = node_params(curve, &sz);
   if(sz < 16) return fail(g, "llap params");
