@@ -209,6 +209,17 @@ int vkdt_hip_unpack_msb16(const uint8_t *packed, size_t bytes, void *out, uint32
 // (4:2:0).  *out (malloc'd, free with vkdt_jpg_free) gets the whole file.
 int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, uint32_t pitch, int quality, int hs, int vs,
                         uint8_t **out, size_t *out_len, void *stream);
+// vkdt_hip_jpg_encode with the rest of o-jpg/main.c:142-156: libjpeg's
+// dct_method (VKDT_JPG_DCT_*, JDCT_ISLOW / JDCT_IFAST / JDCT_FLOAT),
+// smoothing_factor (0..100; full-size components and 2x2 chroma) and an ICC
+// profile written in APP2 markers after the JFIF header (write_icc_profile,
+// main.c:36-93; none when icc_len is 0)
+#define VKDT_JPG_DCT_ISLOW 0
+#define VKDT_JPG_DCT_IFAST 1
+#define VKDT_JPG_DCT_FLOAT 2
+int vkdt_hip_jpg_encode_ex(const void *rgba, uint32_t wd, uint32_t ht, uint32_t pitch, int quality, int hs, int vs,
+                           int dct, int smoothing, const uint8_t *icc, size_t icc_len, uint8_t **out, size_t *out_len,
+                           void *stream);
 void vkdt_jpg_free(uint8_t *p);
 // IJG quality scaling of the T.81 Annex K tables (natural order), as
 // jpeg_set_quality(quality, force_baseline) computes them

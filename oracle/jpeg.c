@@ -11,10 +11,18 @@
  *               samples replicated past the edges; blocks of the last MCUs
  *               beyond a component's own block extent are dummies (zero, the
  *               DC of the block to the left or of the row above's last one)
+ *   smoothing   libjpeg's smoothing_factor filter (or_jpg_encode_ex): full-size
+ *               components 3x3, 2x2-subsampled chroma 4x4 (members, edge
+ *               and corner neighbours), 16-bit fixed point
  *   dct         the "islow" integer DCT: Loeffler-Ligtenberg-Moschytz with
- *               13-bit constants, 2 extra bits after the row pass
+ *               13-bit constants, 2 extra bits after the row pass; or
+ *               (or_jpg_encode_ex) the "ifast" and "float" scaled DCTs of
+ *               Arai, Agui and Nakajima
  *   quant       IJG quality scaling of the Annex K tables, coefficients
- *               divided by 8 q with rounding half away from zero
+ *               divided by 8 q with rounding half away from zero (islow),
+ *               by 8 q times the AAN scale factors (ifast), or multiplied by
+ *               that divisor's float reciprocal and rounded to nearest even
+ *               (float)
  *   huffman     optimal tables per image: Annex K.2 code lengths (the
  *               smallest frequencies merged first, ties to the larger
  *               symbol, a reserved symbol 256), K.3 length limit 16, values
@@ -24,6 +32,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -98,6 +107,95 @@ void or_jpg_fdct(int32_t *d)
       p[3 * s] = DESC(b6 + a2 + a3, sh);
       p[s] = DESC(b7 + a1 + a4, sh);
     }
+}
+
+// The other two DCTs libjpeg offers (dct_method JDCT_IFAST and JDCT_FLOAT):
+// the scaled DCT of Arai, Agui and Nakajima (one flow graph, 5 multiplies a
+// row pass), whose outputs carry the per-frequency factors
+// s_u s_v 8, s_0 = 1, s_k = sqrt(2) cos(k pi/16), which the quantiser
+// divides out.  ifast: 8 fractional bits, products shifted right without
+// rounding, no extra precision between the passes.  float: single precision
+// in the same order.
+static const double aan_s[8] = { 1.0, 1.387039845, 1.306562965, 1.175875602, 1.0, 0.785694958, 0.541196100,
+                                 0.275899379 };
+
+#define AAN8(v, c) ((int32_t)(((v) * (c)) >> 8))
+void or_jpg_fdct_ifast(int32_t *d)
+{
+  for(int pass = 0; pass < 2; pass++)
+    for(int k = 0; k < 8; k++)
+    {
+      int32_t *p = pass == 0 ? d + 8 * k : d + k;
+      const int s = pass == 0 ? 1 : 8;
+      const int32_t a0 = p[0] + p[7 * s], a7 = p[0] - p[7 * s], a1 = p[s] + p[6 * s], a6 = p[s] - p[6 * s];
+      const int32_t a2 = p[2 * s] + p[5 * s], a5 = p[2 * s] - p[5 * s], a3 = p[3 * s] + p[4 * s], a4 = p[3 * s] - p[4 * s];
+      // even half
+      const int32_t e0 = a0 + a3, e3 = a0 - a3, e1 = a1 + a2, e2 = a1 - a2;
+      p[0] = e0 + e1;
+      p[4 * s] = e0 - e1;
+      const int32_t r = AAN8(e2 + e3, 181);                     // cos(pi/4)
+      p[2 * s] = e3 + r;
+      p[6 * s] = e3 - r;
+      // odd half
+      const int32_t o0 = a4 + a5, o1 = a5 + a6, o2 = a6 + a7;
+      const int32_t rot = AAN8(o0 - o2, 98);                    // cos(3pi/8)
+      const int32_t q2 = AAN8(o0, 139) + rot;                   // cos(pi/8) - cos(3pi/8)
+      const int32_t q4 = AAN8(o2, 334) + rot;                   // cos(pi/8) + cos(3pi/8)
+      const int32_t q3 = AAN8(o1, 181);
+      const int32_t h1 = a7 + q3, h3 = a7 - q3;
+      p[5 * s] = h3 + q2;
+      p[3 * s] = h3 - q2;
+      p[s] = h1 + q4;
+      p[7 * s] = h1 - q4;
+    }
+}
+
+void or_jpg_fdct_float(float *d)
+{
+  for(int pass = 0; pass < 2; pass++)
+    for(int k = 0; k < 8; k++)
+    {
+      float *p = pass == 0 ? d + 8 * k : d + k;
+      const int s = pass == 0 ? 1 : 8;
+      const float a0 = p[0] + p[7 * s], a7 = p[0] - p[7 * s], a1 = p[s] + p[6 * s], a6 = p[s] - p[6 * s];
+      const float a2 = p[2 * s] + p[5 * s], a5 = p[2 * s] - p[5 * s], a3 = p[3 * s] + p[4 * s], a4 = p[3 * s] - p[4 * s];
+      const float e0 = a0 + a3, e3 = a0 - a3, e1 = a1 + a2, e2 = a1 - a2;
+      p[0] = e0 + e1;
+      p[4 * s] = e0 - e1;
+      const float r = (e2 + e3) * 0.707106781f;
+      p[2 * s] = e3 + r;
+      p[6 * s] = e3 - r;
+      const float o0 = a4 + a5, o1 = a5 + a6, o2 = a6 + a7;
+      const float rot = (o0 - o2) * 0.382683433f;
+      const float q2 = 0.541196100f * o0 + rot;
+      const float q4 = 1.306562965f * o2 + rot;
+      const float q3 = o1 * 0.707106781f;
+      const float h1 = a7 + q3, h3 = a7 - q3;
+      p[5 * s] = h3 + q2;
+      p[3 * s] = h3 - q2;
+      p[s] = h1 + q4;
+      p[7 * s] = h1 - q4;
+    }
+}
+
+// divisors of the DCT methods for quantiser q at natural index i:
+// islow 8 q; ifast 8 q s_u s_v in integer (14-bit factors, rounded to
+// 3 fractional bits); float the reciprocal of 8 q s_u s_v
+int or_jpg_divisor_ifast(int q, int i)
+{
+  const int32_t f = (int32_t)(16384.0 * aan_s[i / 8] * aan_s[i % 8] + 0.5);
+  return (int)(((int64_t)q * f + (1 << 10)) >> 11);
+}
+float or_jpg_divisor_float(int q, int i)
+{
+  return (float)(1.0 / ((double)q * aan_s[i / 8] * aan_s[i % 8] * 8.0));
+}
+
+// a float coefficient times its reciprocal divisor, rounded to nearest with
+// ties to even (the SIMD quantiser's conversion)
+int or_jpg_quant_float(float v, float rdiv)
+{
+  return (int)nearbyintf(v * rdiv);
 }
 
 int or_jpg_quant(int32_t v, int q8)
@@ -195,8 +293,59 @@ static void hcodes(const uint8_t bits[17], const uint8_t *val, hcode_t *h)
 // all blocks of the scan: (component, block x, block y) in MCU order
 static int nblocks(int W, int H, int hs, int vs) { return ((W + 8 * hs - 1) / (8 * hs)) * ((H + 8 * vs - 1) / (8 * vs)) * (hs * vs + 2); }
 
+// libjpeg's input smoothing (smoothing_factor sf, 0..100) of a plane of
+// ycc samples pw x ph (replicated past the image, as the planes here are):
+// a full-size component mixes each sample with its 8 neighbours,
+// (m (65536 - 512 sf) + sum8 64 sf + 32768) >> 16; a 2x2-subsampled one
+// forms each output from its 4 members, their 8 edge neighbours (twice) and
+// 4 corner neighbours, (sum4 (16384 - 80 sf) + (2 edge + corner) 16 sf +
+// 32768) >> 16.  Samples outside the image repeat the nearest edge sample.
+static int plane_at(const uint8_t *pl, int pw, int ph, int x, int y)
+{
+  x = x < 0 ? 0 : (x >= pw ? pw - 1 : x);
+  y = y < 0 ? 0 : (y >= ph ? ph - 1 : y);
+  return pl[(size_t)y * pw + x];
+}
+
+static void smooth_full(const uint8_t *in, uint8_t *out, int pw, int ph, int sf)
+{
+  const int32_t ms = 65536 - 512 * sf, ns = 64 * sf;
+  for(int y = 0; y < ph; y++) for(int x = 0; x < pw; x++)
+  {
+    int32_t n = 0;
+    for(int j = -1; j <= 1; j++) for(int i = -1; i <= 1; i++)
+      if(i || j) n += plane_at(in, pw, ph, x + i, y + j);
+    out[(size_t)y * pw + x] = (uint8_t)((plane_at(in, pw, ph, x, y) * ms + n * ns + 32768) >> 16);
+  }
+}
+
+static int smooth_2x2(const uint8_t *in, int pw, int ph, int x, int y, int sf)
+{ // output (x, y) of members (2x..2x+1, 2y..2y+1)
+  const int X = 2 * x, Y = 2 * y;
+  int32_t m = 0, e = 0, c = 0;
+  for(int j = 0; j < 2; j++) for(int i = 0; i < 2; i++) m += plane_at(in, pw, ph, X + i, Y + j);
+  for(int i = 0; i < 2; i++) e += plane_at(in, pw, ph, X + i, Y - 1) + plane_at(in, pw, ph, X + i, Y + 2);
+  for(int j = 0; j < 2; j++) e += plane_at(in, pw, ph, X - 1, Y + j) + plane_at(in, pw, ph, X + 2, Y + j);
+  c = plane_at(in, pw, ph, X - 1, Y - 1) + plane_at(in, pw, ph, X + 2, Y - 1) + plane_at(in, pw, ph, X - 1, Y + 2)
+    + plane_at(in, pw, ph, X + 2, Y + 2);
+  return (int)((m * (16384 - 80 * sf) + (2 * e + c) * (16 * sf) + 32768) >> 16);
+}
+
+int or_jpg_encode_ex(const uint8_t *rgba, int W, int H, int stride, int quality, int hs, int vs, int dct, int smooth,
+                     uint8_t **out, size_t *out_len);
+
 int or_jpg_encode(const uint8_t *rgba, int W, int H, int stride, int quality, int hs, int vs, uint8_t **out, size_t *out_len)
 {
+  return or_jpg_encode_ex(rgba, W, H, stride, quality, hs, vs, 0, 0, out, out_len);
+}
+
+// dct: 0 islow, 1 ifast, 2 float (libjpeg's JDCT_ISLOW, JDCT_IFAST,
+// JDCT_FLOAT); smooth: libjpeg's smoothing_factor (full-size components and
+// 2x2 chroma; 2x1 chroma is not smoothed, as in libjpeg)
+int or_jpg_encode_ex(const uint8_t *rgba, int W, int H, int stride, int quality, int hs, int vs, int dct, int smooth,
+                     uint8_t **out, size_t *out_len)
+{
+  if(dct < 0 || dct > 2 || smooth < 0 || smooth > 100) return -1;
   const int mw = 8 * hs, mh = 8 * vs;
   const int mx = (W + mw - 1) / mw, my = (H + mh - 1) / mh;
   const int PW = mx * mw, PH = my * mh;         // padded luma extent
@@ -211,11 +360,31 @@ int or_jpg_encode(const uint8_t *rgba, int W, int H, int stride, int quality, in
     or_jpg_ycc(p[0], p[1], p[2], &a, &b, &c);
     Y[(size_t)y * PW + x] = (uint8_t)a; Cb[(size_t)y * PW + x] = (uint8_t)b; Cr[(size_t)y * PW + x] = (uint8_t)c;
   }
+  if(smooth)
+  { // the full-size planes: luma always, chroma at 4:4:4
+    uint8_t *t = malloc((size_t)PW * PH);
+    smooth_full(Y, t, PW, PH, smooth);
+    memcpy(Y, t, (size_t)PW * PH);
+    if(hs * vs == 1)
+    {
+      smooth_full(Cb, t, PW, PH, smooth);
+      memcpy(Cb, t, (size_t)PW * PH);
+      smooth_full(Cr, t, PW, PH, smooth);
+      memcpy(Cr, t, (size_t)PW * PH);
+    }
+    free(t);
+  }
   for(int y = 0; y < CH; y++)
   {
     int bias = hs == 2 ? (vs == 2 ? 1 : 0) : 0;
     for(int x = 0; x < CW; x++)
     {
+      if(smooth && hs == 2 && vs == 2)
+      {
+        cb[(size_t)y * CW + x] = (uint8_t)smooth_2x2(Cb, PW, PH, x, y, smooth);
+        cr[(size_t)y * CW + x] = (uint8_t)smooth_2x2(Cr, PW, PH, x, y, smooth);
+        continue;
+      }
       int s1 = 0, s2 = 0;
       for(int j = 0; j < vs; j++) for(int i = 0; i < hs; i++)
       {
@@ -264,8 +433,23 @@ int or_jpg_encode(const uint8_t *rgba, int W, int H, int stride, int quality, in
         int32_t d[64];
         const int x0 = bx * 8, y0 = by * 8;
         for(int k = 0; k < 64; k++) d[k] = (int32_t)pl[(size_t)(y0 + k / 8) * pw + x0 + k % 8] - 128;
-        or_jpg_fdct(d);
-        for(int k = 0; k < 64; k++) o16[k] = (int16_t)or_jpg_quant(d[zz[k]], q[zz[k]] << 3);
+        if(dct == 2)
+        {
+          float f[64];
+          for(int k = 0; k < 64; k++) f[k] = (float)d[k];
+          or_jpg_fdct_float(f);
+          for(int k = 0; k < 64; k++) o16[k] = (int16_t)or_jpg_quant_float(f[zz[k]], or_jpg_divisor_float(q[zz[k]], zz[k]));
+        }
+        else if(dct == 1)
+        {
+          or_jpg_fdct_ifast(d);
+          for(int k = 0; k < 64; k++) o16[k] = (int16_t)or_jpg_quant(d[zz[k]], or_jpg_divisor_ifast(q[zz[k]], zz[k]));
+        }
+        else
+        {
+          or_jpg_fdct(d);
+          for(int k = 0; k < 64; k++) o16[k] = (int16_t)or_jpg_quant(d[zz[k]], q[zz[k]] << 3);
+        }
       }
     }
   }

@@ -4,10 +4,11 @@
 // T.81 baseline with IJG's choices), so the file's tables and entropy-coded
 // data are those libjpeg writes for the same image and settings:
 //   k_jpg_coef    one lane per 8x8 block in MCU order: fixed-point YCbCr,
-//                 2:1 chroma boxes with the alternating bias, samples past
-//                 the edges replicated, the islow integer DCT and the
-//                 rounded quantisation, all in registers; stored zigzag
-//                 ordered, so the passes below load a block in 8 vectors
+//                 2:1 chroma boxes with the alternating bias (or libjpeg's
+//                 smoothing filter), samples past the edges replicated,
+//                 the islow, ifast or float DCT and its quantisation, all
+//                 in registers; stored zigzag ordered, so the passes below
+//                 load a block in 8 vectors
 //   k_jpg_dummy   blocks of the last MCUs past a component's own extent:
 //                 zero, with the DC of the block to the left / above
 //   k_jpg_stats   symbol counts of the four tables (LDS, then global)
@@ -19,6 +20,7 @@
 //                 scatter
 #include "launch.h"
 
+#include <algorithm>
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
@@ -37,6 +39,7 @@ struct jpg_geom
   int hs, vs, bpm;           // luma sampling, blocks per MCU
   int mx, my, nb;            // MCUs across, down; blocks
   int wib[3], hib[3];        // each component's own extent in blocks
+  int smooth;                // libjpeg's smoothing_factor, 0..100
 };
 
 // block b -> component, block column and row
@@ -50,8 +53,8 @@ DEV void jpg_block(const jpg_geom &g, int b, int &c, int &bx, int &by, int &i)
 
 DEV void jpg_ycc(const jpg_geom &g, int x, int y, int &Y, int &cb, int &cr)
 {
-  x = x < g.W ? x : g.W - 1;
-  y = y < g.H ? y : g.H - 1;
+  x = x < 0 ? 0 : (x < g.W ? x : g.W - 1);
+  y = y < 0 ? 0 : (y < g.H ? y : g.H - 1);
   const uchar4 p = *(const uchar4 *)(g.rgba + 4 * ((long)y * g.pitch + x));
   const int r = p.x, gg = p.y, b = p.z;
   // 16-bit fixed point: round(0.299 * 65536) etc.
@@ -60,8 +63,50 @@ DEV void jpg_ycc(const jpg_geom &g, int x, int y, int &Y, int &cb, int &cr)
   cr = (32768 * r - 27439 * gg - 5329 * b + (128 << 16) + 32767) >> 16;
 }
 
+DEV int jpg_comp(const jpg_geom &g, int c, int x, int y)
+{ // full-resolution sample of component c, replicated past the edges
+  int v[3];
+  jpg_ycc(g, x, y, v[0], v[1], v[2]);
+  return v[c];
+}
+
+// libjpeg's input smoothing (smoothing_factor sf): a full-size component
+// mixes a sample with its 8 neighbours, weights 1 - 8 sf/1024 and sf/1024
+// in 16-bit fixed point; 2x2 chroma forms an output from its 4 members
+// ((1 - 5 sf/1024)/4 each), their 8 edge neighbours (sf/512) and 4 corner
+// neighbours (sf/1024).  2x1 chroma is not smoothed.
+DEV int jpg_sample_smooth(const jpg_geom &g, int c, int x, int y)
+{
+  const int sf = g.smooth;
+  if(c == 0 || g.hs * g.vs == 1)
+  {
+    int n = 0;
+#pragma unroll
+    for(int j = -1; j <= 1; j++)
+#pragma unroll
+      for(int i = -1; i <= 1; i++)
+        if(i || j) n += jpg_comp(g, c, x + i, y + j);
+    return (jpg_comp(g, c, x, y) * (65536 - 512 * sf) + n * (64 * sf) + 32768) >> 16;
+  }
+  const int X = 2 * x, Y = 2 * y;
+  int m = 0, e = 0, k = 0;
+#pragma unroll
+  for(int j = -1; j <= 2; j++)
+#pragma unroll
+    for(int i = -1; i <= 2; i++)
+    {
+      const int v = jpg_comp(g, c, X + i, Y + j);
+      const bool ix = i == 0 || i == 1, iy = j == 0 || j == 1;
+      if(ix && iy) m += v;
+      else if(ix || iy) e += v;
+      else k += v;
+    }
+  return (m * (16384 - 80 * sf) + (2 * e + k) * (16 * sf) + 32768) >> 16;
+}
+
 DEV int jpg_sample(const jpg_geom &g, int c, int x, int y)
 {
+  if(g.smooth && !(c > 0 && g.hs == 2 && g.vs == 1)) return jpg_sample_smooth(g, c, x, y);
   int Y, cb, cr;
   if(c == 0) { jpg_ycc(g, x, y, Y, cb, cr); return Y; }
   int s = 0;
@@ -103,6 +148,53 @@ DEV void jpg_fdct8(int32_t *p)
   p[S] = JDESC(b7 + a1 + a4, sh);
 }
 
+// the scaled DCT of Arai, Agui and Nakajima, libjpeg's "ifast" (integer,
+// 8 fractional bits, products truncated) and "float": outputs carry the
+// factors 8 s_u s_v (s_0 = 1, s_k = sqrt(2) cos(k pi/16)), which the
+// divisors hold
+#define AAN8(v, c) ((int32_t)(((v) * (c)) >> 8))
+template<int S>
+DEV void jpg_fdct8_ifast(int32_t *p)
+{
+  const int32_t a0 = p[0] + p[7 * S], a7 = p[0] - p[7 * S], a1 = p[S] + p[6 * S], a6 = p[S] - p[6 * S];
+  const int32_t a2 = p[2 * S] + p[5 * S], a5 = p[2 * S] - p[5 * S], a3 = p[3 * S] + p[4 * S], a4 = p[3 * S] - p[4 * S];
+  const int32_t e0 = a0 + a3, e3 = a0 - a3, e1 = a1 + a2, e2 = a1 - a2;
+  p[0] = e0 + e1;
+  p[4 * S] = e0 - e1;
+  const int32_t r = AAN8(e2 + e3, 181);
+  p[2 * S] = e3 + r;
+  p[6 * S] = e3 - r;
+  const int32_t o0 = a4 + a5, o1 = a5 + a6, o2 = a6 + a7;
+  const int32_t rot = AAN8(o0 - o2, 98);
+  const int32_t q2 = AAN8(o0, 139) + rot, q4 = AAN8(o2, 334) + rot, q3 = AAN8(o1, 181);
+  const int32_t h1 = a7 + q3, h3 = a7 - q3;
+  p[5 * S] = h3 + q2;
+  p[3 * S] = h3 - q2;
+  p[S] = h1 + q4;
+  p[7 * S] = h1 - q4;
+}
+
+template<int S>
+DEV void jpg_fdct8_float(float *p)
+{
+  const float a0 = p[0] + p[7 * S], a7 = p[0] - p[7 * S], a1 = p[S] + p[6 * S], a6 = p[S] - p[6 * S];
+  const float a2 = p[2 * S] + p[5 * S], a5 = p[2 * S] - p[5 * S], a3 = p[3 * S] + p[4 * S], a4 = p[3 * S] - p[4 * S];
+  const float e0 = a0 + a3, e3 = a0 - a3, e1 = a1 + a2, e2 = a1 - a2;
+  p[0] = e0 + e1;
+  p[4 * S] = e0 - e1;
+  const float r = (e2 + e3) * 0.707106781f;
+  p[2 * S] = e3 + r;
+  p[6 * S] = e3 - r;
+  const float o0 = a4 + a5, o1 = a5 + a6, o2 = a6 + a7;
+  const float rot = (o0 - o2) * 0.382683433f;
+  const float q2 = 0.541196100f * o0 + rot, q4 = 1.306562965f * o2 + rot, q3 = o1 * 0.707106781f;
+  const float h1 = a7 + q3, h3 = a7 - q3;
+  p[5 * S] = h3 + q2;
+  p[3 * S] = h3 - q2;
+  p[S] = h1 + q4;
+  p[7 * S] = h1 - q4;
+}
+
 DEV int jpg_quant(int32_t v, int q8)
 { // v / q8 rounded half away from zero
   if(v < 0) { v = -v + (q8 >> 1); return -(v >= q8 ? v / q8 : 0); }
@@ -110,7 +202,10 @@ DEV int jpg_quant(int32_t v, int q8)
   return v >= q8 ? v / q8 : 0;
 }
 
-// coefficients in natural order; q8: 8 x quantiser, luma then chroma
+// coefficients in natural order; q8: the divisors, luma then chroma: 8 x
+// quantiser (DCT 0, islow), 8 x quantiser x the AAN factors in integer
+// (1, ifast), or the float reciprocal of that (2, float; its bits)
+template<int DCT>
 __global__ void __launch_bounds__(64) k_jpg_coef(jpg_geom g, const int32_t *q8, int16_t *coef, uint8_t *dummy)
 {
   const int b = blockIdx.x * 64 + threadIdx.x;
@@ -128,11 +223,33 @@ __global__ void __launch_bounds__(64) k_jpg_coef(jpg_geom g, const int32_t *q8, 
   int32_t d[64];
 #pragma unroll
   for(int k = 0; k < 64; k++) d[k] = jpg_sample(g, c, bx * 8 + (k & 7), by * 8 + (k >> 3)) - 128;
-#pragma unroll
-  for(int k = 0; k < 8; k++) jpg_fdct8<0, 1>(d + 8 * k);
-#pragma unroll
-  for(int k = 0; k < 8; k++) jpg_fdct8<1, 8>(d + k);
   const int32_t *q = q8 + (c ? 64 : 0);
+  if(DCT == 2)
+  { // float: multiply by the reciprocal, round to nearest even
+    float f[64];
+#pragma unroll
+    for(int k = 0; k < 64; k++) f[k] = (float)d[k];
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8_float<1>(f + 8 * k);
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8_float<8>(f + k);
+#pragma unroll
+    for(int k = 0; k < 64; k++) d[k] = (int32_t)rintf(f[k] * __int_as_float(q[k]));
+  }
+  else if(DCT == 1)
+  {
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8_ifast<1>(d + 8 * k);
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8_ifast<8>(d + k);
+  }
+  else
+  {
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8<0, 1>(d + 8 * k);
+#pragma unroll
+    for(int k = 0; k < 8; k++) jpg_fdct8<1, 8>(d + k);
+  }
   // stored in zigzag order, the order the entropy coder walks them
   constexpr uint8_t zz[64] = {
     0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21,
@@ -141,7 +258,8 @@ __global__ void __launch_bounds__(64) k_jpg_coef(jpg_geom g, const int32_t *q8, 
 #pragma unroll
   for(int k = 0; k < 64; k += 2)
   {
-    const int a = jpg_quant(d[zz[k]], q[zz[k]]), e = jpg_quant(d[zz[k + 1]], q[zz[k + 1]]);
+    const int a = DCT == 2 ? d[zz[k]] : jpg_quant(d[zz[k]], q[zz[k]]);
+    const int e = DCT == 2 ? d[zz[k + 1]] : jpg_quant(d[zz[k + 1]], q[zz[k + 1]]);
     *(uint32_t *)(o + k) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)e << 16);
   }
 }
@@ -465,12 +583,28 @@ extern "C" void vkdt_jpg_quant_tables(int quality, uint8_t luma[64], uint8_t chr
 extern "C" int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, uint32_t pitch, int quality, int hs,
                                    int vs, uint8_t **out, size_t *out_len, void *stream)
 {
+  return vkdt_hip_jpg_encode_ex(rgba, wd, ht, pitch, quality, hs, vs, VKDT_JPG_DCT_ISLOW, 0, nullptr, 0, out, out_len,
+                                stream);
+}
+
+extern "C" int vkdt_hip_jpg_encode_ex(const void *rgba, uint32_t wd, uint32_t ht, uint32_t pitch, int quality, int hs,
+                                      int vs, int dct, int smoothing, const uint8_t *icc, size_t icc_len, uint8_t **out,
+                                      size_t *out_len, void *stream)
+{
   *out = nullptr;
   *out_len = 0;
   if(!rgba || wd == 0 || ht == 0 || wd > 65535 || ht > 65535 || pitch < wd)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "jpg: extent %ux%u (pitch %u)", wd, ht, pitch);
   if(!((hs == 1 && vs == 1) || (hs == 2 && vs == 1) || (hs == 2 && vs == 2)))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "jpg: sampling %dx%d (4:4:4, 4:2:2 or 4:2:0)", hs, vs);
+  if(dct < VKDT_JPG_DCT_ISLOW || dct > VKDT_JPG_DCT_FLOAT)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "jpg: dct method %d (0 islow, 1 ifast, 2 float)", dct);
+  if(smoothing < 0 || smoothing > 100)
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "jpg: smoothing factor %d (0..100)", smoothing);
+  // ICC.1 Annex B.4: at most 255 APP2 markers of 65519 profile bytes
+  const size_t icc_chunk = 65533 - 14;
+  if(icc_len && (!icc || (icc_len + icc_chunk - 1) / icc_chunk > 255))
+    return vkdt_set_error(VKDT_HIP_INCOMPLETE, "jpg: icc profile of %zu bytes", icc_len);
   jpg_geom g;
   g.rgba = (const uint8_t *)rgba;
   g.W = (int)wd; g.H = (int)ht; g.pitch = (int)pitch;
@@ -484,10 +618,28 @@ extern "C" int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, u
   g.hib[0] = (g.H + 7) / 8;
   g.wib[1] = g.wib[2] = ((g.W + hs - 1) / hs + 7) / 8;
   g.hib[1] = g.hib[2] = ((g.H + vs - 1) / vs + 7) / 8;
+  g.smooth = smoothing;
   uint8_t ql[64], qc[64];
   vkdt_jpg_quant_tables(quality, ql, qc);
+  // divisors by DCT method; the AAN factors s_0 = 1, s_k = sqrt(2) cos(k pi/16)
+  static const double aan[8] = { 1.0, 1.387039845, 1.306562965, 1.175875602, 1.0, 0.785694958, 0.541196100,
+                                 0.275899379 };
   int32_t q8[128];
-  for(int k = 0; k < 64; k++) { q8[k] = ql[k] << 3; q8[64 + k] = qc[k] << 3; }
+  for(int t = 0; t < 2; t++)
+    for(int k = 0; k < 64; k++)
+    {
+      const int q = t ? qc[k] : ql[k];
+      const double f = aan[k / 8] * aan[k % 8];
+      if(dct == VKDT_JPG_DCT_FLOAT)
+      {
+        const float r = (float)(1.0 / ((double)q * aan[k / 8] * aan[k % 8] * 8.0));
+        memcpy(q8 + 64 * t + k, &r, 4);
+      }
+      else if(dct == VKDT_JPG_DCT_IFAST)
+        q8[64 * t + k] = (int32_t)(((int64_t)q * (int32_t)(16384.0 * f + 0.5) + 1024) >> 11);
+      else
+        q8[64 * t + k] = q << 3;
+    }
 
   // device: q8 | codes | freq | coef | dummy flags | bits | sums | total
   const long ngroups = (nbl + 1023) / 1024;
@@ -510,7 +662,8 @@ extern "C" int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, u
   if(!err)
   {
     hipLaunchKernelGGL(k_jpg_zero, dim3((4 * 257 + 255) / 256), dim3(256), 0, s, freq, (long)(4 * 257));
-    hipLaunchKernelGGL(k_jpg_coef, dim3((g.nb + 63) / 64), dim3(64), 0, s, g, (const int32_t *)(dev + o_q), coef,
+    const auto coef_kernel = dct == VKDT_JPG_DCT_FLOAT ? k_jpg_coef<2> : (dct == VKDT_JPG_DCT_IFAST ? k_jpg_coef<1> : k_jpg_coef<0>);
+    hipLaunchKernelGGL(coef_kernel, dim3((g.nb + 63) / 64), dim3(64), 0, s, g, (const int32_t *)(dev + o_q), coef,
                        dev + o_dummy);
     hipLaunchKernelGGL(k_jpg_dummy, dim3((g.nb + 255) / 256), dim3(256), 0, s, g, coef, (const uint8_t *)(dev + o_dummy), 1);
     hipLaunchKernelGGL(k_jpg_dummy, dim3((g.nb + 255) / 256), dim3(256), 0, s, g, coef, (const uint8_t *)(dev + o_dummy), 2);
@@ -600,6 +753,17 @@ extern "C" int vkdt_hip_jpg_encode(const void *rgba, uint32_t wd, uint32_t ht, u
     o.w(0xffe0); o.w(16);                                     // JFIF, 300 dpi (o-jpg/main.c:157-159)
     o.b('J'); o.b('F'); o.b('I'); o.b('F'); o.b(0);
     o.b(1); o.b(1); o.b(1); o.w(300); o.w(300); o.b(0); o.b(0);
+    // the ICC profile in APP2 markers: "ICC_PROFILE\0", sequence number
+    // from 1, marker count, then up to 65519 profile bytes (ICC.1 B.4)
+    const int nicc = (int)((icc_len + icc_chunk - 1) / icc_chunk);
+    for(int m = 0; m < nicc; m++)
+    {
+      const size_t n = std::min(icc_chunk, icc_len - m * icc_chunk);
+      o.w(0xffe2); o.w((int)(2 + 14 + n));
+      for(const char *c = "ICC_PROFILE"; *c; c++) o.b(*c);
+      o.b(0); o.b(m + 1); o.b(nicc);
+      o.d.insert(o.d.end(), icc + m * icc_chunk, icc + m * icc_chunk + n);
+    }
     for(int t = 0; t < 2; t++)
     {
       o.w(0xffdb); o.w(67); o.b(t);

@@ -84,6 +84,8 @@ def lib():
         L.vkdt_hip_unpack_msb16.argtypes = [vp, sz, vp, C.c_uint32, C.c_int, vp]
         L.vkdt_hip_jpg_encode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_void_p), C.POINTER(sz), vp]
+        L.vkdt_hip_jpg_encode_ex.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, vp, sz, C.POINTER(C.c_void_p), C.POINTER(sz), vp]
         L.vkdt_jpg_free.argtypes = [vp]
         L.vkdt_hip_last_error.restype = C.c_char_p
         _lib = L
@@ -229,11 +231,20 @@ def unpack_msb16(packed, out, n, bits, stream=None):
     check(lib().vkdt_hip_unpack_msb16(buf, len(packed), out, n, bits, stream), "unpack")
 
 
-def jpg_encode(rgba_dev, wd, ht, pitch, quality=95, hs=1, vs=1, stream=None):
-    """baseline JPEG (bytes) of an 8-bit rgba device image"""
+JPG_DCT_ISLOW, JPG_DCT_IFAST, JPG_DCT_FLOAT = 0, 1, 2
+
+
+def jpg_encode(rgba_dev, wd, ht, pitch, quality=95, hs=1, vs=1, stream=None, dct=None, smooth=0, icc=None):
+    """baseline JPEG (bytes) of an 8-bit rgba device image; dct (JPG_DCT_*),
+    smooth (0..100) and icc (bytes) select vkdt_hip_jpg_encode_ex"""
     p, n = C.c_void_p(), C.c_size_t()
-    check(lib().vkdt_hip_jpg_encode(rgba_dev, wd, ht, pitch, quality, hs, vs, C.byref(p), C.byref(n), stream),
-          "jpg encode")
+    if dct is None and not smooth and icc is None:
+        rc = lib().vkdt_hip_jpg_encode(rgba_dev, wd, ht, pitch, quality, hs, vs, C.byref(p), C.byref(n), stream)
+    else:
+        ib = C.create_string_buffer(icc, len(icc)) if icc else None
+        rc = lib().vkdt_hip_jpg_encode_ex(rgba_dev, wd, ht, pitch, quality, hs, vs, dct or 0, smooth, ib,
+                                          len(icc) if icc else 0, C.byref(p), C.byref(n), stream)
+    check(rc, "jpg encode")
     out = C.string_at(p, n.value)
     lib().vkdt_jpg_free(p)
     return out
