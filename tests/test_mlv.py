@@ -14,7 +14,11 @@ i-mlv/main.c, video_mlv.c).
     padding (frameSpace);
   * the frame index is held at the header's last frame;
   * extent, levels, filters, crop, exposure, aperture, ISO, focal length,
-    maker/model, frame count and rate are the clip's.
+    maker/model, frame count and rate are the clip's;
+  * the audio callback hands back the clip's AUDF bytes synced to the first
+    frame (cut before it, silence after it, in whole samples), limited to
+    the frames' duration in 1024-sample blocks, with the reference's
+    sample/byte count (main.c:214-230, video_mlv.c:18-87).
 """
 import numpy as np
 import pytest
@@ -154,4 +158,101 @@ def test_rejects_bad_clips(tmp_path):
     g.export("o-pfm", bad + ".out")
     with pytest.raises(pipe.PipeError):
         g.plan()
+    g.close()
+
+
+# ---- audio (i-mlv/main.c:214-230, video_mlv.c:18-87) -------------------------
+# Video frames are stamped at 1000 + 40000 k us (25 fps).  The audio blocks
+# of these clips are 16-bit stereo at 48 kHz (4 bytes a sample of both
+# channels, 192 bytes a millisecond) with distinct byte values, so every
+# check below compares bytes the module hands back with bytes of the file.
+
+def audio_clip(tmp_path, start_us, nframes=4, nbytes=96000, blocks=3, space=0, fps=(25000, 1000)):
+    rng = np.random.default_rng(nbytes + start_us)
+    data = rng.integers(0, 256, nbytes, dtype=np.uint8).tobytes()
+    cuts = np.linspace(0, nbytes, blocks + 1).astype(int) // 4 * 4
+    bl = [(start_us + int(cuts[i]) // 192 * 1000, int(cuts[i + 1] - cuts[i])) for i in range(blocks)]
+    path = str(tmp_path / f"a{start_us}.mlv")
+    synth.write_mlv(path, clip_frames(n=nframes), fps=fps,
+                    audio=dict(data=data, channels=2, rate=48000, bits=16, blocks=bl, space=space))
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    g.plan()
+    return g, data
+
+
+def all_audio(g):
+    out, beg = b"", 0
+    while True:
+        k, b = g.audio("i-mlv", "main", beg, 4096, 4)
+        if k <= 0:
+            return out
+        out += b
+        beg += k
+
+
+def test_audio_in_step_with_the_first_frame(tmp_path):
+    """audio stamped with the first frame (1000 us) comes back as written,
+    past per-block padding, described as S16_LE stereo at 48 kHz"""
+    g, data = audio_clip(tmp_path, 1000, space=12)
+    ip = g.img_param("i-mlv", "main")
+    assert (ip["snd_format"], ip["snd_channels"], ip["snd_samplerate"]) == (2, 2, 48000)
+    got = all_audio(g)
+    assert len(got) == 32768 and got == data[:32768]
+    g.close()
+
+
+def test_audio_before_the_first_frame_is_cut(tmp_path):
+    """audio stamped from 0 us, 1 ms (48 samples, 192 bytes) before the first
+    frame, loses those samples"""
+    g, data = audio_clip(tmp_path, 0, nbytes=64000)
+    got = all_audio(g)
+    assert got == data[192:192 + len(got)] and len(got) == 32768
+    g.close()
+
+
+def test_audio_after_the_first_frame_gets_silence(tmp_path):
+    """audio starting 5 ms after the first frame is preceded by 240 silent
+    samples (960 bytes)"""
+    g, data = audio_clip(tmp_path, 6000, nbytes=64000)
+    got = all_audio(g)
+    assert len(got) == 32768 and got[:960] == b"\0" * 960 and got[960:] == data[:32768 - 960]
+    g.close()
+
+
+def test_audio_length_is_the_frames_duration(tmp_path):
+    """4 frames at 25 fps are 160 ms = 30720 bytes, rounded up to whole
+    1024-sample blocks (4096 bytes): 32768 of the 96000 bytes play; a
+    clip with less audio than that plays all of it (rounded up the same way,
+    the rest silent)"""
+    g, data = audio_clip(tmp_path, 1000, nframes=4, nbytes=96000)
+    got = all_audio(g)
+    assert len(got) == 32768 and got == data[:32768]
+    assert g.audio("i-mlv", "main", 32768 // 4, 16, 4)[0] == 0
+    g.close()
+    g, data = audio_clip(tmp_path, 1000, nframes=40, nbytes=10000, blocks=2)
+    got = all_audio(g)
+    assert len(got) == 12288 and got[:10000] == data and got[10000:] == b"\0" * 2288
+    g.close()
+
+
+def test_audio_count_mixes_samples_and_bytes(tmp_path):
+    """the callback returns min(cnt, bytes left) / bytes a sample, as the
+    reference computes it"""
+    g, data = audio_clip(tmp_path, 1000, nframes=4, nbytes=96000)
+    k, b = g.audio("i-mlv", "main", 0, 1000, 4)
+    assert k == 250 and b == data[:1000]
+    k, b = g.audio("i-mlv", "main", 8000, 4096, 4)       # 768 bytes left of 32768
+    assert k == 192 and b == data[32000:32768]
+    g.close()
+
+
+def test_no_audio_without_wavi_or_audf(tmp_path):
+    path = str(tmp_path / "n.mlv")
+    synth.write_mlv(path, clip_frames(n=2))
+    g = pipe.Graph(lines=pipe.mlv_graph_cfg(path))
+    g.export("o-pfm", path + ".out")
+    g.plan()
+    assert g.audio("i-mlv", "main", 0, 100, 4) == (0, b"")
+    assert g.img_param("i-mlv", "main")["snd_channels"] == 0
     g.close()

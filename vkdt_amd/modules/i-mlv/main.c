@@ -10,10 +10,12 @@
  *   read_source_hip  on the GPU: the stored bytes go up as they are and the
  *                    unpack or the lossless-JPEG decode runs on the device,
  *                    straight into the output buffer.
- * The reference looks the camera's colour matrix up in dcraw's table of
- * Adobe coefficients by model name and falls back to the identity for a
- * model it does not know (main.c:176-181); this module has no such table and
- * takes that fallback for every camera.
+ * The camera's colour matrix comes from the Canon rows of the reference's
+ * table of Adobe coefficients by model name (mlv_coeff.h), with the
+ * reference's identity fallback for a model the table does not know
+ * (main.c:172-181).  The clip's audio, synced to its first frame, is served
+ * by the audio callback (main.c:214-230) and described by the snd_* image
+ * parameters.
  */
 #include "pipe/api.h"
 #include "mlv.h"
@@ -104,6 +106,9 @@ void modify_roi_out(dt_graph_t *graph, dt_module_t *mod)
   ip->focal_length = c->lens.focal_length;
   ip->colour_primaries = s_colour_primaries_custom;
   ip->colour_trc = s_colour_trc_linear;
+  ip->snd_samplerate = c->wavi.sampling_rate;   // main.c:136-138: S16_LE whatever WAVI says
+  ip->snd_format = 2;
+  ip->snd_channels = c->wavi.channels;
   ip->noise_a = 1.0f;                   // main.c:128-129, unless a noise profile says otherwise
   ip->noise_b = 1.0f;
   snprintf(ip->datetime, sizeof(ip->datetime), "%4d%2d%2d %2d:%2d%2d", c->rtci.year, c->rtci.mon, c->rtci.mday,
@@ -241,4 +246,20 @@ int read_source_hip(dt_module_t *mod, void *dev, void *stream)
     return -1;
   }
   return 0;
+}
+
+// main.c:214-230: the clip's audio from sample `beg` on; returns how many
+// samples are there, the smaller of `cnt` and the bytes left, in samples
+// (the reference compares a sample count with a byte count)
+int audio(dt_module_t *mod, uint64_t beg, uint32_t cnt, uint8_t **samples)
+{
+  mlv_mod_t *d = mod->data;
+  if(!d || !d->filename[0] || !d->clip.audio || !d->clip.audio_size || !d->clip.wavi.sampling_rate) return 0;
+  const uint64_t bps = d->clip.wavi.bytes_per_second / d->clip.wavi.sampling_rate;
+  if(!bps) return 0;
+  if(beg >= d->clip.audio_size / bps + 1) return 0;
+  *samples = d->clip.audio + beg * bps;
+  if(*samples >= d->clip.audio + d->clip.audio_size) return 0;
+  const uint64_t left = (uint64_t)(d->clip.audio + d->clip.audio_size - *samples);
+  return (int)((cnt < left ? cnt : left) / bps);
 }

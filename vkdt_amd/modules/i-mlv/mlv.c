@@ -72,6 +72,69 @@ static int read_at(FILE *f, uint64_t off, void *dst, size_t bytes)
   return fseek(f, (long)off, SEEK_SET) || fread(dst, 1, bytes, f) != bytes;
 }
 
+static void sort_by_time(mlv_frame_t *fr, uint32_t cnt)
+{
+  for(uint32_t n = cnt; n > 1;)
+  {
+    uint32_t last = 1;
+    for(uint32_t k = 0; k + 1 < n; k++)
+      if(fr[k].timestamp > fr[k + 1].timestamp)
+      {
+        const mlv_frame_t t = fr[k];
+        fr[k] = fr[k + 1];
+        fr[k + 1] = t;
+        last = k + 1;
+      }
+    n = last;
+  }
+}
+
+// the audio blocks as one buffer synced to the first video frame
+// (video_mlv.c:18-87): audio recorded before the first frame is cut, audio
+// that starts later is preceded by silence, in whole samples of all
+// channels.  The buffer is rounded up to blocks of 1024 samples, and what
+// plays with the clip is at most the duration of its frames at the file's
+// frame rate, rounded up the same way.  No audio when the header has no
+// audio class, no WAVI channels or no AUDF block, when a block cannot be read,
+// or when the cut would take all of it.
+static void read_audio(mlv_clip_t *c)
+{
+  if(!c->hdr.audio_class || !c->audfs || !c->wavi.channels || !c->frames) return;
+  uint64_t total = 0;
+  for(uint32_t i = 0; i < c->audfs; i++) total += c->audf[i].size;
+  uint8_t *raw = malloc(total ? total : 1);
+  if(!raw) return;
+  uint64_t at = 0;
+  for(uint32_t i = 0; i < c->audfs; i++)
+  {
+    if(read_at(c->file[c->audf[i].chunk], c->audf[i].offset, raw + at, c->audf[i].size)) { free(raw); return; }
+    at += c->audf[i].size;
+  }
+  const uint64_t sample = (uint64_t)c->wavi.channels * c->wavi.bits_per_sample / 8;   // bytes of all channels
+  const uint16_t align = (uint16_t)(sample * 1024);
+  if(!sample || !align) { free(raw); return; }
+  const int64_t sync = (int64_t)(((double)c->frame[0].timestamp - (double)c->audf[0].timestamp)
+                                 * (double)(c->wavi.sampling_rate * sample / 1000000.0));
+  uint64_t cut = 0, pad = 0;
+  if(sync >= 0) cut = (uint64_t)sync - (uint64_t)sync % sample;
+  else pad = (uint64_t)(-sync) - (uint64_t)(-sync) % sample;
+  if(cut >= total) { free(raw); return; }
+  const uint64_t synced = total - cut + pad;
+  const uint64_t synced_al = synced - synced % align + align;
+  c->audio = calloc(synced_al, 1);
+  if(!c->audio) { free(raw); return; }
+  memcpy(c->audio + pad, raw + cut, total - cut);
+  free(raw);
+  // a zero denominator makes the rate infinite and the duration 0, as in the
+  // reference's double arithmetic
+  const double fps = (double)c->hdr.fps_nom / (double)c->hdr.fps_denom;
+  const double bytes = (double)(c->wavi.sampling_rate * sample * c->frames) / fps;
+  const uint64_t t = bytes > 0.0 ? (uint64_t)bytes : 0;
+  const uint64_t t_al = t - t % align + align;
+  c->audio_buffer_size = synced_al;
+  c->audio_size = t_al < synced_al ? t_al : synced_al;
+}
+
 int mlv_open(mlv_clip_t *c, const char *filename, char *err, size_t errlen)
 {
   memset(c, 0, sizeof(*c));
@@ -118,6 +181,22 @@ int mlv_open(mlv_clip_t *c, const char *filename, char *err, size_t errlen)
                                                  pos, b.timestamp };
         }
       }
+      else if(!memcmp(b.type, "AUDF", 4))
+      {
+        mlv_audf_t a;
+        e = read_at(f, pos, &a, sizeof(a));
+        if(!e && (uint64_t)sizeof(a) + a.frame_space <= b.size)
+        {
+          if(c->audfs % 128 == 0)
+          {
+            mlv_frame_t *na = realloc(c->audf, (c->audfs + 128) * sizeof(mlv_frame_t));
+            if(!na) { snprintf(err, errlen, "out of memory"); mlv_close(c); return 1; }
+            c->audf = na;
+          }
+          c->audf[c->audfs++] = (mlv_frame_t){ i, pos + sizeof(a) + a.frame_space, b.size - sizeof(a) - a.frame_space,
+                                               pos, b.timestamp };
+        }
+      }
       else if(!memcmp(b.type, "RAWI", 4)) e = read_at(f, pos, &c->rawi, sizeof(c->rawi));
       else if(!memcmp(b.type, "EXPO", 4)) e = read_at(f, pos, &c->expo, sizeof(c->expo));
       else if(!memcmp(b.type, "IDNT", 4)) e = read_at(f, pos, &c->idnt, sizeof(c->idnt));
@@ -135,21 +214,12 @@ int mlv_open(mlv_clip_t *c, const char *filename, char *err, size_t errlen)
   if(!c->frames) { snprintf(err, errlen, "%s: no video frames", filename); mlv_close(c); return 1; }
   if(c->rawi.xres == 0 || c->rawi.yres == 0 || c->rawi.raw.bits_per_pixel < 1 || c->rawi.raw.bits_per_pixel > 16)
   { snprintf(err, errlen, "%s: no usable RAWI block", filename); mlv_close(c); return 1; }
-  // frames in timestamp order (stable: equal stamps keep file order)
-  for(uint32_t n = c->frames; n > 1;)
-  {
-    uint32_t last = 1;
-    for(uint32_t k = 0; k + 1 < n; k++)
-      if(c->frame[k].timestamp > c->frame[k + 1].timestamp)
-      {
-        const mlv_frame_t t = c->frame[k];
-        c->frame[k] = c->frame[k + 1];
-        c->frame[k + 1] = t;
-        last = k + 1;
-      }
-    n = last;
-  }
+  // frames and audio blocks in timestamp order (stable: equal stamps keep
+  // file order)
+  sort_by_time(c->frame, c->frames);
+  sort_by_time(c->audf, c->audfs);
   c->frame_rate = c->hdr.fps_denom ? (double)c->hdr.fps_nom / c->hdr.fps_denom : 0.0;
+  read_audio(c);
   return 0;
 }
 
@@ -160,6 +230,8 @@ void mlv_close(mlv_clip_t *c)
       if(c->file[i]) fclose(c->file[i]);
   free(c->file);
   free(c->frame);
+  free(c->audf);
+  free(c->audio);
   memset(c, 0, sizeof(*c));
 }
 

@@ -46,6 +46,14 @@ mlv_vidf_t;
 
 typedef struct
 {
+  mlv_block_t b;              // "AUDF"
+  uint32_t frame_number;
+  uint32_t frame_space;       // padding before the audio data
+}
+mlv_audf_t;
+
+typedef struct
+{
   uint32_t api_version, buffer;
   int32_t  height, width, pitch, frame_size, bits_per_pixel, black_level, white_level;
   int32_t  crop[4];
@@ -103,6 +111,11 @@ typedef struct
   mlv_frame_t *frame;         // in timestamp order
   uint32_t    frames;
   double      frame_rate;
+  mlv_frame_t *audf;          // audio blocks, in timestamp order
+  uint32_t    audfs;
+  uint8_t    *audio;          // the clip's audio synced to its first frame (or 0)
+  uint64_t    audio_size;     // bytes that play with the clip's frames
+  uint64_t    audio_buffer_size;
 }
 mlv_clip_t;
 

@@ -71,6 +71,8 @@ def lib():
         L.dt_graph_commit.argtypes = [vp]
         L.dt_graph_read_source.argtypes = [vp, i32, vp]
         L.dt_graph_node_read_source.argtypes = [vp, i32, vp]
+        L.dt_graph_module_audio.argtypes = [vp, i32, u64, u32, C.POINTER(C.c_void_p)]
+        L.dt_graph_img_snd_get.argtypes = [vp, i32, C.c_int * 3]
         rc = L.dt_pipe_global_init(PKG_DIR.encode())
         if rc != 0:
             raise ImportError(f"no module classes found under {PKG_DIR}/modules")
@@ -140,7 +142,10 @@ class Graph:
         self.L.dt_graph_img_param_get(self.g, m, black, white, wb, cam, C.byref(filters), crop, noise, roi)
         meta, maker, model, dtime = f4(), C.create_string_buffer(32), C.create_string_buffer(32), C.create_string_buffer(20)
         self.L.dt_graph_img_meta_get(self.g, m, meta, maker, model, dtime)
+        snd = (C.c_int * 3)()
+        self.L.dt_graph_img_snd_get(self.g, m, snd)
         return dict(black=list(black), white=list(white), wb=list(wb), cam=list(cam), filters=filters.value,
+                    snd_format=snd[0], snd_channels=snd[1], snd_samplerate=snd[2],
                     crop=list(crop), noise=list(noise), full=(roi[0], roi[1]), exposure=meta[0], aperture=meta[1],
                     iso=meta[2], focal_length=meta[3], maker=maker.value.decode(errors="replace"),
                     model=model.value.decode(errors="replace"), datetime=dtime.value.decode(errors="replace"))
@@ -156,6 +161,15 @@ class Graph:
         if self.L.dt_graph_read_source(self.g, m, buf.ctypes.data):
             raise PipeError(f"read_source {module}:{inst} failed: {self.error()}")
         return buf
+
+    def audio(self, module, inst, beg, cnt, bytes_per_sample):
+        """a module's audio callback: (samples returned, their bytes)"""
+        m = self.L.dt_graph_module(self.g, module.encode(), inst.encode())
+        if m < 0:
+            raise PipeError(f"no module {module}:{inst}")
+        p = C.c_void_p()
+        k = self.L.dt_graph_module_audio(self.g, m, beg, cnt, C.byref(p))
+        return k, (C.string_at(p, k * bytes_per_sample) if k > 0 else b"")
 
     def plan(self, run=RUN_ALL):
         if self.L.dt_graph_plan(self.g, run):

@@ -157,6 +157,17 @@ int dt_graph_img_meta_get(const dt_graph_t *g, int m, float meta[4], char maker[
   return 0;
 }
 
+// audio of a video source: snd_format, snd_channels, snd_samplerate
+int dt_graph_img_snd_get(const dt_graph_t *g, int m, int snd[3])
+{
+  if(m < 0 || m >= g->num_modules) return 1;
+  const dt_image_params_t *p = &g->module[m].img_param;
+  snd[0] = p->snd_format;
+  snd[1] = p->snd_channels;
+  snd[2] = p->snd_samplerate;
+  return 0;
+}
+
 // host part of a source module's upload: read_source into caller memory
 // (the staging copy that precedes the device transfer)
 int dt_graph_read_source(dt_graph_t *g, int m, void *buf)
@@ -177,6 +188,13 @@ int dt_graph_node_read_source(dt_graph_t *g, int n, void *buf)
   if(!m || !m->so->read_source) return 1;
   dt_read_source_params_t p = { .node = g->node + n, .a = 0 };
   return m->so->read_source(m, buf, &p);
+}
+
+int dt_graph_module_audio(dt_graph_t *g, int m, uint64_t beg, uint32_t cnt, uint8_t **samples)
+{
+  *samples = 0;
+  if(m < 0 || m >= g->num_modules || !g->module[m].so || !g->module[m].so->audio) return 0;
+  return g->module[m].so->audio(g->module + m, beg, cnt, samples);
 }
 
 // run every module's commit_params (the uniform upload) without the device

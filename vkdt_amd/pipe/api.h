@@ -250,6 +250,10 @@ typedef struct dt_image_params_t
   float    noise_b;           // raw noise, poissonian part
   dt_token_t input_name;
   dt_image_metadata_t *meta;  // owned by the input module that attached it
+  // audio of a video source (module.h:96-99)
+  int      snd_format;        // alsa's snd_pcm_format_t (2: S16_LE)
+  int      snd_channels;      // 0 none, 1, 2 ...
+  int      snd_samplerate;    // Hz
 }
 dt_image_params_t;
 
@@ -309,6 +313,9 @@ typedef struct dt_module_so_t
   // `stream` directly (GPU encode); 0 done, 1 use download + write_sink, -1 error
   int  (*write_sink_hip)(dt_module_t *, const void *dev, void *stream);
   void (*write_sink)(dt_module_t *, void *, dt_write_sink_params_t *);
+  // optional: *samples points at the clip's audio from sample `beg` on;
+  // returns how many of `cnt` samples are there (module.h audio callback)
+  int  (*audio)(dt_module_t *, uint64_t beg, uint32_t cnt, uint8_t **samples);
   dt_ui_param_t *param[DT_MAX_PARAMS];
   int num_params;
   dt_connector_t connector[DT_MAX_CONNECTORS];
@@ -427,6 +434,10 @@ int  dt_graph_set_export_size(dt_graph_t *graph, int module, int max_wd, int max
 // resolve a resource relative to the graph's search path, then basedir
 FILE *dt_graph_open_resource(dt_graph_t *graph, int frame, const char *fname, const char *mode);
 int  dt_graph_get_resource_filename(dt_module_t *mod, const char *fname, int frame, char *out, size_t size);
+// the snd_format, snd_channels and snd_samplerate a module publishes
+int  dt_graph_img_snd_get(const dt_graph_t *graph, int m, int snd[3]);
+// a module's audio callback: samples from `beg` (count returned, 0 none)
+int  dt_graph_module_audio(dt_graph_t *graph, int m, uint64_t beg, uint32_t cnt, uint8_t **samples);
 const char *dt_graph_last_error(const dt_graph_t *graph);
 
 /* ---- module-side helpers (modules/api.h) ------------------------------ */

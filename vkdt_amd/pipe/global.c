@@ -136,6 +136,7 @@ static int load_so(dt_module_so_t *so, const char *dirname)
     so->read_source_hip = dlsym(so->dlhandle, "read_source_hip");
     so->write_sink_hip = dlsym(so->dlhandle, "write_sink_hip");
     so->write_sink     = dlsym(so->dlhandle, "write_sink");
+    so->audio          = dlsym(so->dlhandle, "audio");
   }
   return 0;
 }

@@ -310,7 +310,7 @@ def pack_msb16(samples, bits):
 
 def write_mlv(path, frames, bits=14, lj92=None, black=2048, white=15000, fps=(25000, 1000),
               camera="Canon EOS 5D Mark III", iso=800, shutter_us=20000, aperture=280, focal=50,
-              order=None, junk_before=(), chunk_split=None, frame_space=0, header_frames=None):
+              order=None, junk_before=(), chunk_split=None, frame_space=0, header_frames=None, audio=None):
     """Write a Magic Lantern raw video clip of uint16 CFA frames (H x W, rggb).
 
     ``lj92`` None packs ``bits`` per sample; a dict (components, predictor)
@@ -319,7 +319,10 @@ def write_mlv(path, frames, bits=14, lj92=None, black=2048, white=15000, fps=(25
     the frame index, so a reader must sort); ``junk_before`` puts a block of
     unknown type before those frame indices; ``chunk_split`` k writes frames
     from index k on into ``<name>.M00``; ``header_frames`` overrides the
-    frame count in the file header."""
+    frame count in the file header.  ``audio`` (dict: data bytes, channels,
+    rate, bits, blocks: [(timestamp_us, nbytes)] splitting the data in file
+    order, space: padding before each block's samples) adds a WAVI block and
+    AUDF blocks after the metadata, and sets the header's audio class."""
     import struct
     from vkdt_amd import lj92 as L
     H, W = frames[0].shape
@@ -328,8 +331,10 @@ def write_mlv(path, frames, bits=14, lj92=None, black=2048, white=15000, fps=(25
 
     def mlvi():
         cnt = n if header_frames is None else header_frames
-        return struct.pack("<4sI8sQHHIHHIIII", b"MLVI", 52, b"v2.0\0\0\0\0", 0x1234, 0, 1, 0, vclass, 0,
-                           cnt, 0, fps[0], fps[1])
+        aclass = 1 if audio else 0
+        nblk = len(audio["blocks"]) if audio else 0
+        return struct.pack("<4sI8sQHHIHHIIII", b"MLVI", 52, b"v2.0\0\0\0\0", 0x1234, 0, 1, 0, vclass, aclass,
+                           cnt, nblk, fps[0], fps[1])
 
     raw_info = struct.pack("<II7i4i4i2i2i18ii", 1, 0, H, W, W * bits // 8, W * H * bits // 8, bits, black, white,
                            0, 0, W, H, 0, 0, H, W, 0, 0, 0x02010100, 0, *([0] * 18), 1100)
@@ -349,6 +354,14 @@ def write_mlv(path, frames, bits=14, lj92=None, black=2048, white=15000, fps=(25
         body = struct.pack("<IHHHHI", k, 0, 0, 0, 0, frame_space) + pad + data
         return struct.pack("<4sIQ", b"VIDF", 16 + len(body), 1000 + 40000 * k) + body
 
+    if audio:
+        ch, rate, ab = audio["channels"], audio["rate"], audio.get("bits", 16)
+        meta.append(struct.pack("<4sIQHHIIHH", b"WAVI", 32, 0, 1, ch, rate, rate * ch * ab // 8, ch * ab // 8, ab))
+        space, at = audio.get("space", 0), 0
+        for i, (ts, nb) in enumerate(audio["blocks"]):
+            body = struct.pack("<II", i, space) + b"\0" * space + audio["data"][at:at + nb]
+            at += nb
+            meta.append(struct.pack("<4sIQ", b"AUDF", 16 + len(body), ts) + body)
     pos = list(order) if order is not None else list(range(n))
     split = chunk_split if chunk_split is not None else n + 1
     chunks = [bytearray(mlvi() + b"".join(meta)), bytearray(mlvi())]
