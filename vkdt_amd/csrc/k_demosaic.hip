@@ -41,12 +41,17 @@ __global__ void __launch_bounds__(256) k_dm_down(dimg in, dimg out, dm_args a)
   st_h1(out, x, y, lum);
 }
 
+// Waves whose every tap lies inside the image read without the mirrored
+// addressing (the identity there): the same values, fewer instructions.
 template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_dm_gauss(dimg orig, dimg out)
 {
   const int x = tid_x(), y = tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, dm_gauss_px<XT, U16>(orig, x, y));
+  if(__all(dm_gauss_inside<XT>(orig, x, y)))
+    st_h4(out, x, y, dm_gauss_at<XT>([&](int px, int py) { return ld_mt<U16>(orig, px, py); }, x, y));
+  else
+    st_h4(out, x, y, dm_gauss_px<XT, U16>(orig, x, y));
 }
 
 // texel of this thread: Bayer tiles are handed out colour-uniform per wave
@@ -64,7 +69,11 @@ __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  st_h1(out, x, y, dm_splat_px<XT, U16>(in, ga, x, y));
+  constexpr int R = XT ? 2 : 1;
+  if(__all(x >= R && y >= R && x + R < in.wd && y + R < in.ht))
+    st_h1(out, x, y, dm_splat_at<XT>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y));
+  else
+    st_h1(out, x, y, dm_splat_px<XT, U16>(in, ga, x, y));
 }
 
 template<int XT, int U16>
@@ -73,7 +82,11 @@ __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg 
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  st_h4(out, x, y, dm_fix_px<XT, U16>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
+  if(__all(x >= r && y >= r && x + r < min(in.wd, gr.wd) && y + r < min(in.ht, gr.ht)))
+    st_h4(out, x, y, dm_fix_r<XT>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, cv, r, x, y,
+                                  [&](int px, int py) { return ld_h1(gr, px, py); }));
+  else
+    st_h4(out, x, y, dm_fix_px<XT, U16>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
 }
 
 // ---- halfsize (halfsize.comp:19-50): one rgba texel per CFA block --------

@@ -463,8 +463,9 @@ DEV float dn_doub_px(const dimg &in, const dimg &crs0, const dimg &crs1b, int ow
 // and feed a 2x2 eigen solve whose discriminant vanishes on flat or clipped
 // blocks: they keep IEEE division so the decision follows the written
 // arithmetic.
-template<int XT, int U16 = 0>
-DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
+//   mos(px, py): the mosaic at mirrored (px, py)
+template<int XT, typename MosAt>
+DEV float4 dm_gauss_at(MosAt mos, int x, int y)
 {
   constexpr int n0 = XT ? 0 : -1, n1 = 3, bs = XT ? 3 : 2;
   constexpr int NS = XT ? 5 : 8;
@@ -478,7 +479,7 @@ DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
     for(int i = n0; i < n1; i++)
     {
       if(XT ? (((j + i) & 1) == 1) : (((j + i) & 1) != 1)) continue;
-      px[n] = centre_mt<U16>(orig, bs * x + i, bs * y + j);
+      px[n] = mos(bs * x + i, bs * y + j);
       rp[n] = 1.0f / px[n];
       pi[n] = i; pj[n] = j;
       n++;
@@ -529,6 +530,21 @@ DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
     else { ev[0] = 0.1f; ev[1] = 0.1f; }
   }
   return make_float4(ev[0], ev[1], e0[0], e0[1]);
+}
+
+template<int XT, int U16 = 0>
+DEV float4 dm_gauss_px(const dimg &orig, int x, int y)
+{
+  return dm_gauss_at<XT>([&](int px, int py) { return centre_mt<U16>(orig, px, py); }, x, y);
+}
+
+// the taps of block texel (x, y) lie inside the mosaic: mirroring is the
+// identity there
+template<int XT>
+DEV bool dm_gauss_inside(const dimg &orig, int x, int y)
+{
+  constexpr int n0 = XT ? 0 : -1, bs = XT ? 3 : 2;
+  return bs * x + n0 >= 0 && bs * y + n0 >= 0 && bs * x + 2 < orig.wd && bs * y + 2 < orig.ht;
 }
 
 // gaussian weight of offset (i,j) under cov = (eval, major axis):
