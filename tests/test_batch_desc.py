@@ -89,6 +89,14 @@ def test_refuses_graphs_the_batch_does_not_run(tmp_path):
     with pytest.raises(pipe.PipeError, match="chain"):
         batch.describe(g)
     g.close()
+    # filmcurv's colour modes other than 1 and 3 run on the per-node graph only
+    for mode in (0, 2, 4, 5):
+        g = pipe.Graph(lines=pipe.raw_graph_cfg(dng) + [f"param:filmcurv:01:colour:{mode}"])
+        g.export("o-pfm", str(tmp_path / "out3"))
+        g.plan()
+        with pytest.raises(pipe.PipeError, match="colour modes 1 and 3"):
+            batch.describe(g)
+        g.close()
 
 
 @pytest.mark.parametrize("kind,W,H", [("bayer", 128, 96), ("xtrans", 198, 132)])
