@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) kb_reduce(bt_launch L)
     in = rfl(im.lp[jb.level - 1]); o = rfl(im.lp[jb.level]); planes = rfl(im.ll.num_gamma + 1);
   }, [&](const bt_job &jb, int lt) {
     const int x0 = ((lt % jb.tiles_x) * BX + threadIdx.x) * 4, y = (lt / jb.tiles_x) * BY + threadIdx.y;
-    if(x0 < o.wd && y < o.ht) ll_reduce_px4(in, o, planes, x0, y, true);
+    if(x0 < o.wd && y < o.ht) ll_reduce_px4(in, o, planes, x0, y, 2 | 4);
   });
 }
 

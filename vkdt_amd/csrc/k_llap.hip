@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_ll_reduce(dimg in, dimg out, int vec)
   dimg i1 = in, o1 = out;
   i1.p = (__half *)in.p + (long)blockIdx.z * in.pstride;
   o1.p = (__half *)out.p + (long)blockIdx.z * out.pstride;
-  ll_reduce_px4(i1, o1, 1, x, y, vec != 0);
+  ll_reduce_px4(i1, o1, 1, x, y, vec);
 }
 
 #define AS_R 4   // fine rows per thread: 64 x 16 fine texels per work group
@@ -85,6 +85,47 @@ __global__ void __launch_bounds__(256) k_ll_assemble(dimg coarse, dimg l0, dimg 
     const int y = Y0 + threadIdx.y + 4 * k;
     if(x < out.wd && y < out.ht) st_h1(out, x, y, res[k]);
   }
+}
+
+// The coarse levels, at most LL_SMALL texels: one thread a texel reads its
+// coarse footprint straight from memory (L2 holds these levels) instead of
+// staging the tile's planes in LDS, which costs the tiny levels several
+// dependent memory round trips; the values, weights and the order of the
+// sums are those of ll_assemble_tile, so the result is the same bit for bit.
+#define LL_SMALL (1 << 19)
+__global__ void __launch_bounds__(256) k_ll_assemble_small(dimg coarse, dimg l0, dimg l1, dimg out, ll_args a)
+{
+  const int x = tid_x(), y = tid_y();
+  if(x >= out.wd || y >= out.ht) return;
+  const float v = ld_h1(l0, x, y, a.num_gamma);
+  const int h = gamma_hi_from_v(v, a);
+  const float pv0 = ld_h1(l0, x, y, h - 1), pv1 = ld_h1(l0, x, y, h);
+  const float we[5] = { 0.5f, 0.5f, 1.0f, 0.5f, 0.5f }, wo[5] = { 0.0f, 1.0f, 0.5f, 0.5f, 1.0f };
+  const float *wx = (x & 1) ? wo : we, *wy = (y & 1) ? wo : we;
+  const float al = clampf((v - a.gam[h - 1]) / (a.gam[h] - a.gam[h - 1]), 0.0f, 1.0f);
+  float rs = 0.0f, e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+  for(int j = 0; j < 5; j++)
+  {
+    const int gy = mirror(y / 2 - 2 + j, l1.ht);
+    float rc = 0.0f, r0 = 0.0f, r1 = 0.0f;
+#pragma unroll
+    for(int i = 0; i < 5; i++)
+    {
+      const int gx = mirror(x / 2 - 2 + i, l1.wd);
+      const float c = a.first <= 0 ? ld_h1(coarse, gx, gy) : ld_h1(l1, gx, gy, a.num_gamma);
+      rc += wx[i] * c;
+      r0 += wx[i] * ld_h1(l1, gx, gy, h - 1);
+      r1 += wx[i] * ld_h1(l1, gx, gy, h);
+    }
+    rs += wy[j] * rc;
+    e0 += wy[j] * r0;
+    e1 += wy[j] * r1;
+  }
+  const float s = 1.0f / 9.0f;
+  const float lv0 = pv0 - e0 * s;
+  const float lv1 = pv1 - e1 * s;
+  st_h1(out, x, y, rs * s + lv0 * (1.0f - al) + lv1 * al);
 }
 
 // colour.comp:221-240; output rgba f16 or f32 (f32 when it feeds the export sink)
@@ -130,9 +171,12 @@ int launch_llap_reduce(const node_call &c)
   if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, c.dp, "llap/reduce inhi")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, c.dp, "llap/reduce outlo")) return e;
   const dimg in = to_dimg(c.conn[0]), out = to_dimg(c.conn[1]);
-  // 16-byte input rows and 8-byte output groups at every row and plane start
-  const int vec = in.pitch % 8 == 0 && in.pstride % 8 == 0 && (uintptr_t)in.p % 16 == 0 &&
-                  out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
+  // 16-byte (2) or 8-byte (1) input rows, and 8-byte output groups (4), at
+  // every row and plane start
+  const int vout = out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
+  const int vin = (in.pitch % 8 == 0 && in.pstride % 8 == 0 && (uintptr_t)in.p % 16 == 0) ? 2
+                : (in.pitch % 4 == 0 && in.pstride % 4 == 0 && (uintptr_t)in.p % 8 == 0) ? 1 : 0;
+  const int vec = vin | (vout ? 4 : 0);
   hipLaunchKernelGGL(k_ll_reduce, grid2d((c.wd + 3) / 4, c.ht, c.dp), dim3(BX, BY), 0, c.stream, in, out, vec);
   return launch_post("llap/reduce");
 }
@@ -151,6 +195,12 @@ int launch_llap_assemble(const node_call &c)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "llap/assemble: currlo smaller than output");
   if(!a.first && (c.conn[0].wd != c.conn[2].wd || c.conn[0].ht != c.conn[2].ht))
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "llap/assemble: coarse and currhi extents differ");
+  if((uint64_t)c.wd * c.ht <= LL_SMALL)
+  {
+    hipLaunchKernelGGL(k_ll_assemble_small, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]), a);
+    return launch_post("llap/assemble");
+  }
   const size_t lds = sizeof(float) * (a.num_gamma + 1) * AS_TN(AS_R);
   const dim3 grid((c.wd + BX - 1) / BX, (c.ht + BY * AS_R - 1) / (BY * AS_R), 1);
   hipLaunchKernelGGL(k_ll_assemble, grid, dim3(BX, BY), lds, c.stream,

@@ -876,18 +876,20 @@ DEV void ll_reduce_px(const dimg &in, const dimg &out, int planes, int x, int y)
 }
 
 // four outputs (x0..x0+3, y) of ll_reduce_px.  When the footprint
-// 2x0-1 .. 2x0+7 lies inside the row and rows and planes are 16-byte
-// aligned (`vec`), the texels 2x0 .. 2x0+7 of each input row arrive in one
-// 16-byte load and the four results leave in one 8-byte store; the sums are
-// ll_reduce_px's, term for term.
-DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int y, bool vec)
+// 2x0-1 .. 2x0+7 lies inside the row and rows and planes are aligned
+// (`vec` bits 0-1: 2 for 16-byte input rows, 1 for 8-byte ones), the texels
+// 2x0 .. 2x0+7 of each input row arrive in one 16-byte load (two 8-byte
+// loads), and the four results leave in one 8-byte store where output rows
+// are 8-byte aligned (`vec` bit 2); the sums are ll_reduce_px's, term for
+// term.
+DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int y, int vec)
 {
-  if(!vec || 2 * x0 + 8 > in.wd)
+  if(!(vec & 3) || 2 * x0 + 8 > in.wd)
   {
     for(int k = 0; k < 4; k++) if(x0 + k < out.wd) ll_reduce_px(in, out, planes, x0 + k, y);
     return;
   }
-  const bool vout = x0 + 4 <= out.wd;
+  const bool vout = (vec & 4) && x0 + 4 <= out.wd;
   const int xl = mirror(2 * x0 - 1, in.wd);
   const long rw[3] = { (long)mirror(2 * y - 1, in.ht) * in.pitch, (long)mirror(2 * y, in.ht) * in.pitch,
                        (long)mirror(2 * y + 1, in.ht) * in.pitch };
@@ -902,8 +904,13 @@ DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int 
     {
       float c[9];
       c[0] = h2f(q[rw[r] + xl]);
-      union { uint4 u; __half e[8]; } v;
-      v.u = *(const uint4 *)(q + rw[r] + 2 * x0);
+      union { uint4 u; uint2 h[2]; __half e[8]; } v;
+      if((vec & 3) == 2) v.u = *(const uint4 *)(q + rw[r] + 2 * x0);
+      else
+      {
+        v.h[0] = *(const uint2 *)(q + rw[r] + 2 * x0);
+        v.h[1] = *(const uint2 *)(q + rw[r] + 2 * x0 + 4);
+      }
 #pragma unroll
       for(int i = 0; i < 8; i++) c[i + 1] = h2f(v.e[i]);
 #pragma unroll
