@@ -171,11 +171,12 @@ int launch_llap_reduce(const node_call &c)
   if(int e = check_conn(c, 0, VKDT_FMT_F16, 1, c.dp, "llap/reduce inhi")) return e;
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, c.dp, "llap/reduce outlo")) return e;
   const dimg in = to_dimg(c.conn[0]), out = to_dimg(c.conn[1]);
-  // 16-byte (2) or 8-byte (1) input rows, and 8-byte output groups (4), at
-  // every row and plane start
+  // 16-byte (2), 8-byte (1) or 4-byte (3) input rows, and 8-byte output
+  // groups (4), at every row and plane start
   const int vout = out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
   const int vin = (in.pitch % 8 == 0 && in.pstride % 8 == 0 && (uintptr_t)in.p % 16 == 0) ? 2
-                : (in.pitch % 4 == 0 && in.pstride % 4 == 0 && (uintptr_t)in.p % 8 == 0) ? 1 : 0;
+                : (in.pitch % 4 == 0 && in.pstride % 4 == 0 && (uintptr_t)in.p % 8 == 0) ? 1
+                : (in.pitch % 2 == 0 && in.pstride % 2 == 0 && (uintptr_t)in.p % 4 == 0) ? 3 : 0;
   const int vec = vin | (vout ? 4 : 0);
   hipLaunchKernelGGL(k_ll_reduce, grid2d((c.wd + 3) / 4, c.ht, c.dp), dim3(BX, BY), 0, c.stream, in, out, vec);
   return launch_post("llap/reduce");

@@ -877,9 +877,10 @@ DEV void ll_reduce_px(const dimg &in, const dimg &out, int planes, int x, int y)
 
 // four outputs (x0..x0+3, y) of ll_reduce_px.  When the footprint
 // 2x0-1 .. 2x0+7 lies inside the row and rows and planes are aligned
-// (`vec` bits 0-1: 2 for 16-byte input rows, 1 for 8-byte ones), the texels
-// 2x0 .. 2x0+7 of each input row arrive in one 16-byte load (two 8-byte
-// loads), and the four results leave in one 8-byte store where output rows
+// (`vec` bits 0-1: 2 for 16-byte input rows, 1 for 8-byte ones, 3 for
+// 4-byte ones), the texels 2x0 .. 2x0+7 of each input row arrive in one
+// 16-byte load (two 8-byte, four 4-byte loads), and the four results leave
+// in one 8-byte store where output rows
 // are 8-byte aligned (`vec` bit 2); the sums are ll_reduce_px's, term for
 // term.
 DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int y, int vec)
@@ -906,10 +907,15 @@ DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int 
       c[0] = h2f(q[rw[r] + xl]);
       union { uint4 u; uint2 h[2]; __half e[8]; } v;
       if((vec & 3) == 2) v.u = *(const uint4 *)(q + rw[r] + 2 * x0);
-      else
+      else if((vec & 3) == 1)
       {
         v.h[0] = *(const uint2 *)(q + rw[r] + 2 * x0);
         v.h[1] = *(const uint2 *)(q + rw[r] + 2 * x0 + 4);
+      }
+      else
+      {
+        const uint32_t *w = (const uint32_t *)(q + rw[r] + 2 * x0);
+        v.u = make_uint4(w[0], w[1], w[2], w[3]);
       }
 #pragma unroll
       for(int i = 0; i < 8; i++) c[i + 1] = h2f(v.e[i]);
