@@ -1,7 +1,8 @@
 // k_colour.hip -- colour (matrix path) and filmcurv per-pixel nodes.
 //   colour/main   src/pipe/modules/colour/main-impl.glsl:200-341
 //   filmcurv/main src/pipe/modules/filmcurv/main.comp:68-167 (colour modes 0..5)
-// Both read and write rgba f16 W x H with one 8-byte load and store per texel.
+// Both read and write rgba f16 W x H: two texels a thread with one 16-byte
+// load and store where the buffers allow, else one texel with 8 bytes.
 #include "launch.h"
 #include "ops.h"
 
@@ -23,6 +24,62 @@ __global__ void __launch_bounds__(256) k_fc_main(dimg in, dimg out, fc_args a)
   st_h4(out, x, y, COLOUR ? fc_px(t, a) : fc_px_basic(t, a));
 }
 
+// Two texels a thread, one 16-byte load and store of the pair, where rows
+// start 16-byte aligned, the extents of input and output agree and the pair
+// lies inside the row (`in` and `out` even pitch, aligned bases).  The same
+// per-texel arithmetic as above.
+DEV void ld_h4x2(const dimg &b, int x, int y, float4 &p, float4 &q)
+{
+  const uint4 u = *(const uint4 *)((const __half *)b.p + 4 * ((long)y * b.pitch + x));
+  const __half2 a = *(const __half2 *)&u.x, c = *(const __half2 *)&u.y;
+  const __half2 d = *(const __half2 *)&u.z, e = *(const __half2 *)&u.w;
+  p = make_float4(__low2float(a), __high2float(a), __low2float(c), __high2float(c));
+  q = make_float4(__low2float(d), __high2float(d), __low2float(e), __high2float(e));
+}
+
+DEV void st_h4x2(const dimg &b, int x, int y, float4 p, float4 q)
+{
+  const __half2 a = __floats2half2_rn(p.x, p.y), c = __floats2half2_rn(p.z, p.w);
+  const __half2 d = __floats2half2_rn(q.x, q.y), e = __floats2half2_rn(q.z, q.w);
+  uint4 u;
+  u.x = *(const uint32_t *)&a;
+  u.y = *(const uint32_t *)&c;
+  u.z = *(const uint32_t *)&d;
+  u.w = *(const uint32_t *)&e;
+  *(uint4 *)((__half *)b.p + 4 * ((long)y * b.pitch + x)) = u;
+}
+
+template<typename Px>
+DEV void px_pair(const dimg &in, const dimg &out, Px px)
+{
+  const int x = 2 * tid_x(), y = tid_y();
+  if(x >= out.wd || y >= out.ht) return;
+  if(x + 1 < out.wd)
+  {
+    float4 p, q;
+    ld_h4x2(in, x, y, p, q);
+    st_h4x2(out, x, y, px(p), px(q));
+  }
+  else st_h4(out, x, y, px(ld_h4(in, x, y)));
+}
+
+static bool pairs_ok(const vkdt_buf_t &i, const vkdt_buf_t &o)
+{
+  return i.wd == o.wd && i.ht == o.ht && i.pitch % 2 == 0 && o.pitch % 2 == 0 && (uintptr_t)i.ptr % 16 == 0 &&
+         (uintptr_t)o.ptr % 16 == 0;
+}
+
+__global__ void __launch_bounds__(256) k_co_main2(dimg in, dimg out, co_args a)
+{
+  px_pair(in, out, [&](float4 t) { return co_px(t, a); });
+}
+
+template<bool COLOUR>
+__global__ void __launch_bounds__(256) k_fc_main2(dimg in, dimg out, fc_args a)
+{
+  px_pair(in, out, [&](float4 t) { return COLOUR ? fc_px(t, a) : fc_px_basic(t, a); });
+}
+
 // ---- host launchers --------------------------------------------------------
 
 int launch_colour_main(const node_call &c)
@@ -38,8 +95,12 @@ int launch_colour_main(const node_call &c)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "colour/main: only the matrix path (no clut/pick/abney) is supported");
   co_args a;
   if(const char *err = co_args_init(a, c.params)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "colour/main: %s", err);
-  hipLaunchKernelGGL(k_co_main, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+  if(pairs_ok(c.conn[0], c.conn[1]) && c.wd == c.conn[1].wd && c.ht == c.conn[1].ht)
+    hipLaunchKernelGGL(k_co_main2, grid2d((c.wd + 1) / 2, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+  else
+    hipLaunchKernelGGL(k_co_main, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
   return launch_post("colour/main");
 }
 
@@ -50,7 +111,12 @@ int launch_filmcurv_main(const node_call &c)
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "filmcurv/main output")) return e;
   fc_args a;
   if(const char *err = fc_args_init(a, c.params, c.params_size, false)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "filmcurv/main: %s", err);
-  hipLaunchKernelGGL((a.mode == 1 || a.mode == 3) ? k_fc_main<false> : k_fc_main<true>, grid2d(c.wd, c.ht, 1),
-                     dim3(BX, BY), 0, c.stream, to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+  const bool basic = a.mode == 1 || a.mode == 3;
+  if(pairs_ok(c.conn[0], c.conn[1]) && c.wd == c.conn[1].wd && c.ht == c.conn[1].ht)
+    hipLaunchKernelGGL(basic ? k_fc_main2<false> : k_fc_main2<true>, grid2d((c.wd + 1) / 2, c.ht, 1), dim3(BX, BY),
+                       0, c.stream, to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
+  else
+    hipLaunchKernelGGL(basic ? k_fc_main<false> : k_fc_main<true>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
   return launch_post("filmcurv/main");
 }
