@@ -139,6 +139,40 @@ __global__ void __launch_bounds__(256) k_ll_colour(dimg lum, dimg org, dimg out)
   else    st_h4(out, x, y, o);
 }
 
+// two texels a thread where lum, org and out have the output's extent and
+// even, aligned rows: a 4-byte luminance pair, a 16-byte rgba f16 pair, and
+// two 16-byte (f32) or one 16-byte (f16) store; the arithmetic of k_ll_colour
+template<int F32>
+__global__ void __launch_bounds__(256) k_ll_colour2(dimg lum, dimg org, dimg out)
+{
+  const int x = 2 * tid_x(), y = tid_y();
+  if(x >= out.wd || y >= out.ht) return;
+  if(x + 1 >= out.wd)
+  {
+    const float4 o = ll_colour_px(ld_h1(lum, x, y), ld_h4(org, x, y));
+    if(F32) st_f4(out, x, y, o);
+    else    st_h4(out, x, y, o);
+    return;
+  }
+  const __half2 l = *(const __half2 *)((const __half *)lum.p + (long)y * lum.pitch + x);
+  const uint4 u = *(const uint4 *)((const __half *)org.p + 4 * ((long)y * org.pitch + x));
+  const __half2 a = *(const __half2 *)&u.x, c = *(const __half2 *)&u.y;
+  const __half2 d = *(const __half2 *)&u.z, e = *(const __half2 *)&u.w;
+  const float4 o0 = ll_colour_px(__low2float(l), make_float4(__low2float(a), __high2float(a), __low2float(c), __high2float(c)));
+  const float4 o1 = ll_colour_px(__high2float(l), make_float4(__low2float(d), __high2float(d), __low2float(e), __high2float(e)));
+  if(F32)
+  {
+    float4 *q = (float4 *)((float *)out.p + 4 * ((long)y * out.pitch + x));
+    q[0] = o0;
+    q[1] = o1;
+  }
+  else
+  {
+    st_h4(out, x, y, o0);
+    st_h4(out, x + 1, y, o1);
+  }
+}
+
 // ---- host launchers --------------------------------------------------------
 
 static int read_args(ll_args &a, const node_call &c, int ng)
@@ -215,7 +249,14 @@ int launch_llap_colour(const node_call &c)
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 4, 1, "llap/colour input")) return e;
   const int f32 = c.nconn > 2 && c.conn[2].format == VKDT_FMT_F32;
   if(int e = check_conn(c, 2, f32 ? VKDT_FMT_F32 : VKDT_FMT_F16, 4, 1, "llap/colour output")) return e;
-  if(f32)
+  const vkdt_buf_t &l = c.conn[0], &o = c.conn[1], &q = c.conn[2];
+  const bool pairs = l.wd == q.wd && l.ht == q.ht && o.wd == q.wd && o.ht == q.ht && c.wd == q.wd && c.ht == q.ht &&
+                     l.pitch % 2 == 0 && o.pitch % 2 == 0 && q.pitch % 2 == 0 && (uintptr_t)l.ptr % 4 == 0 &&
+                     (uintptr_t)o.ptr % 16 == 0 && (uintptr_t)q.ptr % 16 == 0;
+  if(pairs)
+    hipLaunchKernelGGL(f32 ? k_ll_colour2<1> : k_ll_colour2<0>, grid2d((c.wd + 1) / 2, c.ht, 1), dim3(BX, BY), 0,
+                       c.stream, to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]));
+  else if(f32)
     hipLaunchKernelGGL(k_ll_colour<1>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
                        to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]));
   else
