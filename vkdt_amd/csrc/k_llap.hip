@@ -46,6 +46,47 @@ __global__ void __launch_bounds__(256) k_ll_curve(dimg in, dimg out, ll_args a, 
   *(uint2 *)(po + a.num_gamma * out.pstride) = w.u;
 }
 
+// eight adjacent texels per thread where the input rows are 16-byte aligned
+// and the output rows and planes too: four 16-byte loads, and one 16-byte
+// store a plane; the values of k_ll_curve
+__global__ void __launch_bounds__(256) k_ll_curve8(dimg in, dimg out, ll_args a)
+{
+  const int x0 = 8 * tid_x(), y = tid_y();
+  if(x0 >= in.wd || y >= in.ht) return;
+  if(x0 + 8 > in.wd)
+  {
+    for(int k = 0; k < 8 && x0 + k < in.wd; k++)
+    {
+      const float yl = ll_lum(ld_h4(in, x0 + k, y));
+      for(int i = 0; i < a.num_gamma; i++) st_h1(out, x0 + k, y, ll_curve(yl, a.gam[i], a), i);
+      st_h1(out, x0 + k, y, yl, a.num_gamma);
+    }
+    return;
+  }
+  float yl[8];
+  const uint4 *pi = (const uint4 *)((const __half *)in.p + 4 * ((long)y * in.pitch + x0));
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+  {
+    const uint4 u = pi[k];
+    const __half2 a0 = *(const __half2 *)&u.x, c0 = *(const __half2 *)&u.y;
+    const __half2 a1 = *(const __half2 *)&u.z, c1 = *(const __half2 *)&u.w;
+    yl[2 * k] = ll_lum(make_float4(__low2float(a0), __high2float(a0), __low2float(c0), __high2float(c0)));
+    yl[2 * k + 1] = ll_lum(make_float4(__low2float(a1), __high2float(a1), __low2float(c1), __high2float(c1)));
+  }
+  __half *po = (__half *)out.p + (long)y * out.pitch + x0;
+  union { uint4 u; __half e[8]; } w;
+  for(int i = 0; i < a.num_gamma; i++)
+  {
+#pragma unroll
+    for(int k = 0; k < 8; k++) w.e[k] = f2h(ll_curve(yl[k], a.gam[i], a));
+    *(uint4 *)(po + i * out.pstride) = w.u;
+  }
+#pragma unroll
+  for(int k = 0; k < 8; k++) w.e[k] = f2h(yl[k]);
+  *(uint4 *)(po + a.num_gamma * out.pstride) = w.u;
+}
+
 // four adjacent output texels of one plane (blockIdx.z) per thread: the
 // planes' loads are independent, so they go to separate waves rather than
 // one thread's serial loop
@@ -193,7 +234,14 @@ int launch_llap_curve(const node_call &c)
   if(int e = check_conn(c, 1, VKDT_FMT_F16, 1, a.num_gamma + 1, "llap/curve output")) return e;
   if(c.conn[1].wd < c.conn[0].wd || c.conn[1].ht < c.conn[0].ht)
     return vkdt_set_error(VKDT_HIP_INCOMPLETE, "llap/curve: output smaller than input");
-  const dimg out = to_dimg(c.conn[1]);
+  const dimg out = to_dimg(c.conn[1]), in = to_dimg(c.conn[0]);
+  if(out.pitch % 8 == 0 && out.pstride % 8 == 0 && (uintptr_t)out.p % 16 == 0 && in.pitch % 2 == 0 &&
+     (uintptr_t)in.p % 16 == 0)
+  {
+    hipLaunchKernelGGL(k_ll_curve8, grid2d((c.conn[0].wd + 7) / 8, c.conn[0].ht, 1), dim3(BX, BY), 0, c.stream, in,
+                       out, a);
+    return launch_post("llap/curve");
+  }
   const int vec = out.pitch % 4 == 0 && out.pstride % 4 == 0 && (uintptr_t)out.p % 8 == 0;
   hipLaunchKernelGGL(k_ll_curve, grid2d((c.conn[0].wd + 3) / 4, c.conn[0].ht, 1), dim3(BX, BY), 0, c.stream,
                      to_dimg(c.conn[0]), out, a, vec);
