@@ -380,7 +380,11 @@ __global__ void __launch_bounds__(256) kb_gauss(bt_launch L)
     for(int k = 0; k < L.rows; k++)
     {
       const int y = Y0 + threadIdx.y + BY * k;
-      if(x < o.wd && y < o.ht) st_h4(o, x, y, xt ? dm_gauss_px<1>(in, x, y) : dm_gauss_px<0>(in, x, y));
+      if(x >= o.wd || y >= o.ht) continue;
+      // waves whose taps all lie inside the mosaic read it without mirroring
+      auto direct = [&](int px, int py) { return ld_mt<0>(in, px, py); };
+      if(xt) st_h4(o, x, y, __all(dm_gauss_inside<1>(in, x, y)) ? dm_gauss_at<1>(direct, x, y) : dm_gauss_px<1>(in, x, y));
+      else   st_h4(o, x, y, __all(dm_gauss_inside<0>(in, x, y)) ? dm_gauss_at<0>(direct, x, y) : dm_gauss_px<0>(in, x, y));
     }
   });
 }
