@@ -21,6 +21,7 @@ import oracle_py as op
 import test_filmcurv_modes as film
 import test_gainmap as gain
 import test_mcraw as mcraw
+import test_oracle_pins as basic
 import test_oracle_pins_stencils as pins
 
 ORACLE = os.path.join(op.ROOT, "oracle")
@@ -80,6 +81,9 @@ MUTANTS = [
     ("agx_hue_mix_0.6", "lerp_hue(h0[0], h1[0], R(0.4))", "lerp_hue(h0[0], h1[0], R(0.6))", "filmcurv/main.comp:46"),
     ("lum_ratio_square_root", "L0 * POW(lum1 / lum0, R(1.0) / R(3.0))", "L0 * POW(lum1 / lum0, R(1.0) / R(2.0))",
      "filmcurv/main.comp:135"),
+    ("llap_colour_no_rolloff", "if(yv < yo) yv = yo * EXP(cc * (yv - yo));", "", "llap/colour.comp:229"),
+    ("llap_colour_rolloff_sign", "if(yv < yo) yv = yo * EXP(cc * (yv - yo));", "if(yv < yo) yv = yo * EXP(cc * (yo - yv));",
+     "llap/colour.comp:229"),
 ]
 
 
@@ -114,7 +118,7 @@ def test_mutant_fails_a_pin(mutants, mid):
     saved = op._libs.get(64)
     op._libs[64] = lib
     try:
-        for check in pins.CHECKS + gain.CHECKS + mcraw.CHECKS + film.CHECKS:
+        for check in pins.CHECKS + gain.CHECKS + mcraw.CHECKS + film.CHECKS + [basic.check_llap_colour_transfer_and_black_rolloff]:
             try:
                 check()
             except AssertionError:

@@ -286,6 +286,43 @@ def test_filmcurv_dng_hue_constancy():
 # llap (llap/curve.comp, reduce.comp, assemble.comp, colour.comp)
 # ---------------------------------------------------------------------------
 
+def llap_colour_node(yv, rgb):
+    n = len(yv)
+    lum = np.ascontiguousarray(np.asarray(yv, np.float64).reshape(1, n))
+    org = np.ascontiguousarray(np.concatenate([rgb, np.ones((n, 1))], 1).reshape(1, n, 4))
+    out = np.zeros((1, n, 4))
+    op.node(64, "llap", "colour", None, None, [op.orbuf(lum, op.OR_F32, 1), op.orbuf(org, op.OR_F32, 4),
+                                                op.orbuf(out, op.OR_F32, 4)], n, 1, storage_f16=False)
+    return out[0, :, :3]
+
+
+LUMA2020 = np.array([0.26270021201126692, 0.677998071518871148, 0.0593017164698619384])   # Y of rec2020 (matrices.h:49)
+
+
+def check_llap_colour_transfer_and_black_rolloff():
+    """colour.comp:221-240: the output is the input colour scaled to the new
+    luminance yv where yv is at least the input's luminance yo; below it the
+    luminance rolls off smoothly instead of reaching 0: it rises with yv,
+    meets yo at yv = yo, stays above 0 for yv far below, and the colour keeps
+    its channel ratios"""
+    rng = np.random.default_rng(12)
+    rgb = rng.uniform(0.05, 1.0, size=(1, 3)).repeat(201, 0)
+    yo = float(rgb[0] @ LUMA2020)
+    yv = yo + np.linspace(-3.0, 1.0, 201)
+    out = llap_colour_node(yv, rgb)
+    ly = out @ LUMA2020
+    above = yv >= yo
+    assert np.allclose(ly[above], yv[above], rtol=1e-7)
+    assert np.all(np.diff(ly) > 0) and np.all(ly > 0)
+    below = ~above
+    assert np.abs(ly[below].max() - yo) < 0.03 * yo and np.all(ly[below] <= yo + 1e-12)
+    assert np.allclose(out / out[:, :1], rgb / rgb[:, :1], rtol=1e-9)
+
+
+def test_llap_colour_transfer_and_black_rolloff():
+    check_llap_colour_transfer_and_black_rolloff()
+
+
 def test_llap_reduce_is_binomial_with_mirrored_border():
     """sample_semisoft at 2*o+.5 equals [1 2 1]/4 x [1 2 1]/4 centred on 2*o"""
     rng = np.random.default_rng(10)
