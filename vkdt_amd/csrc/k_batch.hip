@@ -397,6 +397,24 @@ __global__ void __launch_bounds__(256) kb_gauss(bt_launch L)
 // (no mirroring or border wrap can occur there); border tiles read the
 // mosaic through the bounds-checked global path.  Both see the same values.
 #define DEV_M 5   // r + rs <= 3 + 2
+
+// (row, column) of the elements i0, i0 + step, ... of a row-major tile w
+// wide, stepped without a division per element
+struct tile_walk
+{
+  int ty, tx, dq, dr, w;
+  __device__ tile_walk(int i0, int step, int w_) : w(w_)
+  {
+    ty = i0 / w; tx = i0 - ty * w;
+    dq = step / w; dr = step - dq * w;
+  }
+  __device__ void next()
+  {
+    tx += dr; ty += dq;
+    if(tx >= w) { tx -= w; ty++; }
+  }
+};
+
 template<int XT>
 DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const co_args &co, const fc_args &fc,
                       int r, int X0, int Y0, float *gt, float *mt)
@@ -410,28 +428,28 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
   auto green = [&](int px, int py) { return gt[(py - Y0 + r) * gw + (px - X0 + r)]; };
   if(X0 - m >= 0 && Y0 - m >= 0 && X0 + BX + m <= W && Y0 + BY * DEV_R + m <= H)
   {
-    for(int i = tid; i < mw * mh; i += BX * BY)
-    {
-      const int ty = i / mw, tx = i - ty * mw;
-      mt[i] = ld_h1(mos, X0 - m + tx, Y0 - m + ty);
-    }
+    tile_walk wm(tid, BX * BY, mw);
+    for(int i = tid; i < mw * mh; i += BX * BY, wm.next())
+      mt[i] = ld_h1(mos, X0 - m + wm.tx, Y0 - m + wm.ty);
     __syncthreads();
     auto mosl = [&](int px, int py) { return mt[(py - Y0 + m) * mw + (px - X0 + m)]; };
     if(XT)
-      for(int i = tid; i < gw * gh; i += BX * BY)
-      {
-        const int ty = i / gw, tx = i - ty * gw;
-        gt[i] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
-      }
+    {
+      tile_walk wg(tid, BX * BY, gw);
+      for(int i = tid; i < gw * gh; i += BX * BY, wg.next())
+        gt[i] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + wg.tx, Y0 - r + wg.ty));
+    }
     else
     { // Bayer: the sites with tx + ty even first, then the odd ones, so that
       // consecutive lanes share one splat pattern (X0, Y0 are even and gw is
       // even, so the parity of tx + ty is that of the texel's x + y)
       const int hw = gw / 2, ht2 = gh * hw;
-      for(int e = tid; e < gw * gh; e += BX * BY)
+      int c = tid >= ht2;
+      tile_walk wf(tid - c * ht2, BX * BY, hw);
+      for(int e = tid; e < gw * gh; e += BX * BY, wf.next())
       {
-        const int c = e >= ht2, f = e - c * ht2;
-        const int ty = f / hw, tx = 2 * (f - ty * hw) + ((ty + c) & 1);
+        if(!c && e >= ht2) { c = 1; wf = tile_walk(e - ht2, BX * BY, hw); }
+        const int ty = wf.ty, tx = 2 * wf.tx + ((ty + c) & 1);
         gt[ty * gw + tx] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
       }
     }
@@ -447,11 +465,9 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
     }
     return;
   }
-  for(int i = tid; i < gw * gh; i += BX * BY)
-  {
-    const int ty = i / gw, tx = i - ty * gw;
-    gt[i] = r16(dm_splat_px<XT>(mos, cov, mirror(X0 - r + tx, W), mirror(Y0 - r + ty, H)));
-  }
+  tile_walk wg(tid, BX * BY, gw);
+  for(int i = tid; i < gw * gh; i += BX * BY, wg.next())
+    gt[i] = r16(dm_splat_px<XT>(mos, cov, mirror(X0 - r + wg.tx, W), mirror(Y0 - r + wg.ty, H)));
   __syncthreads();
 #pragma unroll 1
   for(int k = 0; k < DEV_R; k++)
