@@ -66,9 +66,14 @@ DEV __half f2h(float f) { return __float2half_rn(f); }
 // ---- loads -------------------------------------------------------------
 
 DEV float ld_u16(const dimg &b, int x, int y)
-{ // R16_UNORM: the float32 value v/65535
-  const uint16_t v = ((const uint16_t *)b.p)[(long)y * b.pitch + x];
-  return (float)v / 65535.0f;
+{ // R16_UNORM: the float32 value v/65535, correctly rounded.  The product
+  // with the rounded reciprocal and one FMA correction step equal the IEEE
+  // quotient for each of the 65536 codes (an exhaustive check), in three
+  // instructions instead of the division's ten
+  const float v = (float)((const uint16_t *)b.p)[(long)y * b.pitch + x];
+  const float r = 1.0f / 65535.0f;
+  const float q = v * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, 65535.0f, v), r, q);
 }
 
 DEV void st_u16(const dimg &b, int x, int y, float v)

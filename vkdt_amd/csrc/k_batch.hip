@@ -268,6 +268,20 @@ __global__ void __launch_bounds__(256) kb_doub(bt_launch L)
     lds_copy(a, im.dn_doub);
     cfa = rfl(im.cfa); crs0 = rfl(im.qa); crs1 = rfl(im.qh); o = rfl(im.dmos);
   }, [&](const bt_job &jb, int lt) {
+    if(a.filters != 9u)
+    { // Bayer: 2x2 blocks per thread, tiles of 2 BX x 2 BY rows texels
+      const int x = ((lt % jb.tiles_x) * BX + threadIdx.x) * 2, Y0 = (lt / jb.tiles_x) * (BY * L.rows);
+#pragma unroll 1
+      for(int k = 0; k < L.rows; k++)
+      {
+        const int y = (Y0 + threadIdx.y + BY * k) * 2;
+        if(x >= o.wd || y >= o.ht) continue;
+        float v[4];
+        dn_doub_quad(cfa, crs0, crs1, o.wd, o.ht, a, x, y, v);
+        st_h1_quad(o, x, y, v);
+      }
+      return;
+    }
     PIX_TILE(L, jb, lt, X0, Y0);
     const int x = X0 + threadIdx.x;
 #pragma unroll 1
@@ -891,7 +905,7 @@ static int tiles_of(int kind, const dims_t &m, int level, int rows, int &tx)
     case K_HALF: case K_DNASM: w = m.qw; h = m.qh; th = BY * rows; break;
     case K_DOWNCOV: w = m.qw; h = m.qh; th = BY * COV_R; break;
     case K_DOWN: w = m.qw; h = m.qh; th = BY * DOWN_R; break;
-    case K_DOUB: w = m.W; h = m.H; th = BY * rows; break;
+    case K_DOUB: w = m.block == 2 ? (m.W + 1) / 2 : m.W; h = m.block == 2 ? (m.H + 1) / 2 : m.H; th = BY * rows; break;
     case K_HL_HALF: w = m.qw; h = m.qh; th = BY * rows; break;
     case K_HL_REDUCE: w = m.hw[level]; h = m.hh[level]; break;
     case K_HL_ASM: w = m.hw[level]; h = m.hh[level]; th = BY * rows; break;

@@ -100,6 +100,25 @@ __global__ void __launch_bounds__(256) k_dn_doub(dimg in, dimg crs0, dimg crs1b,
   st_h1(out, x, y, v);
 }
 
+// Bayer mosaics: one 2x2 block per thread (dn_doub_quad)
+__global__ void __launch_bounds__(256) k_dn_doub_bayer(dimg in, dimg crs0, dimg crs1b, dimg out, dimg gm, dn_args a)
+{
+  const int x = 2 * tid_x(), y = 2 * tid_y();
+  if(x >= out.wd || y >= out.ht) return;
+  float v[4];
+  dn_doub_quad(in, crs0, crs1b, out.wd, out.ht, a, x, y, v);
+  if(a.gainmap)
+#pragma unroll
+    for(int k = 0; k < 4; k++)
+    {
+      const int xx = x + (k & 1), yy = y + (k >> 1);
+      const float px = (0.5f + (float)((xx + a.crop[0]) / 2)) / (float)(out.wd / 2);
+      const float py = (0.5f + (float)((yy + a.crop[1]) / 2)) / (float)(out.ht / 2);
+      v[k] *= dn_gain_at(gm, a, px, py, xx, yy);
+    }
+  st_h1_quad(out, x, y, v);
+}
+
 // the gain map binding of noop (connector 2) and doub (connector 4): the
 // map applies when the push constant and the module's `gainmap` parameter
 // (uniform word 8) are both 1 on a Bayer mosaic (filters neither 0 nor 9);
@@ -229,8 +248,13 @@ int launch_denoise_doub(const node_call &c)
   a.noise_b = pf[18];
   read_params(a, c);
   if(int e = read_gainmap(a, c, 4, pi[19], pf + 20, "denoise/doub gainmap")) return e;
-  hipLaunchKernelGGL(k_dn_doub, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
-                     to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]),
-                     a.gainmap ? to_dimg(c.conn[4]) : dimg{}, a);
+  if(a.filters == 9)
+    hipLaunchKernelGGL(k_dn_doub, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]),
+                       a.gainmap ? to_dimg(c.conn[4]) : dimg{}, a);
+  else
+    hipLaunchKernelGGL(k_dn_doub_bayer, grid2d((c.wd + 1) / 2, (c.ht + 1) / 2, 1), dim3(BX, BY), 0, c.stream,
+                       to_dimg(c.conn[0]), to_dimg(c.conn[1]), to_dimg(c.conn[2]), to_dimg(c.conn[3]),
+                       a.gainmap ? to_dimg(c.conn[4]) : dimg{}, a);
   return launch_post("denoise/doub");
 }

@@ -409,6 +409,28 @@ DEV float4 dn_assemble_px(const dimg &s0, const dimg &s1, const dimg &s2, const 
   return make_float4(rgb[0], rgb[1], rgb[2], test);
 }
 
+// doub.comp:80-116: the texel's shrinkage against the interpolated coarse
+// levels (channel ch of the upsampled level crs and of the finer one crs1,
+// alpha upw of the upsampled level, threshold T) and its normalisation
+DEV float dn_doub_tail(const dn_args &a, int ch, float crs, float crs1, float upw, float T, float blendw, float val)
+{
+  const float black = a.black[ch], white = a.white[ch], inv_range = a.inv_range[ch];
+  float sig[3];
+  noise_sigma(sig, a, black, inv_range, crs);
+  blendw = 0.5f * (blendw + 1.0f);
+  if(val < white)
+  {
+    const float wav = (val - crs1) * frcp(fmaxf(sig[0] + sig[2], 1e-8f));
+    const float tt = fminf(1.0f, wav * frcp(fmaxf(2.0f * T, 1e-8f)));
+    float uw = fminf(1.0f, 1.0f * upw);
+    uw *= uw; uw *= uw;    // ^4
+    uw = 1.0f - (1.0f - uw) * a.detail;
+    const float aw = fabsf(wav);
+    val = mixf(val, fmaxf(0.0f, crs + sig[1] * signf_(wav) * mixf(fmaxf(0.0f, aw - T), aw, tt)), uw * blendw);
+  }
+  return fmaxf(0.0f, (val - black) * inv_range);
+}
+
 // doub.comp:35-116: one mosaic texel at full resolution
 DEV float dn_doub_px(const dimg &in, const dimg &crs0, const dimg &crs1b, int owd, int oht, const dn_args &a,
                      int x, int y)
@@ -437,22 +459,100 @@ DEV float dn_doub_px(const dimg &in, const dimg &crs0, const dimg &crs1b, int ow
     if(ox == 0 && oy == 0)      { ch = 0; crs = upsm.x; crs1 = down.x; blendw = 1.0f; }
     else if(ox == 1 && oy == 1) { ch = 2; crs = upsm.z; crs1 = down.z; blendw = 1.0f; }
   }
-  const float black = a.black[ch], white = a.white[ch], inv_range = a.inv_range[ch];
-  float sig[3];
-  noise_sigma(sig, a, black, inv_range, crs);
-  float val = centre_u16(in, x + a.crop[0], y + a.crop[1]);
-  blendw = 0.5f * (blendw + 1.0f);
-  if(val < white)
+  return dn_doub_tail(a, ch, crs, crs1, upsm.w, T, blendw, centre_u16(in, x + a.crop[0], y + a.crop[1]));
+}
+
+// one axis of a bilinear texture() lookup: the two mirrored taps and the
+// weight of the second (bilin_h4's arithmetic)
+struct bl_axis { int i0, i1; float f; };
+DEV bl_axis bl_axis_at(float s, int n)
+{
+  const float fs = s - 0.5f, fl = floorf(fs);
+  const int i = (int)fl;
+  bl_axis r;
+  r.i0 = mirror(i, n);
+  r.i1 = mirror(i + 1, n);
+  r.f = fs - fl;
+  return r;
+}
+
+// channel CH of an rgba f16 texel held as two words
+template<int CH> DEV float h4_chan(uint2 u)
+{
+  const uint32_t w = CH < 2 ? u.x : u.y;
+  const __half2 h = *(const __half2 *)&w;
+  return (CH & 1) ? __high2float(h) : __low2float(h);
+}
+
+// bilinear channel CH (x) and, with ALPHA, channel 3 (y) of an rgba f16
+// image: bilin_h4's arithmetic for those channels alone
+template<int CH, bool ALPHA>
+DEV float2 bilin_h4_chan(const dimg &b, const bl_axis &ax, const bl_axis &ay)
+{
+  const __half *p = (const __half *)b.p;
+  const long r0 = (long)ay.i0 * b.pitch, r1 = (long)ay.i1 * b.pitch;
+  const uint2 u00 = *(const uint2 *)(p + 4 * (r0 + ax.i0)), u10 = *(const uint2 *)(p + 4 * (r0 + ax.i1));
+  const uint2 u01 = *(const uint2 *)(p + 4 * (r1 + ax.i0)), u11 = *(const uint2 *)(p + 4 * (r1 + ax.i1));
+  const float a = ax.f, c = ay.f;
+  float2 r;
+  r.x = (h4_chan<CH>(u00) * (1.f - a) + h4_chan<CH>(u10) * a) * (1.f - c)
+      + (h4_chan<CH>(u01) * (1.f - a) + h4_chan<CH>(u11) * a) * c;
+  r.y = ALPHA ? (h4_chan<3>(u00) * (1.f - a) + h4_chan<3>(u10) * a) * (1.f - c)
+              + (h4_chan<3>(u01) * (1.f - a) + h4_chan<3>(u11) * a) * c : 0.0f;
+  return r;
+}
+
+// a Bayer texel of colour CH (0 red, 1 green, 2 blue) from its taps
+template<int CH>
+DEV float dn_doub_bayer_texel(const dimg &crs0, const dimg &crs1b, const dn_args &a, const bl_axis &ux,
+                              const bl_axis &uy, const bl_axis &dx, const bl_axis &dy, float val)
+{
+  const float2 up = bilin_h4_chan<CH, true>(crs0, ux, uy);
+  const float dn = bilin_h4_chan<CH, false>(crs1b, dx, dy).x;
+  return dn_doub_tail(a, CH, up.x, dn, up.y, 0.5f * a.strength * up.y, CH == 1 ? a.luma : 1.0f, val);
+}
+
+// dn_doub_px on Bayer mosaics for the 2x2 block at even (x, y), v in row
+// order: the block's texels share their columns' and rows' taps, and each
+// interpolates only the channel its colour reads and alpha -- per channel
+// the same arithmetic, so the same values bit for bit
+DEV void dn_doub_quad(const dimg &in, const dimg &crs0, const dimg &crs1b, int owd, int oht, const dn_args &a,
+                      int x, int y, float v[4])
+{
+  bl_axis ux[2], uy[2], dx[2], dy[2];
+#pragma unroll
+  for(int k = 0; k < 2; k++)
   {
-    const float wav = (val - crs1) * frcp(fmaxf(sig[0] + sig[2], 1e-8f));
-    const float tt = fminf(1.0f, wav * frcp(fmaxf(2.0f * T, 1e-8f)));
-    float uw = fminf(1.0f, 1.0f * upsm.w);
-    uw *= uw; uw *= uw;    // ^4
-    uw = 1.0f - (1.0f - uw) * a.detail;
-    const float aw = fabsf(wav);
-    val = mixf(val, fmaxf(0.0f, crs + sig[1] * signf_(wav) * mixf(fmaxf(0.0f, aw - T), aw, tt)), uw * blendw);
+    const float sx = ((float)(x + k) + 0.5f) / (float)owd, sy = ((float)(y + k) + 0.5f) / (float)oht;
+    ux[k] = bl_axis_at(sx * (float)crs0.wd, crs0.wd);
+    dx[k] = bl_axis_at(sx * (float)crs1b.wd, crs1b.wd);
+    uy[k] = bl_axis_at(sy * (float)crs0.ht, crs0.ht);
+    dy[k] = bl_axis_at(sy * (float)crs1b.ht, crs1b.ht);
   }
-  return fmaxf(0.0f, (val - black) * inv_range);
+  const int cx = x + a.crop[0], cy = y + a.crop[1];
+  v[0] = dn_doub_bayer_texel<0>(crs0, crs1b, a, ux[0], uy[0], dx[0], dy[0], centre_u16(in, cx, cy));
+  v[1] = dn_doub_bayer_texel<1>(crs0, crs1b, a, ux[1], uy[0], dx[1], dy[0], centre_u16(in, cx + 1, cy));
+  v[2] = dn_doub_bayer_texel<1>(crs0, crs1b, a, ux[0], uy[1], dx[0], dy[1], centre_u16(in, cx, cy + 1));
+  v[3] = dn_doub_bayer_texel<2>(crs0, crs1b, a, ux[1], uy[1], dx[1], dy[1], centre_u16(in, cx + 1, cy + 1));
+}
+
+// store the 2x2 block at even (x, y) of an f16 plane, v in row order: each
+// row as one 4-byte store where both texels lie inside and rows start 4-byte
+// aligned, texel by texel otherwise
+DEV void st_h1_quad(const dimg &o, int x, int y, const float v[4])
+{
+  const bool pair = x + 1 < o.wd && ((o.pitch | (int)((uintptr_t)o.p >> 1)) & 1) == 0;
+#pragma unroll
+  for(int r = 0; r < 2; r++)
+  {
+    if(y + r >= o.ht) break;
+    if(pair) *(__half2 *)((__half *)o.p + (long)(y + r) * o.pitch + x) = __halves2half2(f2h(v[2 * r]), f2h(v[2 * r + 1]));
+    else
+    {
+      st_h1(o, x, y + r, v[2 * r]);
+      if(x + 1 < o.wd) st_h1(o, x + 1, y + r, v[2 * r + 1]);
+    }
+  }
 }
 
 // ==========================================================================
