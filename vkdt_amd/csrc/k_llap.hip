@@ -141,28 +141,29 @@ __global__ void __launch_bounds__(256) k_ll_assemble_small(dimg coarse, dimg l0,
   const float v = ld_h1(l0, x, y, a.num_gamma);
   const int h = gamma_hi_from_v(v, a);
   const float pv0 = ld_h1(l0, x, y, h - 1), pv1 = ld_h1(l0, x, y, h);
-  const float we[5] = { 0.5f, 0.5f, 1.0f, 0.5f, 0.5f }, wo[5] = { 0.0f, 1.0f, 0.5f, 0.5f, 1.0f };
-  const float *wx = (x & 1) ? wo : we, *wy = (y & 1) ? wo : we;
+  const ll_expand_w wx = ll_expand_weights(x & 1), wy = ll_expand_weights(y & 1);
   const float al = clampf((v - a.gam[h - 1]) / (a.gam[h] - a.gam[h - 1]), 0.0f, 1.0f);
-  float rs = 0.0f, e0 = 0.0f, e1 = 0.0f;
+  float sc[5], s0[5], s1[5];
 #pragma unroll
   for(int j = 0; j < 5; j++)
   {
     const int gy = mirror(y / 2 - 2 + j, l1.ht);
-    float rc = 0.0f, r0 = 0.0f, r1 = 0.0f;
+    float c[5], r0[5], r1[5];
 #pragma unroll
     for(int i = 0; i < 5; i++)
     {
       const int gx = mirror(x / 2 - 2 + i, l1.wd);
-      const float c = a.first <= 0 ? ld_h1(coarse, gx, gy) : ld_h1(l1, gx, gy, a.num_gamma);
-      rc += wx[i] * c;
-      r0 += wx[i] * ld_h1(l1, gx, gy, h - 1);
-      r1 += wx[i] * ld_h1(l1, gx, gy, h);
+      c[i] = a.first <= 0 ? ld_h1(coarse, gx, gy) : ld_h1(l1, gx, gy, a.num_gamma);
+      r0[i] = ld_h1(l1, gx, gy, h - 1);
+      r1[i] = ld_h1(l1, gx, gy, h);
     }
-    rs += wy[j] * rc;
-    e0 += wy[j] * r0;
-    e1 += wy[j] * r1;
+    sc[j] = ll_expand5(wx, c[0], c[1], c[2], c[3], c[4]);
+    s0[j] = ll_expand5(wx, r0[0], r0[1], r0[2], r0[3], r0[4]);
+    s1[j] = ll_expand5(wx, r1[0], r1[1], r1[2], r1[3], r1[4]);
   }
+  const float rs = 0.25f * ll_expand5(wy, sc[0], sc[1], sc[2], sc[3], sc[4]);
+  const float e0 = 0.25f * ll_expand5(wy, s0[0], s0[1], s0[2], s0[3], s0[4]);
+  const float e1 = 0.25f * ll_expand5(wy, s1[0], s1[1], s1[2], s1[3], s1[4]);
   const float s = 1.0f / 9.0f;
   const float lv0 = pv0 - e0 * s;
   const float lv1 = pv1 - e1 * s;

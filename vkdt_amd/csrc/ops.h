@@ -1057,6 +1057,28 @@ DEV void ll_reduce_px4(const dimg &in, const dimg &out, int planes, int x0, int 
 #define AS_TY(R) (2 * (R) + 4)         // coarse tile height for a 4R high fine tile
 #define AS_TN(R) (AS_TX * AS_TY(R))
 
+// The expand filter's five taps along one axis: weights 1/2 1/2 1 1/2 1/2 at
+// even texels and 0 1 1/2 1/2 1 at odd ones, summed from zero in tap order
+// (llap.glsl upsample).  Scaling by two commutes with rounding, so twice that
+// sum is exactly (((t0 + t1) + 2 t2) + t3) + t4 at even texels and
+// ((2 t1 + t2) + t3) + 2 t4 at odd ones (an all-zero sum may differ in the
+// sign of zero): ll_expand5 returns it, one multiply, three FMAs (each product
+// by 1 or 2 is exact) and an add instead of five multiplies and five adds.
+struct ll_expand_w { float z0, m1, m2, m4; };
+DEV ll_expand_w ll_expand_weights(int odd)
+{
+  ll_expand_w w;
+  w.z0 = odd ? 0.0f : 1.0f;
+  w.m1 = odd ? 2.0f : 1.0f;
+  w.m2 = odd ? 1.0f : 2.0f;
+  w.m4 = odd ? 2.0f : 1.0f;
+  return w;
+}
+DEV float ll_expand5(const ll_expand_w &w, float t0, float t1, float t2, float t3, float t4)
+{
+  return __builtin_fmaf(t4, w.m4, __builtin_fmaf(t2, w.m2, __builtin_fmaf(t1, w.m1, t0 * w.z0)) + t3);
+}
+
 DEV int wave_min(int v) { for(int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o)); return v; }
 DEV int wave_max(int v) { for(int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o)); return v; }
 
@@ -1118,36 +1140,32 @@ DEV void ll_assemble_tile(const dimg &coarse, const dimg &l1, int owd, int oht, 
       if(i0 + u * S < n) tile[i0 + u * S] = c[u];
   }
   __syncthreads();
-  const float we[5] = { 0.5f, 0.5f, 1.0f, 0.5f, 0.5f }, wo[5] = { 0.0f, 1.0f, 0.5f, 0.5f, 1.0f };
-  const float *wx = (x & 1) ? wo : we;
+  const ll_expand_w wx = ll_expand_weights(x & 1);
   const int lx = x / 2 - cx0 - 2;   // tile column of offset -2
 #pragma unroll
   for(int k = 0; k < R; k++)
   {
     const int y = Y0 + threadIdx.y + 4 * k;
     if(x >= owd || y >= oht) continue;
-    const float *wy = (y & 1) ? wo : we;
+    const ll_expand_w wy = ll_expand_weights(y & 1);
     const int ly = y / 2 - cy0 - 2;
     const int h = hi[k];
     const float al = clampf((v[k] - a.gam[h - 1]) / (a.gam[h] - a.gam[h - 1]), 0.0f, 1.0f);
     const float *tc = tile + np * CN, *t0 = tile + (h - 1 - p0) * CN, *t1 = tile + (h - p0) * CN;
-    float rs = 0.0f, e0 = 0.0f, e1 = 0.0f;
+    // rows: twice the horizontal sums; columns: twice again, so the
+    // reference's weighted sums are a quarter of these, exactly
+    float sc[5], s0[5], s1[5];
 #pragma unroll
     for(int j = 0; j < 5; j++)
     {
-      float rc = 0.0f, r0 = 0.0f, r1 = 0.0f;
       const int o = (ly + j) * AS_TX + lx;
-#pragma unroll
-      for(int i = 0; i < 5; i++)
-      {
-        rc += wx[i] * tc[o + i];
-        r0 += wx[i] * t0[o + i];
-        r1 += wx[i] * t1[o + i];
-      }
-      rs += wy[j] * rc;
-      e0 += wy[j] * r0;
-      e1 += wy[j] * r1;
+      sc[j] = ll_expand5(wx, tc[o], tc[o + 1], tc[o + 2], tc[o + 3], tc[o + 4]);
+      s0[j] = ll_expand5(wx, t0[o], t0[o + 1], t0[o + 2], t0[o + 3], t0[o + 4]);
+      s1[j] = ll_expand5(wx, t1[o], t1[o + 1], t1[o + 2], t1[o + 3], t1[o + 4]);
     }
+    const float rs = 0.25f * ll_expand5(wy, sc[0], sc[1], sc[2], sc[3], sc[4]);
+    const float e0 = 0.25f * ll_expand5(wy, s0[0], s0[1], s0[2], s0[3], s0[4]);
+    const float e1 = 0.25f * ll_expand5(wy, s1[0], s1[1], s1[2], s1[3], s1[4]);
     const float s = 1.0f / 9.0f;
     const float lv0 = pv0[k] - e0 * s;
     const float lv1 = pv1[k] - e1 * s;
