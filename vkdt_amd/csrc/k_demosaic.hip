@@ -63,17 +63,37 @@ DEV int2 cfa_xy()
   return bayer_wave_xy(t.x * BX, t.y * BY);
 }
 
+template<int XT, int U16, int PAR>
+DEV float dm_splat_node(const dimg &in, const dimg &ga, int x, int y)
+{
+  constexpr int R = XT ? 2 : 1;
+  if(__all(x >= R && y >= R && x + R < in.wd && y + R < in.ht))
+    return dm_splat_at<XT, PAR>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y);
+  return dm_splat_px<XT, U16, PAR>(in, ga, x, y);
+}
+
+// Bayer waves are colour-uniform (cfa_xy), so the branch on the site's
+// parity is uniform too, and each wave weights only its colour's taps
 template<int XT, int U16>
 __global__ void __launch_bounds__(256) k_dm_splat(dimg in, dimg ga, dimg out)
 {
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  constexpr int R = XT ? 2 : 1;
-  if(__all(x >= R && y >= R && x + R < in.wd && y + R < in.ht))
-    st_h1(out, x, y, dm_splat_at<XT>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y));
-  else
-    st_h1(out, x, y, dm_splat_px<XT, U16>(in, ga, x, y));
+  float v;
+  if(XT) v = dm_splat_node<XT, U16, -1>(in, ga, x, y);
+  else if((x + y) & 1) v = dm_splat_node<XT, U16, 0>(in, ga, x, y);
+  else v = dm_splat_node<XT, U16, 1>(in, ga, x, y);
+  st_h1(out, x, y, v);
+}
+
+template<int XT, int U16, int PAR>
+DEV float4 dm_fix_node(const dimg &in, const dimg &gr, const dimg &cv, int r, int x, int y)
+{
+  if(__all(x >= r && y >= r && x + r < min(in.wd, gr.wd) && y + r < min(in.ht, gr.ht)))
+    return dm_fix_r<XT, PAR>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, cv, r, x, y,
+                             [&](int px, int py) { return ld_h1(gr, px, py); });
+  return dm_fix_px<XT, U16, PAR>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); });
 }
 
 template<int XT, int U16>
@@ -82,11 +102,11 @@ __global__ void __launch_bounds__(256) k_dm_fix(dimg in, dimg gr, dimg cv, dimg 
   const int2 p = cfa_xy<XT>();
   const int x = p.x, y = p.y;
   if(x >= out.wd || y >= out.ht) return;
-  if(__all(x >= r && y >= r && x + r < min(in.wd, gr.wd) && y + r < min(in.ht, gr.ht)))
-    st_h4(out, x, y, dm_fix_r<XT>([&](int px, int py) { return ld_mt<U16>(in, px, py); }, cv, r, x, y,
-                                  [&](int px, int py) { return ld_h1(gr, px, py); }));
-  else
-    st_h4(out, x, y, dm_fix_px<XT, U16>(in, cv, r, x, y, [&](int px, int py) { return centre_h1(gr, px, py); }));
+  float4 v;
+  if(XT) v = dm_fix_node<XT, U16, -1>(in, gr, cv, r, x, y);
+  else if((x + y) & 1) v = dm_fix_node<XT, U16, 1>(in, gr, cv, r, x, y);
+  else v = dm_fix_node<XT, U16, 0>(in, gr, cv, r, x, y);
+  st_h4(out, x, y, v);
 }
 
 // ---- halfsize (halfsize.comp:19-50): one rgba texel per CFA block --------

@@ -674,7 +674,10 @@ DEV float gauss_w(float4 cov, float r0, float r1, int i, int j, float lo)
 // products whose sign flips exactly and are then squared), so each
 // exponential is taken once, for the half plane j > 0 or (j = 0, i > 0),
 // and mirrored; every tap then reads its weight whatever the CFA colour.
-template<int R>
+// With PAR >= 0 only the offsets whose i + j has that parity are weighted
+// (a Bayer site's taps of one colour class all share it; (i,j) and (-i,-j)
+// do too), the rest are left unset and must not be read.
+template<int R, int PAR = -1>
 struct sym_weights
 {
   float e[2 * R + 1][2 * R + 1];
@@ -686,11 +689,12 @@ struct sym_weights
       for(int i = -R; i <= R; i++)
       {
         if(j == 0 && i <= 0) continue;
+        if(PAR >= 0 && ((i + j) & 1) != PAR) continue;
         const float w = gauss_w(cov, r0, r1, i, j, lo);
         e[j + R][i + R] = w;
         e[R - j][R - i] = w;
       }
-    e[R][R] = gauss_w(cov, r0, r1, 0, 0, lo);
+    if(PAR <= 0) e[R][R] = gauss_w(cov, r0, r1, 0, 0, lo);
   }
   __device__ __forceinline__ float operator()(int i, int j) const { return e[j + R][i + R]; }
 };
@@ -699,19 +703,22 @@ struct sym_weights
 // stored channel, so only green sites are weighted (same sum and order).
 //   mos(px, py): the mosaic at (px, py), 0 outside (texelFetch);
 //   W, H its extent
-template<int XT, typename MosAt>
+//   PAR: -1, or on Bayer mosaics the parity of i + j of the site's green taps
+//        (1 - ((x + y) & 1)): then only those taps' weights are evaluated
+template<int XT, int PAR = -1, typename MosAt>
 DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
 {
   const float4 cov = XT ? fetch_h4(ga, x / 3, y / 3) : fetch_h4(ga, (x + 1) / 2, (y + 1) / 2);
   const float r0 = -0.5f * frcp(clampf(cov.x, 0.01f, 25.0f)), r1 = -0.5f * frcp(clampf(cov.y, 0.01f, 25.0f));
   constexpr int r = XT ? 2 : 1;
-  const sym_weights<r> tapw(cov, r0, r1, 1e-4f);
+  const sym_weights<r, PAR> tapw(cov, r0, r1, 1e-4f);
   float g = 0.0f, w = 0.0f;
 #pragma unroll
   for(int j = -r; j <= r; j++)
 #pragma unroll
     for(int i = -r; i <= r; i++)
     {
+      if(PAR >= 0 && ((i + j) & 1) != PAR) continue;   // not green
       const bool green = XT ? xtrans_colour(x + i + 6, y + j + 6) == 1 : (((x + i + y + j) & 1) == 1);
       if(!green) continue;
       int px = x + i, py = y + j;
@@ -727,17 +734,19 @@ DEV float dm_splat_at(MosAt mos, int W, int H, const dimg &ga, int x, int y)
   return g * frcp(fmaxf(1e-8f, w));
 }
 
-template<int XT, int U16 = 0>
+template<int XT, int U16 = 0, int PAR = -1>
 DEV float dm_splat_px(const dimg &in, const dimg &ga, int x, int y)
 {
-  return dm_splat_at<XT>([&](int px, int py) { return fetch_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y);
+  return dm_splat_at<XT, PAR>([&](int px, int py) { return fetch_mt<U16>(in, px, py); }, in.wd, in.ht, ga, x, y);
 }
 
 // fix.comp:21-135: rgb at (x,y) from the mosaic, the green plane (through
 // `green(px,py)`, mirrored coordinates) and the block covariance.  Green
 // sites copy the centre green, red and blue sites are ratio-weighted.
 //   mos(px, py): the mosaic at mirrored (px, py); R the fix radius
-template<int XT, int R, typename MosAt, typename GreenAt>
+//   PAR: -1, or on Bayer mosaics the parity of i + j of the site's red and
+//        blue taps ((x + y) & 1): then only those taps' weights are evaluated
+template<int XT, int R, int PAR = -1, typename MosAt, typename GreenAt>
 DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int x, int y, GreenAt green)
 {
   float sr = 0.0f, sb = 0.0f, wr = 0.0f, wb = 0.0f;
@@ -746,13 +755,14 @@ DEV float4 dm_fix_at(MosAt mos, const dimg &cv, int x, int y, GreenAt green)
   const float hi = XT ? 10.0f : 49.0f, sc = XT ? 3.0f : 2.0f;
   const float r0 = -0.5f * frcp(sc * clampf(cov.x, 1.0f, hi)), r1 = -0.5f * frcp(sc * clampf(cov.y, 1.0f, hi));
   const float gcn = 1e-4f + gc;
-  const sym_weights<R> tapw(cov, r0, r1, 1e-3f);
+  const sym_weights<R, PAR> tapw(cov, r0, r1, 1e-3f);
   bool any_green = false;
 #pragma unroll
   for(int j = -R; j <= R; j++)
 #pragma unroll
     for(int i = -R; i <= R; i++)
     {
+      if(PAR >= 0 && ((i + j) & 1) != PAR) { any_green = true; continue; }   // a green tap
       const int px = x + i, py = y + j;   // may be negative: truncating division as in GLSL
       int k;
       if(XT)
@@ -811,18 +821,18 @@ DEV float4 dm_fix_loop(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt g
 
 // the fix radius r is uniform per image (1..2 Bayer, 2..3 X-Trans); the
 // default radius (demosaic colour 0) runs unrolled with symmetric weights
-template<int XT, typename MosAt, typename GreenAt>
+template<int XT, int PAR = -1, typename MosAt, typename GreenAt>
 DEV float4 dm_fix_r(MosAt mos, const dimg &cv, int r, int x, int y, GreenAt green)
 {
   constexpr int R0 = XT ? 2 : 1;
-  if(r == R0) return dm_fix_at<XT, R0>(mos, cv, x, y, green);
+  if(r == R0) return dm_fix_at<XT, R0, PAR>(mos, cv, x, y, green);
   return dm_fix_loop<XT>(mos, cv, r, x, y, green);
 }
 
-template<int XT, int U16 = 0, typename GreenAt>
+template<int XT, int U16 = 0, int PAR = -1, typename GreenAt>
 DEV float4 dm_fix_px(const dimg &in, const dimg &cv, int r, int x, int y, GreenAt green)
 {
-  return dm_fix_r<XT>([&](int px, int py) { return centre_mt<U16>(in, px, py); }, cv, r, x, y, green);
+  return dm_fix_r<XT, PAR>([&](int px, int py) { return centre_mt<U16>(in, px, py); }, cv, r, x, y, green);
 }
 
 // ==========================================================================
