@@ -464,7 +464,9 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
       {
         if(!c && e >= ht2) { c = 1; wf = tile_walk(e - ht2, BX * BY, hw); }
         const int ty = wf.ty, tx = 2 * wf.tx + ((ty + c) & 1);
-        gt[ty * gw + tx] = r16(dm_splat_at<XT>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
+        // c is the site's parity: only its green taps' weights are evaluated
+        gt[ty * gw + tx] = r16(c ? dm_splat_at<XT, 0>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty)
+                                 : dm_splat_at<XT, 1>(mosl, W, H, cov, X0 - r + tx, Y0 - r + ty));
       }
     }
     __syncthreads();
@@ -473,7 +475,9 @@ DEV void develop_tile(const dimg &mos, const dimg &cov, const dimg &film, const 
     { // Bayer: each wave one CFA colour (bayer_wave_xy) in each 64x4 band
       const int2 q = XT ? make_int2(x, Y0 + threadIdx.y + BY * k) : bayer_wave_xy(X0, Y0 + BY * k);
       const int xq = q.x, y = q.y;
-      const float4 f = dm_fix_r<XT>(mosl, cov, r, xq, y, green);
+      const float4 f = XT ? dm_fix_r<XT>(mosl, cov, r, xq, y, green)
+                          : (((xq + y) & 1) ? dm_fix_r<XT, 1>(mosl, cov, r, xq, y, green)
+                                            : dm_fix_r<XT, 0>(mosl, cov, r, xq, y, green));
       const float4 c = r16(co_px(r16(f), co));
       st_h4(film, xq, y, fc_px_basic(c, fc));
     }
