@@ -880,6 +880,7 @@ static int build_image(bt_img &b, const vkdt_develop_image_t &d, const dims_t &m
     dn_level_scale(b.dn_down[l]);
   }
   dn_common(b.dn_asm, d);
+  dn_asm_scales(b.dn_asm);
   dn_common(b.dn_doub, d);
   for(int k = 0; k < 4; k++) b.dn_doub.crop[k] = d.crop[k];
   if(const char *err = co_args_init(b.co, d.co_params))

@@ -224,6 +224,7 @@ int launch_denoise_assemble(const node_call &c)
   a.noise_b = pf[17];
   a.filters = ((const uint32_t *)c.pc)[18];
   read_params(a, c);
+  dn_asm_scales(a);
   asm_mats m;
   asm_mats_init(m);
   hipLaunchKernelGGL(k_dn_assemble, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,

@@ -28,6 +28,8 @@ struct dn_args
   float    strength, luma, detail, edges[4];
   float    edge_gain[3];     // exp2(12*edges.rgb + edges.a) of noise_sigma
   float    level_scale;      // 0.7^level / block factor (down.comp:85)
+  float    asm_bl[4];        // assemble's level scales 0.7^(l+1) / block factor (dn_asm_scales)
+  float    asm_rb[4];        // and their reciprocals
 };
 
 struct asm_mats { float rgb_to_yuv[9], yuv_to_rgb[9]; };
@@ -65,6 +67,20 @@ static inline void dn_level_scale(dn_args &a)
 
 // host: assemble.comp's colour matrices (matrices.h: rec709_to_yuv *
 // rec2020_to_rec709 and rec709_to_rec2020 * yuv_to_rec709), products in double
+// assemble.comp's level scales bl = 0.7^(l+1) / block (block factor by CFA:
+// 1, sqrt 2, sqrt 5) and 1 / bl: the same IEEE float divisions the kernel
+// would take for every texel, taken once on the host
+static inline void dn_asm_scales(dn_args &a)
+{
+  const float block = a.filters == 0u ? 1.0f : (a.filters == 9u ? 2.23607f : 1.414213f);
+  const float c[4] = { 0.7000f, 0.4900f, 0.3430f, 0.2401f };
+  for(int l = 0; l < 4; l++)
+  {
+    a.asm_bl[l] = c[l] / block;
+    a.asm_rb[l] = 1.0f / a.asm_bl[l];
+  }
+}
+
 static inline void asm_mats_init(asm_mats &m)
 {
   static const double m709yuv[9]  = { 0.299, 0.587, 0.114, -0.147, -0.289, 0.436, 0.615, -0.515, -0.100 };
@@ -364,8 +380,6 @@ DEV float4 dn_assemble_px(const dimg &s0, const dimg &s1, const dimg &s2, const 
                              { d3.x, d3.y, d3.z }, { d4.x, d4.y, d4.z } };
   float sig[3];
   noise_sigma(sig, a, a.black[1], a.inv_range[1], fmaxf(down[2][0], 0.0f));
-  const float block = a.filters == 0u ? 1.0f : (a.filters == 9u ? 2.23607f : 1.414213f);
-  const float bl[4] = { 0.7000f / block, 0.4900f / block, 0.3430f / block, 0.2401f / block };
   float rs[3];
 #pragma unroll
   for(int k = 0; k < 3; k++) rs[k] = frcp(sig[k]);
@@ -373,7 +387,7 @@ DEV float4 dn_assemble_px(const dimg &s0, const dimg &s1, const dimg &s2, const 
 #pragma unroll
   for(int l = 0; l < 4; l++)
   {
-    const float rb = 1.0f / bl[l];
+    const float rb = a.asm_rb[l];
 #pragma unroll
     for(int k = 0; k < 3; k++) d[l][k] = (down[l][k] - down[l + 1][k]) * (rs[k] * rb);
     len[l] = fsqrt(d[l][0] * d[l][0] + d[l][1] * d[l][1] + d[l][2] * d[l][2]);
@@ -396,7 +410,7 @@ DEV float4 dn_assemble_px(const dimg &s0, const dimg &s1, const dimg &s2, const 
     {
       const float ad = fabsf(d[l][k]);
       const float tt = fminf(1.0f, ad * rt);
-      d4v[k] += sig[k] * bl[l] * signf_(d[l][k]) * mixf(fmaxf(ad - thrs, 0.0f), ad, tt);
+      d4v[k] += sig[k] * a.asm_bl[l] * signf_(d[l][k]) * mixf(fmaxf(ad - thrs, 0.0f), ad, tt);
     }
   }
   float t0[3], t1[3];
