@@ -49,18 +49,32 @@ DEV void st_h4x2(const dimg &b, int x, int y, float4 p, float4 q)
   *(uint4 *)((__half *)b.p + 4 * ((long)y * b.pitch + x)) = u;
 }
 
+// a 2x2 block of texels per thread, each row's pair in one 16-byte load and
+// store: both rows' loads are in flight before either pair is converted
 template<typename Px>
-DEV void px_pair(const dimg &in, const dimg &out, Px px)
+DEV void px_pair2(const dimg &in, const dimg &out, Px px)
 {
-  const int x = 2 * tid_x(), y = tid_y();
+  const int x = 2 * tid_x(), y = 2 * tid_y();
   if(x >= out.wd || y >= out.ht) return;
-  if(x + 1 < out.wd)
+  if(x + 1 < out.wd && y + 1 < out.ht)
   {
-    float4 p, q;
-    ld_h4x2(in, x, y, p, q);
-    st_h4x2(out, x, y, px(p), px(q));
+    float4 p0, q0, p1, q1;
+    ld_h4x2(in, x, y, p0, q0);
+    ld_h4x2(in, x, y + 1, p1, q1);
+    st_h4x2(out, x, y, px(p0), px(q0));
+    st_h4x2(out, x, y + 1, px(p1), px(q1));
+    return;
   }
-  else st_h4(out, x, y, px(ld_h4(in, x, y)));
+  for(int r = 0; r < 2 && y + r < out.ht; r++)
+  {
+    if(x + 1 < out.wd)
+    {
+      float4 p, q;
+      ld_h4x2(in, x, y + r, p, q);
+      st_h4x2(out, x, y + r, px(p), px(q));
+    }
+    else st_h4(out, x, y + r, px(ld_h4(in, x, y + r)));
+  }
 }
 
 static bool pairs_ok(const vkdt_buf_t &i, const vkdt_buf_t &o)
@@ -71,13 +85,13 @@ static bool pairs_ok(const vkdt_buf_t &i, const vkdt_buf_t &o)
 
 __global__ void __launch_bounds__(256) k_co_main2(dimg in, dimg out, co_args a)
 {
-  px_pair(in, out, [&](float4 t) { return co_px(t, a); });
+  px_pair2(in, out, [&](float4 t) { return co_px(t, a); });
 }
 
 template<bool COLOUR>
 __global__ void __launch_bounds__(256) k_fc_main2(dimg in, dimg out, fc_args a)
 {
-  px_pair(in, out, [&](float4 t) { return COLOUR ? fc_px(t, a) : fc_px_basic(t, a); });
+  px_pair2(in, out, [&](float4 t) { return COLOUR ? fc_px(t, a) : fc_px_basic(t, a); });
 }
 
 // ---- host launchers --------------------------------------------------------
@@ -96,7 +110,7 @@ int launch_colour_main(const node_call &c)
   co_args a;
   if(const char *err = co_args_init(a, c.params)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "colour/main: %s", err);
   if(pairs_ok(c.conn[0], c.conn[1]) && c.wd == c.conn[1].wd && c.ht == c.conn[1].ht)
-    hipLaunchKernelGGL(k_co_main2, grid2d((c.wd + 1) / 2, c.ht, 1), dim3(BX, BY), 0, c.stream,
+    hipLaunchKernelGGL(k_co_main2, grid2d((c.wd + 1) / 2, (c.ht + 1) / 2, 1), dim3(BX, BY), 0, c.stream,
                        to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
   else
     hipLaunchKernelGGL(k_co_main, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
@@ -113,7 +127,7 @@ int launch_filmcurv_main(const node_call &c)
   if(const char *err = fc_args_init(a, c.params, c.params_size, false)) return vkdt_set_error(VKDT_HIP_INCOMPLETE, "filmcurv/main: %s", err);
   const bool basic = a.mode == 1 || a.mode == 3;
   if(pairs_ok(c.conn[0], c.conn[1]) && c.wd == c.conn[1].wd && c.ht == c.conn[1].ht)
-    hipLaunchKernelGGL(basic ? k_fc_main2<false> : k_fc_main2<true>, grid2d((c.wd + 1) / 2, c.ht, 1), dim3(BX, BY),
+    hipLaunchKernelGGL(basic ? k_fc_main2<false> : k_fc_main2<true>, grid2d((c.wd + 1) / 2, (c.ht + 1) / 2, 1), dim3(BX, BY),
                        0, c.stream, to_dimg(c.conn[0]), to_dimg(c.conn[1]), a);
   else
     hipLaunchKernelGGL(basic ? k_fc_main<false> : k_fc_main<true>, grid2d(c.wd, c.ht, 1), dim3(BX, BY), 0, c.stream,
